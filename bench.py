@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark: bit-exact 4Mi x 4Mi-limb multiply on one MI355X.
+
+A step is one full multiply (limb -> residue split, forward NTTs, pointwise
+product, inverse NTT, Garner CRT, carry scan) of two 2^22-limb operands that
+are already resident in HBM (xorshift64 seed 42, A then B from one stream).
+value = ns per input limb = step time / 2^22 (the reference's convention,
+bench/nt_sweep.c:33), lower is better.
+
+Also reported, from the same process:
+  roofline      the dominant kernel: algorithmic bytes per launch / its
+                average duration (HIP events on its launch stream), vs the
+                8 TB/s HBM peak;
+  roofline_pipeline  the whole multiply's algorithmic bytes Z*(16+32K), K=6
+                (the accounting of the reference-size model) / step time;
+  cpu_baseline  the reference's nt_mul compiled from source (oracle/_ref,
+                "reference") when present and the host CPU has its ISA, else
+                the oracle's C restatement ("port"), on the same operands.
+"""
+import argparse
+import ctypes
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "ns/limb at 4Mi×4Mi-limb mul; batched vs per-call muls/s, bit-exact"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_flags():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("flags"):
+                    return set(ln.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return set()
+
+
+def cpu_baseline(a, b, budget_s):
+    import numpy as np
+    n = len(a)
+    need = {"avx512f", "avx512bw", "avx512vl", "avx512dq", "avx512ifma", "avx512vbmi",
+            "avx512_vbmi2", "avx512_vpopcntdq", "avx512_bitalg", "bmi2", "adx"}
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libsbref.so")
+    u64p = np.ctypeslib.ndpointer(np.uint64, flags="C")
+    r = np.zeros(2 * n, np.uint64)
+    if os.path.exists(ref_so) and need <= cpu_flags():
+        L = ctypes.CDLL(ref_so)
+        fn = L.sbref_nt_mul
+        kind, cores = "reference", 1
+        what = "reference nt_mul (include/ntmul.h, p50 engine at this size) compiled from source"
+    else:
+        L = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+        fn = L.orc_nt_mul
+        kind, cores = "port", os.cpu_count() or 1
+        what = "oracle C restatement of the p30 math (six primes, OpenMP over primes)"
+    fn.argtypes = [u64p, u64p, ctypes.c_ssize_t, u64p, ctypes.c_ssize_t]
+    fn.restype = ctypes.c_int
+    times = []
+    t_start = time.time()
+    while True:
+        t0 = time.perf_counter()
+        ok = fn(r, a, n, b, n)
+        times.append(time.perf_counter() - t0)
+        if ok != 1:
+            raise RuntimeError("cpu baseline rejected the shape")
+        if time.time() - t_start > budget_s or len(times) >= 50:
+            break
+    best = min(times)
+    mean = sum(times) / len(times)
+    return {"value": round(best * 1e9 / n, 3), "unit": "ns/limb", "cores": cores, "kind": kind,
+            "sample": "%s; %d full 4Mi x 4Mi multiplies on the same operands, best of %d "
+                      "(mean %.3f ns/limb)" % (what, len(times), len(times), mean * 1e9 / n)}, r
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-budget", type=float, default=12.0,
+                    help="seconds of CPU work for the cpu_baseline sample (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(the multi-GPU layout is described in DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import simd_bigint_amd as sba
+    from simd_bigint_amd.workloads import xorshift_fill
+
+    L = sba.lib()
+    L.sba_gpu_profile_enable.argtypes = [ctypes.c_int]
+    L.sba_gpu_profile_get.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
+                                      ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_uint64)]
+    n = 1 << 22
+    st = [42]
+    a = xorshift_fill(n, st)
+    b = xorshift_fill(n, st)
+    log("bench: operands generated")
+    dev = torch.device("cuda:0")
+    ta = torch.from_numpy(a.view(np.int64)).to(dev)
+    tb = torch.from_numpy(b.view(np.int64)).to(dev)
+    tr = torch.empty(2 * n, dtype=torch.int64, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+    sp = ctypes.c_void_p(int(stream.cuda_stream))
+
+    def step():
+        rc = L.sba_gpu_mul_dev(tr.data_ptr(), ta.data_ptr(), n, tb.data_ptr(), n, sp)
+        if rc != 1:
+            raise RuntimeError("GPU multiply returned %d" % rc)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    prod = tr.cpu().numpy().view(np.uint64)
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
+    want = [e for e in gold["large_mul"] if e["name"] == "metric_4Mi"][0]["sha256"]
+    bit_exact = hashlib.sha256(prod.tobytes()).hexdigest() == want
+    log("bench: warmup done, bit_exact =", bit_exact)
+
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    log("bench: %.4f ms per multiply" % ms)
+
+    # per-kernel timing pass (events around each launch on the same stream)
+    L.sba_gpu_profile_reset()
+    L.sba_gpu_profile_enable(1)
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    L.sba_gpu_profile_collect()
+    L.sba_gpu_profile_enable(0)
+    kernels = {}
+    i = 0
+    while True:
+        nm = ctypes.c_char_p()
+        kms = ctypes.c_double()
+        kb = ctypes.c_double()
+        kc = ctypes.c_uint64()
+        if not L.sba_gpu_profile_get(i, ctypes.byref(nm), ctypes.byref(kms), ctypes.byref(kb),
+                                     ctypes.byref(kc)):
+            break
+        i += 1
+        cnt = max(kc.value, 1)
+        avg_ms = kms.value / cnt
+        kernels[nm.value.decode()] = {
+            "avg_us": round(avg_ms * 1e3, 2), "launches_per_step": round(cnt / args.steps, 2),
+            "alg_bytes_per_launch": int(kb.value / cnt),
+            "achieved_GBs": round(kb.value / cnt / (avg_ms * 1e-3) / 1e9, 1),
+            "share": 0.0}
+    tot = sum(k["avg_us"] * k["launches_per_step"] for k in kernels.values()) or 1.0
+    for k in kernels.values():
+        k["share"] = round(k["avg_us"] * k["launches_per_step"] / tot, 3)
+    dom_name = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches_per_step"])
+    dom = kernels[dom_name]
+
+    Z = 2 * n
+    K_model = 6
+    b_alg = Z * (16 + 32 * K_model)
+    pipe_gbs = b_alg / (ms * 1e-3) / 1e9
+
+    cpu = None
+    if args.cpu_budget > 0:
+        log("bench: cpu baseline")
+        cpu, rcpu = cpu_baseline(a, b, args.cpu_budget)
+        if not (rcpu == prod).all():
+            raise RuntimeError("cpu baseline product differs from the GPU product")
+
+    out = {
+        "metric": METRIC,
+        "value": round(ms * 1e6 / n, 5),
+        "unit": "ns/limb",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (xorshift64 seed 42 operands, resident in HBM)",
+        "config": {"workload": "mul 4Mi x 4Mi u64 limbs (an = bn = 2^22), product 2^23 limbs",
+                   "engine": "5 primes < 2^31, N = 2^23 NTT, Garner, carry scan",
+                   "bit_exact": bool(bit_exact)},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": dom["achieved_GBs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(dom["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": None},
+        "roofline_pipeline": {"bound": "hbm", "alg_bytes": b_alg, "achieved": round(pipe_gbs, 1),
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(pipe_gbs / HBM_PEAK_GBS, 4)},
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    if not bit_exact:
+        sys.exit(1)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,138 @@
+/* simd_bigint_amd.h -- C ABI of the MI355X (gfx950) large-operand multiply.
+ *
+ * Drop-in for the reference's mpn-style multiply path: the entries below
+ * carry the reference's names, argument meaning, length conventions and
+ * return codes. Operands are little-endian u64 limb arrays passed as
+ * pointer + length. All arithmetic on the GPU is exact integer work (a
+ * five-prime 31-bit NTT, Garner CRT and a carry-lookahead scan); results
+ * are bit-identical to the reference's.
+ *
+ * Host-pointer entries are synchronous: they copy the operands to the
+ * device, multiply there, copy the product back and return. Device-pointer
+ * entries (sba_gpu_*) take device buffers and a HIP stream and only
+ * enqueue work.
+ *
+ * Citations are files of the reference repository (iamstupid/simd_bigint).
+ */
+#ifndef SIMD_BIGINT_AMD_H
+#define SIMD_BIGINT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes of the int-valued entries */
+#define SBA_OK 1             /* product written */
+#define SBA_BAND 0           /* outside the engine's band (reference: return 0) */
+#define SBA_ERR_HIP (-1)     /* a HIP runtime call failed (message on stderr) */
+#define SBA_ERR_WORKSPACE (-2) /* caller workspace missing or too small */
+#define SBA_ERR_NODEV (-3)   /* no HIP device: the library never computes on the CPU */
+
+/* The reference's scratch arena type (include/scratch.h:248-256). The _r
+ * entries accept the caller's arena for signature compatibility; device
+ * workspace comes from the library's per-device grow-only pool, the same
+ * no-per-call-allocation rule the arena enforces. May be NULL. */
+struct scratch;
+
+/* ---------------------------------------------------------------------- */
+/* NTT dispatcher: replaces include/ntmul.h:47-79.                         */
+/* rp[0 .. an+bn) = {ap,an} * {bp,bn}. Returns 1, or 0 only beyond the     */
+/* band (an <= 0, bn <= 0, min(an,bn) > sba_gpu_min_operand_cap(), or      */
+/* an+bn-1 > 2^sba_gpu_max_lg()). rp must not overlap the inputs.          */
+/* ---------------------------------------------------------------------- */
+int nt_mul(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, const uint64_t *bp,
+           ptrdiff_t bn);                                   /* ntmul.h:71-74 */
+int nt_sqr(uint64_t *rp, const uint64_t *ap, ptrdiff_t an); /* ntmul.h:76-78 */
+int nt_mul_r(uint64_t *rp, const uint64_t *ap, ptrdiff_t an,
+             const uint64_t *bp, ptrdiff_t bn,
+             struct scratch *sc);                           /* ntmul.h:47-58 */
+int nt_sqr_r(uint64_t *rp, const uint64_t *ap, ptrdiff_t an,
+             struct scratch *sc);                           /* ntmul.h:60-69 */
+
+/* Engine entries with the reference engines' bands (include/p30.h:1847,
+ * 1940,1950,2018 and include/p50.h:1020,1096,1106,1176). Both run the same
+ * GPU engine; p30_* additionally return 0 outside p30's band
+ * (min(an,bn) > 2,344,425 or ceil((an+bn-1)/16) > 2^18, p30.h:1851-1858). */
+int p30_mul(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, const uint64_t *bp,
+            ptrdiff_t bn);
+int p30_sqr(uint64_t *rp, const uint64_t *ap, ptrdiff_t an);
+int p30_mul_r(uint64_t *rp, const uint64_t *ap, ptrdiff_t an,
+              const uint64_t *bp, ptrdiff_t bn, struct scratch *sc);
+int p30_sqr_r(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, struct scratch *sc);
+int p50_mul(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, const uint64_t *bp,
+            ptrdiff_t bn);
+int p50_sqr(uint64_t *rp, const uint64_t *ap, ptrdiff_t an);
+int p50_mul_r(uint64_t *rp, const uint64_t *ap, ptrdiff_t an,
+              const uint64_t *bp, ptrdiff_t bn, struct scratch *sc);
+int p50_sqr_r(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, struct scratch *sc);
+
+/* ---------------------------------------------------------------------- */
+/* Public multiply with threshold dispatch: replaces include/u64cvt.h:94-  */
+/* 132. rp[0 .. an+bn) = product; operands in either order; never writes   */
+/* past an+bn; zero operands give zero. Sizes below the GPU crossover run  */
+/* the library's own CPU basecase / Karatsuba / Toom-3 (the reference's    */
+/* ladder, include/mul.h:764-795); above it, the GPU NTT.                  */
+/* ---------------------------------------------------------------------- */
+void simd_mpn_mul(uint64_t *rp, const uint64_t *ap, uint64_t an,
+                  const uint64_t *bp, uint64_t bn);
+
+/* ---------------------------------------------------------------------- */
+/* Division: replaces include/divrem.h:218-221 and include/dc_div.h:154-   */
+/* 157. Q = floor(N/D) into qp (>= nn64-dn64+1 limbs), R = N mod D into rp */
+/* (>= dn64 limbs), trimmed lengths (>= 1) into *qn64_out / *rn64_out.     */
+/* dp[dn64-1] != 0. Newton reciprocal on the GPU multiply.                 */
+/* ---------------------------------------------------------------------- */
+void divrem_u64(uint64_t *qp, uint64_t *rp, const uint64_t *np, uint64_t nn64,
+                const uint64_t *dp, uint64_t dn64, uint64_t *qn64_out,
+                uint64_t *rn64_out);
+void divrem_u64_dc(uint64_t *qp, uint64_t *rp, const uint64_t *np,
+                   uint64_t nn64, const uint64_t *dp, uint64_t dn64,
+                   uint64_t *qn64_out, uint64_t *rn64_out);
+
+/* ---------------------------------------------------------------------- */
+/* Device-resident entries (no counterpart in the reference, which has no  */
+/* device): device pointers, a HIP stream (hipStream_t passed as void*,    */
+/* NULL = default stream). Enqueue only. Return SBA_OK / SBA_BAND / < 0.   */
+/* ---------------------------------------------------------------------- */
+int sba_gpu_mul_dev(uint64_t *rp, const uint64_t *ap, int64_t an,
+                    const uint64_t *bp, int64_t bn, void *stream);
+int sba_gpu_sqr_dev(uint64_t *rp, const uint64_t *ap, int64_t an, void *stream);
+/* same with a caller-provided device workspace of ws_bytes >=
+ * sba_gpu_workspace_bytes(an, bn) bytes */
+int sba_gpu_mul_dev_ws(uint64_t *rp, const uint64_t *ap, int64_t an,
+                       const uint64_t *bp, int64_t bn, void *ws, size_t ws_bytes,
+                       void *stream);
+int sba_gpu_sqr_dev_ws(uint64_t *rp, const uint64_t *ap, int64_t an, void *ws,
+                       size_t ws_bytes, void *stream);
+size_t sba_gpu_workspace_bytes(int64_t an, int64_t bn);
+uint64_t sba_gpu_min_operand_cap(void);
+int sba_gpu_max_lg(void);
+int sba_gpu_device_count(void);
+
+/* ---------------------------------------------------------------------- */
+/* Tuning and test hooks (no counterpart in the reference, whose tier      */
+/* thresholds are compile-time macros, include/u52-mparam.h:1-19).         */
+/* ---------------------------------------------------------------------- */
+/* simd_mpn_mul and the division send products with an+bn >= this many
+ * limbs to the GPU, smaller ones to the CPU Karatsuba. */
+void sba_set_gpu_threshold(uint64_t min_total_limbs);
+uint64_t sba_get_gpu_threshold(void);
+/* per-kernel timing of the GPU path (HIP events on the launch stream):
+ * enable, run, synchronize, collect; then read per-kernel totals (name,
+ * milliseconds, algorithmic bytes, launches) by index until it returns 0. */
+void sba_gpu_profile_enable(int on);
+int sba_gpu_profile_collect(void);
+void sba_gpu_profile_reset(void);
+int sba_gpu_profile_get(int idx, const char **name, double *ms, double *bytes,
+                        uint64_t *count);
+/* the CPU tier alone (basecase + Karatsuba), either operand order */
+void sba_cpu_mul(uint64_t *rp, const uint64_t *ap, uint64_t an,
+                 const uint64_t *bp, uint64_t bn);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

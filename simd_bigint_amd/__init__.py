@@ -1,0 +1,190 @@
+"""simd_bigint_amd -- MI355X (gfx950) large-operand integer multiply.
+
+Python mirror of the reference's mpn-style C entries over the C ABI of
+``lib/libsimd_bigint_amd.so`` (declared in ``include/simd_bigint_amd.h``).
+Operands are numpy ``uint64`` limb arrays, little-endian, as in the
+reference (``include/ntmul.h:47-79``, ``include/u64cvt.h:94-132``,
+``include/divrem.h:218-221``). Device-resident entries take torch CUDA
+(HIP) tensors of dtype ``torch.uint64`` (or int64 viewed as limbs); torch is
+plumbing for device memory and streams only.
+
+The library has no CPU fallback for the GPU path: if the shared library is
+missing, importing the loader raises; if no HIP device is present, the GPU
+entries abort with a message.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libsimd_bigint_amd.so")
+
+SBA_OK = 1
+SBA_BAND = 0
+
+_lib: Optional[ctypes.CDLL] = None
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+def _arr(a: np.ndarray):
+    return a.ctypes.data_as(_u64p)
+
+
+def lib() -> ctypes.CDLL:
+    """Load the shared library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"simd_bigint_amd: {LIB_PATH} is missing; build it first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    L = ctypes.CDLL(LIB_PATH)
+    sp = ctypes.c_ssize_t
+    u64 = ctypes.c_uint64
+    vp = ctypes.c_void_p
+    for name in ("nt_mul", "p30_mul", "p50_mul"):
+        f = getattr(L, name)
+        f.argtypes = [_u64p, _u64p, sp, _u64p, sp]
+        f.restype = ctypes.c_int
+    for name in ("nt_sqr", "p30_sqr", "p50_sqr"):
+        f = getattr(L, name)
+        f.argtypes = [_u64p, _u64p, sp]
+        f.restype = ctypes.c_int
+    L.simd_mpn_mul.argtypes = [_u64p, _u64p, u64, _u64p, u64]
+    L.simd_mpn_mul.restype = None
+    L.sba_cpu_mul.argtypes = [_u64p, _u64p, u64, _u64p, u64]
+    L.sba_cpu_mul.restype = None
+    for name in ("divrem_u64", "divrem_u64_dc"):
+        f = getattr(L, name)
+        f.argtypes = [_u64p, _u64p, _u64p, u64, _u64p, u64, _u64p, _u64p]
+        f.restype = None
+    L.sba_gpu_mul_dev.argtypes = [vp, vp, ctypes.c_int64, vp, ctypes.c_int64, vp]
+    L.sba_gpu_mul_dev.restype = ctypes.c_int
+    L.sba_gpu_sqr_dev.argtypes = [vp, vp, ctypes.c_int64, vp]
+    L.sba_gpu_sqr_dev.restype = ctypes.c_int
+    L.sba_gpu_mul_dev_ws.argtypes = [vp, vp, ctypes.c_int64, vp, ctypes.c_int64, vp,
+                                     ctypes.c_size_t, vp]
+    L.sba_gpu_mul_dev_ws.restype = ctypes.c_int
+    L.sba_gpu_sqr_dev_ws.argtypes = [vp, vp, ctypes.c_int64, vp, ctypes.c_size_t, vp]
+    L.sba_gpu_sqr_dev_ws.restype = ctypes.c_int
+    L.sba_gpu_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int64]
+    L.sba_gpu_workspace_bytes.restype = ctypes.c_size_t
+    L.sba_gpu_min_operand_cap.argtypes = []
+    L.sba_gpu_min_operand_cap.restype = ctypes.c_uint64
+    L.sba_gpu_max_lg.argtypes = []
+    L.sba_gpu_max_lg.restype = ctypes.c_int
+    L.sba_gpu_device_count.argtypes = []
+    L.sba_gpu_device_count.restype = ctypes.c_int
+    L.sba_set_gpu_threshold.argtypes = [u64]
+    L.sba_set_gpu_threshold.restype = None
+    L.sba_get_gpu_threshold.argtypes = []
+    L.sba_get_gpu_threshold.restype = u64
+    _lib = L
+    return L
+
+
+def _as_limbs(a) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    return a
+
+
+# ---------------------------------------------------------------------------
+# host-memory entries (reference semantics)
+# ---------------------------------------------------------------------------
+def nt_mul(a, b) -> Tuple[int, np.ndarray]:
+    """include/ntmul.h:71-74: returns (1, product) or (0, zeros) beyond band."""
+    a, b = _as_limbs(a), _as_limbs(b)
+    r = np.zeros(len(a) + len(b), dtype=np.uint64)
+    ok = lib().nt_mul(_arr(r), _arr(a), len(a), _arr(b), len(b))
+    return ok, r
+
+
+def nt_sqr(a) -> Tuple[int, np.ndarray]:
+    """include/ntmul.h:76-78."""
+    a = _as_limbs(a)
+    r = np.zeros(2 * len(a), dtype=np.uint64)
+    ok = lib().nt_sqr(_arr(r), _arr(a), len(a))
+    return ok, r
+
+
+def p30_mul(a, b) -> Tuple[int, np.ndarray]:
+    a, b = _as_limbs(a), _as_limbs(b)
+    r = np.zeros(len(a) + len(b), dtype=np.uint64)
+    return lib().p30_mul(_arr(r), _arr(a), len(a), _arr(b), len(b)), r
+
+
+def p50_mul(a, b) -> Tuple[int, np.ndarray]:
+    a, b = _as_limbs(a), _as_limbs(b)
+    r = np.zeros(len(a) + len(b), dtype=np.uint64)
+    return lib().p50_mul(_arr(r), _arr(a), len(a), _arr(b), len(b)), r
+
+
+def simd_mpn_mul(a, b) -> np.ndarray:
+    """include/u64cvt.h:94-132: void, writes exactly an+bn limbs."""
+    a, b = _as_limbs(a), _as_limbs(b)
+    r = np.zeros(len(a) + len(b), dtype=np.uint64)
+    lib().simd_mpn_mul(_arr(r), _arr(a), len(a), _arr(b), len(b))
+    return r
+
+
+def cpu_mul(a, b) -> np.ndarray:
+    a, b = _as_limbs(a), _as_limbs(b)
+    r = np.zeros(len(a) + len(b), dtype=np.uint64)
+    lib().sba_cpu_mul(_arr(r), _arr(a), len(a), _arr(b), len(b))
+    return r
+
+
+def divrem_u64(n, d, dc: bool = False) -> Tuple[np.ndarray, np.ndarray]:
+    """include/divrem.h:218-221 (dc=True: include/dc_div.h:154-157).
+    Returns (Q trimmed to *qn64_out limbs, R trimmed to *rn64_out limbs)."""
+    n, d = _as_limbs(n), _as_limbs(d)
+    if len(d) == 0 or d[-1] == 0:
+        raise ValueError("divisor's top limb must be nonzero")
+    qm = max(len(n) - len(d) + 1, 1)
+    q = np.zeros(qm, dtype=np.uint64)
+    r = np.zeros(len(d), dtype=np.uint64)
+    qn = ctypes.c_uint64()
+    rn = ctypes.c_uint64()
+    f = lib().divrem_u64_dc if dc else lib().divrem_u64
+    f(_arr(q), _arr(r), _arr(n), len(n), _arr(d), len(d), ctypes.byref(qn), ctypes.byref(rn))
+    return q[: qn.value], r[: rn.value]
+
+
+def set_gpu_threshold(min_total_limbs: int) -> None:
+    lib().sba_set_gpu_threshold(int(min_total_limbs))
+
+
+def get_gpu_threshold() -> int:
+    return int(lib().sba_get_gpu_threshold())
+
+
+# ---------------------------------------------------------------------------
+# device-resident entries (torch tensors as device buffers)
+# ---------------------------------------------------------------------------
+def _ptr(t) -> int:
+    return int(t.data_ptr())
+
+
+def mul_dev(r, a, b, stream=None) -> int:
+    """Enqueue r = a * b on device tensors (uint64/int64 limbs)."""
+    s = None if stream is None else ctypes.c_void_p(int(stream.cuda_stream))
+    return lib().sba_gpu_mul_dev(_ptr(r), _ptr(a), a.numel(), _ptr(b), b.numel(), s)
+
+
+def sqr_dev(r, a, stream=None) -> int:
+    s = None if stream is None else ctypes.c_void_p(int(stream.cuda_stream))
+    return lib().sba_gpu_sqr_dev(_ptr(r), _ptr(a), a.numel(), s)
+
+
+def min_operand_cap() -> int:
+    return int(lib().sba_gpu_min_operand_cap())
+
+
+def max_lg() -> int:
+    return int(lib().sba_gpu_max_lg())

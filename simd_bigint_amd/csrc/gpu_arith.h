@@ -1,0 +1,47 @@
+// gpu_arith.h -- mod-p arithmetic for the gfx950 NTT engine.
+//
+// Five primes p = k*2^25 + 1 < 2^31 (ascending; Garner relies on the order).
+// Values between butterflies are lazy in [0, 2p) (< 2^32 since p < 2^31).
+// Every multiply is a Montgomery product with R = 2^32 built from two
+// v_mad_u64_u32 and one v_mul_lo_u32; the constant operand (twiddles, Garner
+// constants) is stored in Montgomery form and canonical (< p), which keeps
+// the product in [0, 2p) for ANY 32-bit left operand:
+//   (a*b + m*p) / 2^32 < (2^32*p + 2^32*p) / 2^32 = 2p.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SBA_K 5            // primes
+#define SBA_MAX_LG 25      // 2^25 | p - 1 for every prime
+
+__device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t p,
+                                             uint32_t pinv) {
+  uint64_t t = (uint64_t)a * b;
+  uint32_t m = (uint32_t)t * pinv;          // pinv = -p^-1 mod 2^32
+  uint64_t u = (uint64_t)m * p + t;         // < 2^63 + 2^63
+  return (uint32_t)(u >> 32);
+}
+
+// [0, 2p) -> [0, p): if x < p, x - p wraps above 2^31 and min keeps x.
+__device__ __forceinline__ uint32_t red1(uint32_t x, uint32_t p) {
+  uint32_t y = x - p;
+  return y < x ? y : x;
+}
+
+// forward (Cooley-Tukey, node twiddle w): (x, y) -> (x + w y, x - w y)
+__device__ __forceinline__ void bfly_fwd(uint32_t &x, uint32_t &y, uint32_t w,
+                                         uint32_t p, uint32_t pinv) {
+  uint32_t a = red1(x, p);
+  uint32_t t = red1(mont_mul(y, w, p, pinv), p);
+  x = a + t;            // [0, 2p)
+  y = a + p - t;        // (0, 2p)
+}
+
+// inverse (Gentleman-Sande, inverse node twiddle wi): (x, y) -> (x + y, (x - y) wi)
+__device__ __forceinline__ void bfly_inv(uint32_t &x, uint32_t &y, uint32_t wi,
+                                         uint32_t p, uint32_t pinv) {
+  uint32_t a = red1(x, p);
+  uint32_t b = red1(y, p);
+  x = a + b;                                   // [0, 2p)
+  y = mont_mul(a + p - b, wi, p, pinv);        // [0, 2p)
+}

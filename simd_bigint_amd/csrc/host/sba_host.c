@@ -1,0 +1,323 @@
+/* sba_host.c -- host side (C) of the MI355X multiply: the reference's
+ * mpn-style entries on host memory, the threshold dispatch, the CPU tiers
+ * below the GPU crossover, and Newton division on the GPU multiply.
+ *
+ * Citations are files of the reference repository (iamstupid/simd_bigint).
+ */
+#include "sba_internal.h"
+
+#include <hip/hip_runtime_api.h>
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef unsigned __int128 u128;
+
+/* ------------------------------------------------------------------ */
+/* device staging: grow-only, one set per process (guarded), the       */
+/* device-side analogue of the reference's scratch arena rule "no      */
+/* per-call allocation" (include/scratch.h:1-20).                      */
+/* ------------------------------------------------------------------ */
+static pthread_mutex_t g_stage_mu = PTHREAD_MUTEX_INITIALIZER;
+static uint64_t *g_da, *g_db, *g_dr;
+static size_t g_na, g_nb, g_nr;
+static int g_dev_checked = 0, g_dev_ok = 0;
+
+static void die(const char *what, int rc) {
+  fprintf(stderr, "simd_bigint_amd: %s failed (code %d); the library has no CPU "
+                  "fallback for the GPU path\n", what, rc);
+  abort();
+}
+
+static int device_ok(void) {
+  if (!g_dev_checked) {
+    int n = 0;
+    g_dev_ok = hipGetDeviceCount(&n) == hipSuccess && n > 0;
+    g_dev_checked = 1;
+  }
+  return g_dev_ok;
+}
+
+static void grow(uint64_t **p, size_t *cap, size_t n) {
+  if (*cap >= n) return;
+  if (*p) hipFree(*p);
+  *p = NULL;
+  *cap = 0;
+  size_t want = n + n / 8 + 64;
+  if (hipMalloc((void **)p, want * 8) != hipSuccess) die("hipMalloc", SBA_ERR_HIP);
+  *cap = want;
+}
+
+/* H2D, multiply on the default stream, D2H. Returns the engine's code. */
+static int gpu_mul_host(uint64_t *rp, const uint64_t *ap, size_t an,
+                        const uint64_t *bp, size_t bn, int sqr) {
+  if (!device_ok()) die("device lookup (no HIP device)", SBA_ERR_NODEV);
+  pthread_mutex_lock(&g_stage_mu);
+  grow(&g_da, &g_na, an);
+  if (!sqr) grow(&g_db, &g_nb, bn);
+  grow(&g_dr, &g_nr, an + bn);
+  if (hipMemcpy(g_da, ap, an * 8, hipMemcpyHostToDevice) != hipSuccess)
+    die("hipMemcpy H2D", SBA_ERR_HIP);
+  if (!sqr && hipMemcpy(g_db, bp, bn * 8, hipMemcpyHostToDevice) != hipSuccess)
+    die("hipMemcpy H2D", SBA_ERR_HIP);
+  int rc = sqr ? sba_gpu_sqr_dev(g_dr, g_da, (int64_t)an, NULL)
+               : sba_gpu_mul_dev(g_dr, g_da, (int64_t)an, g_db, (int64_t)bn, NULL);
+  if (rc < 0) die("GPU multiply", rc);
+  if (rc == SBA_OK &&
+      hipMemcpy(rp, g_dr, (an + bn) * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    die("hipMemcpy D2H", SBA_ERR_HIP);
+  pthread_mutex_unlock(&g_stage_mu);
+  return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* NTT dispatcher entries: include/ntmul.h:47-79 (1 ok / 0 beyond band) */
+/* ------------------------------------------------------------------ */
+int nt_mul_r(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, const uint64_t *bp,
+             ptrdiff_t bn, struct scratch *sc) {
+  (void)sc;
+  if (an <= 0 || bn <= 0) return 0;
+  return gpu_mul_host(rp, ap, (size_t)an, bp, (size_t)bn, 0) == SBA_OK;
+}
+int nt_sqr_r(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, struct scratch *sc) {
+  (void)sc;
+  if (an <= 0) return 0;
+  return gpu_mul_host(rp, ap, (size_t)an, ap, (size_t)an, 1) == SBA_OK;
+}
+int nt_mul(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, const uint64_t *bp,
+           ptrdiff_t bn) {
+  return nt_mul_r(rp, ap, an, bp, bn, NULL);
+}
+int nt_sqr(uint64_t *rp, const uint64_t *ap, ptrdiff_t an) {
+  return nt_sqr_r(rp, ap, an, NULL);
+}
+
+/* p30's band: include/p30.h:1851-1858 */
+#define P30_CAP 2344425u
+static int p30_band(ptrdiff_t an, ptrdiff_t bn) {
+  if (an <= 0 || bn <= 0) return 0;
+  uint64_t mn = (uint64_t)(an < bn ? an : bn);
+  uint64_t zlen = (uint64_t)an + (uint64_t)bn - 1;
+  return mn <= P30_CAP && (zlen + 15) / 16 <= ((uint64_t)1 << 18);
+}
+int p30_mul_r(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, const uint64_t *bp,
+              ptrdiff_t bn, struct scratch *sc) {
+  if (!p30_band(an, bn)) return 0;
+  return nt_mul_r(rp, ap, an, bp, bn, sc);
+}
+int p30_sqr_r(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, struct scratch *sc) {
+  if (!p30_band(an, an)) return 0;
+  return nt_sqr_r(rp, ap, an, sc);
+}
+int p30_mul(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, const uint64_t *bp,
+            ptrdiff_t bn) {
+  return p30_mul_r(rp, ap, an, bp, bn, NULL);
+}
+int p30_sqr(uint64_t *rp, const uint64_t *ap, ptrdiff_t an) {
+  return p30_sqr_r(rp, ap, an, NULL);
+}
+int p50_mul_r(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, const uint64_t *bp,
+              ptrdiff_t bn, struct scratch *sc) {
+  return nt_mul_r(rp, ap, an, bp, bn, sc);
+}
+int p50_sqr_r(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, struct scratch *sc) {
+  return nt_sqr_r(rp, ap, an, sc);
+}
+int p50_mul(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, const uint64_t *bp,
+            ptrdiff_t bn) {
+  return p50_mul_r(rp, ap, an, bp, bn, NULL);
+}
+int p50_sqr(uint64_t *rp, const uint64_t *ap, ptrdiff_t an) {
+  return p50_sqr_r(rp, ap, an, NULL);
+}
+
+/* ------------------------------------------------------------------ */
+/* CPU tiers below the GPU crossover (the library's own code):         */
+/* u128 basecase and Karatsuba with strip-mining for unbalanced shapes  */
+/* -- the roles of include/mul.h:233 (basecase), :326 (Karatsuba) and  */
+/* :709 (strip-mining) in the reference's ladder.                      */
+/* ------------------------------------------------------------------ */
+#define SBA_KARA_MIN 32     /* basecase below this many limbs of the smaller operand */
+
+static void cpu_basecase(uint64_t *rp, const uint64_t *ap, size_t an, const uint64_t *bp,
+                         size_t bn) {
+  memset(rp, 0, (an + bn) * 8);
+  for (size_t j = 0; j < bn; ++j) {
+    uint64_t c = 0, b = bp[j];
+    for (size_t i = 0; i < an; ++i) {
+      u128 t = (u128)ap[i] * b + rp[i + j] + c;
+      rp[i + j] = (uint64_t)t;
+      c = (uint64_t)(t >> 64);
+    }
+    rp[j + an] = c;
+  }
+}
+
+static uint64_t add_n(uint64_t *r, const uint64_t *a, const uint64_t *b, size_t n) {
+  uint64_t c = 0;
+  for (size_t i = 0; i < n; ++i) {
+    u128 s = (u128)a[i] + b[i] + c;
+    r[i] = (uint64_t)s;
+    c = (uint64_t)(s >> 64);
+  }
+  return c;
+}
+/* r = a(an) + b(bn), an >= bn; returns carry */
+static uint64_t add_nm(uint64_t *r, const uint64_t *a, size_t an, const uint64_t *b,
+                       size_t bn) {
+  uint64_t c = add_n(r, a, b, bn);
+  for (size_t i = bn; i < an; ++i) {
+    u128 s = (u128)a[i] + c;
+    r[i] = (uint64_t)s;
+    c = (uint64_t)(s >> 64);
+  }
+  return c;
+}
+/* r[0..n) += a[0..m), propagate carry through r; returns carry out */
+static uint64_t add_into(uint64_t *r, size_t n, const uint64_t *a, size_t m) {
+  uint64_t c = 0;
+  size_t i = 0;
+  for (; i < m; ++i) {
+    u128 s = (u128)r[i] + a[i] + c;
+    r[i] = (uint64_t)s;
+    c = (uint64_t)(s >> 64);
+  }
+  for (; c && i < n; ++i) {
+    r[i] += 1;
+    c = r[i] == 0;
+  }
+  return c;
+}
+static uint64_t sub_into(uint64_t *r, size_t n, const uint64_t *a, size_t m) {
+  uint64_t bw = 0;
+  size_t i = 0;
+  for (; i < m; ++i) {
+    uint64_t x = r[i], y = a[i];
+    r[i] = x - y - bw;
+    bw = (x < y) | ((x == y) & bw);
+  }
+  for (; bw && i < n; ++i) {
+    bw = r[i] == 0;
+    r[i] -= 1;
+  }
+  return bw;
+}
+
+static void cpu_mul_bal(uint64_t *rp, const uint64_t *ap, const uint64_t *bp, size_t n,
+                        uint64_t *tmp);
+
+/* balanced Karatsuba, both operands n limbs; tmp holds >= 4n + 64 limbs per level */
+static void cpu_mul_bal(uint64_t *rp, const uint64_t *ap, const uint64_t *bp, size_t n,
+                        uint64_t *tmp) {
+  if (n < SBA_KARA_MIN) {
+    cpu_basecase(rp, ap, n, bp, n);
+    return;
+  }
+  size_t h = n / 2, l = n - h; /* low part l >= high part h */
+  uint64_t *sa = tmp, *sb = tmp + l + 1, *mid = tmp + 2 * (l + 1);
+  uint64_t *next = mid + 2 * (l + 1);
+  sa[l] = add_nm(sa, ap, l, ap + l, h);
+  sb[l] = add_nm(sb, bp, l, bp + l, h);
+  cpu_mul_bal(rp, ap, bp, l, next);                 /* low  -> rp[0, 2l) */
+  cpu_mul_bal(rp + 2 * l, ap + l, bp + l, h, next); /* high -> rp[2l, 2l+2h) */
+  cpu_mul_bal(mid, sa, sb, l + 1, next);            /* (a0+a1)(b0+b1), 2l+2 limbs */
+  sub_into(mid, 2 * l + 2, rp, 2 * l);
+  sub_into(mid, 2 * l + 2, rp + 2 * l, 2 * h);
+  add_into(rp + l, 2 * n - l, mid, 2 * l + 2 < 2 * n - l ? 2 * l + 2 : 2 * n - l);
+}
+
+static size_t kara_tmp(size_t n) {
+  size_t t = 0;
+  while (n >= SBA_KARA_MIN) {
+    size_t l = n - n / 2;
+    t += 4 * (l + 1);
+    n = l + 1;
+  }
+  return t + 64;
+}
+
+/* an >= bn >= 1 */
+static void cpu_mul(uint64_t *rp, const uint64_t *ap, size_t an, const uint64_t *bp,
+                    size_t bn) {
+  if (bn < SBA_KARA_MIN) {
+    cpu_basecase(rp, ap, an, bp, bn);
+    return;
+  }
+  uint64_t *tmp = (uint64_t *)malloc((kara_tmp(bn) + 4 * bn + 8) * 8);
+  if (!tmp) die("malloc", 0);
+  if (an == bn) {
+    cpu_mul_bal(rp, ap, bp, bn, tmp);
+    free(tmp);
+    return;
+  }
+  /* strip-mine a in bn-limb blocks, overlap-adding each 2bn product */
+  uint64_t *blk = tmp + kara_tmp(bn);
+  memset(rp, 0, (an + bn) * 8);
+  size_t done = 0;
+  while (done < an) {
+    size_t w = an - done < bn ? an - done : bn;
+    if (w == bn) cpu_mul_bal(blk, ap + done, bp, bn, tmp);
+    else cpu_mul(blk, bp, bn, ap + done, w);
+    add_into(rp + done, an + bn - done, blk, w + bn);
+    done += w;
+  }
+  free(tmp);
+}
+
+/* ------------------------------------------------------------------ */
+/* simd_mpn_mul: include/u64cvt.h:94-132 semantics (swap to an >= bn,  */
+/* smallest sizes on the basecase, threshold dispatch above).          */
+/* ------------------------------------------------------------------ */
+static uint64_t g_gpu_min_total = SBA_GPU_MIN_TOTAL_DEFAULT;
+
+void sba_set_gpu_threshold(uint64_t min_total_limbs) { g_gpu_min_total = min_total_limbs; }
+uint64_t sba_get_gpu_threshold(void) { return g_gpu_min_total; }
+
+void simd_mpn_mul(uint64_t *rp, const uint64_t *ap, uint64_t an, const uint64_t *bp,
+                  uint64_t bn) {
+  if (an < bn) {
+    const uint64_t *t = ap; ap = bp; bp = t;
+    uint64_t tn = an; an = bn; bn = tn;
+  }
+  if (bn == 0) {
+    if (an) memset(rp, 0, an * 8);
+    return;
+  }
+  if (an + bn >= g_gpu_min_total) {
+    int rc = gpu_mul_host(rp, ap, an, bp, bn, ap == bp && an == bn);
+    if (rc == SBA_OK) return;
+    /* beyond the GPU band (operands of > 43M limbs): split a in halves */
+    size_t h = an / 2;
+    uint64_t *t = (uint64_t *)malloc((an - h + bn) * 8);
+    if (!t) die("malloc", 0);
+    simd_mpn_mul(rp, ap, h, bp, bn);
+    memset(rp + h + bn, 0, (an - h) * 8);
+    simd_mpn_mul(t, ap + h, an - h, bp, bn);
+    add_into(rp + h, an + bn - h, t, an - h + bn);
+    free(t);
+    return;
+  }
+  cpu_mul(rp, ap, an, bp, bn);
+}
+
+/* internal multiply used by the division (either order, an,bn >= 1) */
+void sba_mul(uint64_t *rp, const uint64_t *ap, size_t an, const uint64_t *bp, size_t bn) {
+  simd_mpn_mul(rp, ap, an, bp, bn);
+}
+
+/* ------------------------------------------------------------------ */
+/* exported helpers for the tests of the host logic                    */
+/* ------------------------------------------------------------------ */
+void sba_cpu_mul(uint64_t *rp, const uint64_t *ap, uint64_t an, const uint64_t *bp,
+                 uint64_t bn) {
+  if (an < bn) {
+    const uint64_t *t = ap; ap = bp; bp = t;
+    uint64_t tn = an; an = bn; bn = tn;
+  }
+  if (bn == 0) {
+    if (an) memset(rp, 0, an * 8);
+    return;
+  }
+  cpu_mul(rp, ap, an, bp, bn);
+}

@@ -1,0 +1,957 @@
+// ntt_engine.hip -- gfx950 kernels of the multi-prime NTT multiply and the
+// thin C-ABI layer that launches them (declared in sba_gpu.h).
+//
+// Math (one prime p, transform length N = 2^L):
+//   * Root table in the FLINT / reference convention (include/p30.h:312-329,
+//     tests/p30_tft_model_test.c:66-94): w[0] = 1, w[2^(t-1) + s] =
+//     zeta_t * w[s], zeta_t of order 2^t, zeta_1 = -1. The butterfly of
+//     node j uses w[2j]; the device keeps the node table wn[j] = w[2j]
+//     (and wni[j] = w[2j]^-1) in Montgomery form. Both are prefix-stable,
+//     so one table built for the largest L serves every smaller L.
+//   * Forward = the flat N-point Cooley-Tukey network: level s, block j
+//     (j = i >> (L - s)) pairs i, i + N/2^(s+1): (x, y) -> (x + wn[j] y,
+//     x - wn[j] y). Output position k holds the evaluation at w[k], the
+//     same spectral order as the reference's forward transform.
+//   * Inverse = the Gentleman-Sande mirror with wni, levels reversed; the
+//     factor N is removed in the Garner stage.
+//   * A PASS runs ELL consecutive levels [s0, s0 + ELL) on TILES: 2^ELL
+//     elements of stride S = 2^(L - s0 - ELL) (one block of level s0) times
+//     TC = 2^LOGTC adjacent such groups, staged in LDS (<= 8192 words).
+//     Inside the tile, CHUNKS of up to 5 levels run in registers (radix-32
+//     sub-butterflies per thread), with LDS exchanges between chunks.
+//   * Garner over the five residues (ascending primes) composes each
+//     coefficient into 3 u64 words; a 3-kernel carry-lookahead scan over a
+//     4-state carry map adds the words into limbs bit-exactly.
+#include "gpu_arith.h"
+#include "simd_bigint_amd.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <mutex>
+
+// ---------------------------------------------------------------------------
+// kernel argument block (by value)
+// ---------------------------------------------------------------------------
+struct PrimeArgs {
+  uint32_t p[SBA_K], pinv[SBA_K], r2[SBA_K];
+  const uint32_t *wn[SBA_K];
+  const uint32_t *wni[SBA_K];
+};
+
+struct GarnerArgs {
+  uint32_t p[SBA_K], pinv[SBA_K];
+  uint32_t scale[SBA_K];          // N^-1 * R^2 mod p (undoes N and the pointwise R^-1)
+  uint32_t cm[SBA_K][SBA_K];      // (prod_{j<k} p_j mod p_q) * R mod p_q
+  uint32_t im[SBA_K];             // (prod_{j<q} p_j)^-1 * R mod p_q
+};
+
+#define NT 256
+__device__ __forceinline__ uint32_t lpad(uint32_t i) { return i + (i >> 5); }
+
+// blockIdx -> tile so that consecutive tiles (which share cache lines and
+// twiddle rows) run on the same XCD: the dispatcher hands block b to XCD
+// b % 8.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
+  if (n & 7) return b;
+  return (b & 7) * (n >> 3) + (b >> 3);
+}
+
+template <int X> struct ILog2 { static constexpr int v = 1 + ILog2<X / 2>::v; };
+template <> struct ILog2<1> { static constexpr int v = 0; };
+
+// residue of a limb, lazy in [0, 2p): (hi * 2^32 mod p) + (lo mod p).
+// p > 2^30 so lo < 4p: two folds bring lo into [0, p).
+__device__ __forceinline__ uint32_t limb_residue(uint64_t x, uint32_t p, uint32_t pinv,
+                                                 uint32_t r2) {
+  uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+  uint32_t h = red1(mont_mul(hi, r2, p, pinv), p);
+  uint32_t l2 = lo - 2u * p;
+  uint32_t l = l2 < lo ? l2 : lo;
+  return h + red1(l, p);
+}
+
+// ---------------------------------------------------------------------------
+// one chunk of B levels (local levels T0 .. T0+B-1 of an ELL-level pass)
+// ---------------------------------------------------------------------------
+template <int ELL, int LOGTC, int T0, int B, bool INV>
+__device__ __forceinline__ void tile_chunk(uint32_t *lds, const uint32_t *__restrict__ tw,
+                                           uint32_t H, uint32_t p, uint32_t pinv) {
+  constexpr int TC = 1 << LOGTC;
+  constexpr int NE = 1 << (ELL + LOGTC);
+  constexpr int GPT = NE / ((1 << B) * NT);
+  constexpr int LB = ELL - T0 - B;
+  static_assert(GPT >= 1, "chunk too wide for the tile");
+  __syncthreads();
+#pragma unroll
+  for (int gi = 0; gi < GPT; ++gi) {
+    const uint32_t g = threadIdx.x + gi * NT;
+    const uint32_t c = g & (TC - 1);
+    const uint32_t rest = g >> LOGTC;
+    const uint32_t lo = rest & ((1u << LB) - 1);
+    const uint32_t hb = rest >> LB;
+    const uint32_t kbase = (hb << (ELL - T0)) | lo;
+    const uint32_t hj = (H << T0) | hb;
+    uint32_t v[1 << B];
+#pragma unroll
+    for (int m = 0; m < (1 << B); ++m)
+      v[m] = lds[lpad(((kbase | ((uint32_t)m << LB)) << LOGTC) | c)];
+    if (!INV) {
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        const int half = 1 << (B - u - 1);
+#pragma unroll
+        for (int m2 = 0; m2 < (1 << u); ++m2) {
+          const uint32_t w = tw[(hj << u) | (uint32_t)m2];
+#pragma unroll
+          for (int r = 0; r < half; ++r) {
+            const int m = (m2 << (B - u)) | r;
+            bfly_fwd(v[m], v[m + half], w, p, pinv);
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int u = B - 1; u >= 0; --u) {
+        const int half = 1 << (B - u - 1);
+#pragma unroll
+        for (int m2 = 0; m2 < (1 << u); ++m2) {
+          const uint32_t w = tw[(hj << u) | (uint32_t)m2];
+#pragma unroll
+          for (int r = 0; r < half; ++r) {
+            const int m = (m2 << (B - u)) | r;
+            bfly_inv(v[m], v[m + half], w, p, pinv);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < (1 << B); ++m)
+      lds[lpad(((kbase | ((uint32_t)m << LB)) << LOGTC) | c)] = v[m];
+  }
+}
+
+template <int ELL, int LOGTC> struct ChunkPlan {
+  static constexpr int E = (1 << (ELL + LOGTC)) / NT;
+  static constexpr int LE = ILog2<E>::v;
+  static constexpr int BM = LE < 5 ? LE : 5;
+};
+
+template <int ELL, int LOGTC, int T0>
+__device__ __forceinline__ void tile_fwd(uint32_t *lds, const uint32_t *tw, uint32_t H,
+                                         uint32_t p, uint32_t pinv) {
+  if constexpr (T0 < ELL) {
+    constexpr int BM = ChunkPlan<ELL, LOGTC>::BM;
+    constexpr int B = (ELL - T0) < BM ? (ELL - T0) : BM;
+    tile_chunk<ELL, LOGTC, T0, B, false>(lds, tw, H, p, pinv);
+    tile_fwd<ELL, LOGTC, T0 + B>(lds, tw, H, p, pinv);
+  }
+}
+
+template <int ELL, int LOGTC, int T0>
+__device__ __forceinline__ void tile_inv(uint32_t *lds, const uint32_t *tw, uint32_t H,
+                                         uint32_t p, uint32_t pinv) {
+  if constexpr (T0 < ELL) {
+    constexpr int BM = ChunkPlan<ELL, LOGTC>::BM;
+    constexpr int B = (ELL - T0) < BM ? (ELL - T0) : BM;
+    tile_inv<ELL, LOGTC, T0 + B>(lds, tw, H, p, pinv);
+    tile_chunk<ELL, LOGTC, T0, B, true>(lds, tw, H, p, pinv);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass kernels
+// ---------------------------------------------------------------------------
+// First forward pass (global levels [0, ELL)): limbs -> residues of all K
+// primes -> ELL levels -> spec[q][.]. Tiles = column tiles (H = 0).
+template <int ELL, int LOGTC>
+__global__ __launch_bounds__(NT) void k_first_fwd(PrimeArgs pa, const uint64_t *__restrict__ limbs,
+                                                  uint64_t nlimbs, uint32_t *__restrict__ spec,
+                                                  int L) {
+  constexpr int TC = 1 << LOGTC;
+  constexpr int NE = 1 << (ELL + LOGTC);
+  constexpr int E = NE / NT;
+  __shared__ uint32_t lds[NE + NE / 32];
+  const size_t N = (size_t)1 << L;
+  const uint32_t S = 1u << (L - ELL);
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t col0 = tile << LOGTC;
+  uint64_t x[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const uint32_t e = threadIdx.x + i * NT;
+    const size_t gidx = (size_t)(e >> LOGTC) * S + col0 + (e & (TC - 1));
+    x[i] = gidx < nlimbs ? limbs[gidx] : 0;
+  }
+#pragma unroll 1
+  for (int q = 0; q < SBA_K; ++q) {
+    const uint32_t p = pa.p[q], pinv = pa.pinv[q], r2 = pa.r2[q];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const uint32_t e = threadIdx.x + i * NT;
+      lds[lpad(e)] = limb_residue(x[i], p, pinv, r2);
+    }
+    tile_fwd<ELL, LOGTC, 0>(lds, pa.wn[q], 0, p, pinv);
+    __syncthreads();
+    uint32_t *out = spec + q * N;
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const uint32_t e = threadIdx.x + i * NT;
+      const size_t gidx = (size_t)(e >> LOGTC) * S + col0 + (e & (TC - 1));
+      out[gidx] = lds[lpad(e)];
+    }
+  }
+}
+
+// In-place pass over global levels [s0, s0 + ELL) of one prime
+// (blockIdx.y), forward or inverse.
+template <int ELL, int LOGTC, bool INV>
+__global__ __launch_bounds__(NT) void k_pass(PrimeArgs pa, uint32_t *__restrict__ spec, int L,
+                                             int s0) {
+  constexpr int TC = 1 << LOGTC;
+  constexpr int NE = 1 << (ELL + LOGTC);
+  constexpr int E = NE / NT;
+  __shared__ uint32_t lds[NE + NE / 32];
+  const int q = blockIdx.y;
+  const size_t N = (size_t)1 << L;
+  const uint32_t S = 1u << (L - s0 - ELL);
+  const uint32_t ctiles = S >> LOGTC;
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t H = tile / ctiles;
+  const uint32_t col0 = (tile % ctiles) << LOGTC;
+  const size_t base = ((size_t)H << (L - s0)) + col0;
+  uint32_t *buf = spec + q * N;
+  const uint32_t p = pa.p[q], pinv = pa.pinv[q];
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const uint32_t e = threadIdx.x + i * NT;
+    lds[lpad(e)] = buf[base + (size_t)(e >> LOGTC) * S + (e & (TC - 1))];
+  }
+  if (!INV) tile_fwd<ELL, LOGTC, 0>(lds, pa.wn[q], H, p, pinv);
+  else tile_inv<ELL, LOGTC, 0>(lds, pa.wni[q], H, p, pinv);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const uint32_t e = threadIdx.x + i * NT;
+    buf[base + (size_t)(e >> LOGTC) * S + (e & (TC - 1))] = lds[lpad(e)];
+  }
+}
+
+// Last forward levels of b + pointwise product with a's spectrum (or b's own
+// square) + first inverse levels, on contiguous tiles (S = 1, TC = 1).
+// LIMB_IN: the pass is the whole transform (s0 = 0) and reads b's limbs.
+template <int ELL, bool LIMB_IN, bool SQR>
+__global__ __launch_bounds__(NT) void k_fused(PrimeArgs pa, const uint64_t *__restrict__ blimbs,
+                                              uint64_t bn, const uint32_t *__restrict__ aspec,
+                                              uint32_t *__restrict__ bspec, int L) {
+  constexpr int NE = 1 << ELL;
+  constexpr int E = NE / NT;
+  __shared__ uint32_t lds[NE + NE / 32];
+  const int q = blockIdx.y;
+  const size_t N = (size_t)1 << L;
+  const uint32_t H = blockIdx.x;
+  const size_t base = (size_t)H << ELL;
+  const uint32_t p = pa.p[q], pinv = pa.pinv[q];
+  uint32_t *buf = bspec + q * N;
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const uint32_t e = threadIdx.x + i * NT;
+    uint32_t v;
+    if (LIMB_IN) {
+      const size_t gidx = base + e;
+      v = limb_residue(gidx < bn ? blimbs[gidx] : 0, p, pinv, pa.r2[q]);
+    } else {
+      v = buf[base + e];
+    }
+    lds[lpad(e)] = v;
+  }
+  tile_fwd<ELL, 0, 0>(lds, pa.wn[q], H, p, pinv);
+  __syncthreads();
+  const uint32_t *as = aspec + q * N;
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const uint32_t e = threadIdx.x + i * NT;
+    const uint32_t v = lds[lpad(e)];
+    const uint32_t o = SQR ? red1(v, p) : red1(as[base + e], p);
+    lds[lpad(e)] = mont_mul(v, o, p, pinv);
+  }
+  tile_inv<ELL, 0, 0>(lds, pa.wni[q], H, p, pinv);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const uint32_t e = threadIdx.x + i * NT;
+    buf[base + e] = lds[lpad(e)];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Garner: five residues -> coefficient as 3 u64 words (SoA: w0, w1, w2)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_garner(GarnerArgs ga, const uint32_t *__restrict__ spec,
+                                               size_t N, uint64_t zlen, uint64_t *__restrict__ w0,
+                                               uint64_t *__restrict__ w1,
+                                               uint64_t *__restrict__ w2) {
+  const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
+  if (i >= zlen) return;
+  uint32_t u[SBA_K], x[SBA_K];
+#pragma unroll
+  for (int q = 0; q < SBA_K; ++q)
+    u[q] = red1(mont_mul(spec[q * N + i], ga.scale[q], ga.p[q], ga.pinv[q]), ga.p[q]);
+  x[0] = u[0];
+#pragma unroll
+  for (int q = 1; q < SBA_K; ++q) {
+    const uint32_t p = ga.p[q], pinv = ga.pinv[q];
+    uint32_t t = x[0];
+#pragma unroll
+    for (int k = 1; k < q; ++k) {
+      t += red1(mont_mul(x[k], ga.cm[q][k], p, pinv), p);
+      t = red1(t, p);
+    }
+    const uint32_t d = red1(u[q] + p - t, p);
+    x[q] = red1(mont_mul(d, ga.im[q], p, pinv), p);
+  }
+  // Horner: c = (((x4 p3 + x3) p2 + x2) p1 + x1) p0 + x0  (< 2^155)
+  unsigned __int128 c = x[4];
+  c = c * ga.p[3] + x[3];
+  c = c * ga.p[2] + x[2];
+  c = c * ga.p[1] + x[1];
+  const unsigned __int128 t0 = (unsigned __int128)(uint64_t)c * ga.p[0] + x[0];
+  const unsigned __int128 t1 =
+      (unsigned __int128)(uint64_t)(c >> 64) * ga.p[0] + (uint64_t)(t0 >> 64);
+  w0[i] = (uint64_t)t0;
+  w1[i] = (uint64_t)t1;
+  w2[i] = (uint64_t)(t1 >> 64);
+}
+
+// ---------------------------------------------------------------------------
+// carry propagation. Limb i receives s_i = w0[i] + w1[i-1] + w2[i-2]
+// (< 3 * 2^64); with carry c in {0..3}: out_i = (s_i + c) mod 2^64,
+// c' = floor((s_i + c) / 2^64) <= 3. A chunk of limbs is a map
+// {0..3} -> {0..3} (2 bits per entry, 8 bits); maps compose
+// associatively, so the carries are an exclusive scan of maps.
+// ---------------------------------------------------------------------------
+#define CARRY_LPT 8                     // limbs per thread
+#define CARRY_LPB (CARRY_LPT * NT)      // limbs per block
+#define MAP_ID 0xE4u                    // c -> c
+
+__device__ __forceinline__ uint32_t map_apply(uint32_t f, uint32_t c) { return (f >> (2 * c)) & 3u; }
+// first f, then g
+__device__ __forceinline__ uint32_t map_then(uint32_t f, uint32_t g) {
+  uint32_t r = 0;
+#pragma unroll
+  for (uint32_t c = 0; c < 4; ++c) r |= map_apply(g, map_apply(f, c)) << (2 * c);
+  return r;
+}
+
+__device__ __forceinline__ void limb_sum(const uint64_t *__restrict__ w0,
+                                         const uint64_t *__restrict__ w1,
+                                         const uint64_t *__restrict__ w2, uint64_t zlen,
+                                         uint64_t i, uint64_t &v, uint32_t &h) {
+  const uint64_t a = i < zlen ? w0[i] : 0;
+  const uint64_t b = (i >= 1 && i - 1 < zlen) ? w1[i - 1] : 0;
+  const uint64_t c = (i >= 2 && i - 2 < zlen) ? w2[i - 2] : 0;
+  const uint64_t s = a + b;
+  uint32_t hh = s < a;
+  const uint64_t s2 = s + c;
+  hh += s2 < s;
+  v = s2;
+  h = hh;
+}
+
+// thread's chunk map (also returns the chunk's sums for reuse)
+__device__ __forceinline__ uint32_t chunk_map(const uint64_t *v, const uint32_t *h) {
+  uint32_t f = 0;
+#pragma unroll
+  for (uint32_t c0 = 0; c0 < 4; ++c0) {
+    uint32_t c = c0;
+#pragma unroll
+    for (int k = 0; k < CARRY_LPT; ++k) {
+      const uint64_t t = v[k] + c;
+      c = h[k] + (t < v[k] ? 1u : 0u);
+    }
+    f |= c << (2 * c0);
+  }
+  return f;
+}
+
+// block-wide exclusive scan of maps; returns the exclusive prefix of this
+// thread and writes the block total into *total (all threads).
+__device__ uint32_t block_scan_maps(uint32_t f, uint32_t *total) {
+  __shared__ uint32_t wtot[NT / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t inc = f;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc = map_then(o, inc);
+  }
+  uint32_t exc = __shfl_up(inc, 1, 64);
+  if (lane == 0) exc = MAP_ID;
+  if (lane == 63) wtot[wid] = inc;
+  __syncthreads();
+  uint32_t pre = MAP_ID;
+  for (int w = 0; w < wid; ++w) pre = map_then(pre, wtot[w]);
+  uint32_t tot = MAP_ID;
+  for (int w = 0; w < NT / 64; ++w) tot = map_then(tot, wtot[w]);
+  *total = tot;
+  __syncthreads();
+  return map_then(pre, exc);
+}
+
+__global__ __launch_bounds__(NT) void k_carry_maps(const uint64_t *__restrict__ w0,
+                                                   const uint64_t *__restrict__ w1,
+                                                   const uint64_t *__restrict__ w2,
+                                                   uint64_t zlen, uint64_t Z,
+                                                   uint32_t *__restrict__ bmaps) {
+  const uint64_t i0 = (uint64_t)blockIdx.x * CARRY_LPB + (uint64_t)threadIdx.x * CARRY_LPT;
+  uint64_t v[CARRY_LPT];
+  uint32_t h[CARRY_LPT];
+#pragma unroll
+  for (int k = 0; k < CARRY_LPT; ++k) {
+    if (i0 + k < Z) limb_sum(w0, w1, w2, zlen, i0 + k, v[k], h[k]);
+    else { v[k] = 0; h[k] = 0; }
+  }
+  uint32_t tot;
+  block_scan_maps(chunk_map(v, h), &tot);
+  if (threadIdx.x == 0) bmaps[blockIdx.x] = tot;
+}
+
+// one block: carry into every block = exclusive scan of block maps applied to 0
+__global__ __launch_bounds__(NT) void k_carry_blocks(const uint32_t *__restrict__ bmaps,
+                                                     uint32_t nb, uint32_t *__restrict__ bcin) {
+  const uint32_t per = (nb + NT - 1) / NT;
+  const uint32_t b0 = threadIdx.x * per;
+  uint32_t f = MAP_ID;
+  for (uint32_t k = 0; k < per; ++k)
+    if (b0 + k < nb) f = map_then(f, bmaps[b0 + k]);
+  uint32_t tot;
+  const uint32_t pre = block_scan_maps(f, &tot);
+  uint32_t c = map_apply(pre, 0);
+  for (uint32_t k = 0; k < per; ++k)
+    if (b0 + k < nb) {
+      bcin[b0 + k] = c;
+      c = map_apply(bmaps[b0 + k], c);
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_carry_out(const uint64_t *__restrict__ w0,
+                                                  const uint64_t *__restrict__ w1,
+                                                  const uint64_t *__restrict__ w2,
+                                                  uint64_t zlen, uint64_t Z,
+                                                  const uint32_t *__restrict__ bcin,
+                                                  uint64_t *__restrict__ rp) {
+  const uint64_t i0 = (uint64_t)blockIdx.x * CARRY_LPB + (uint64_t)threadIdx.x * CARRY_LPT;
+  uint64_t v[CARRY_LPT];
+  uint32_t h[CARRY_LPT];
+#pragma unroll
+  for (int k = 0; k < CARRY_LPT; ++k) {
+    if (i0 + k < Z) limb_sum(w0, w1, w2, zlen, i0 + k, v[k], h[k]);
+    else { v[k] = 0; h[k] = 0; }
+  }
+  uint32_t tot;
+  const uint32_t pre = block_scan_maps(chunk_map(v, h), &tot);
+  uint32_t c = map_apply(pre, bcin[blockIdx.x]);
+#pragma unroll
+  for (int k = 0; k < CARRY_LPT; ++k) {
+    const uint64_t t = v[k] + c;
+    c = h[k] + (t < v[k] ? 1u : 0u);
+    if (i0 + k < Z) rp[i0 + k] = t;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// twiddle tables: wn[j] = prod over set bits t of j of zeta_{t+2} (Montgomery
+// form), i.e. w[2j] of the FLINT table; wni the same with inverse roots.
+// ---------------------------------------------------------------------------
+struct TwArgs {
+  uint32_t p, pinv, one;          // one = R mod p
+  uint32_t z[SBA_MAX_LG], zi[SBA_MAX_LG];
+};
+
+__global__ __launch_bounds__(NT) void k_twiddles(TwArgs ta, uint32_t *__restrict__ wn,
+                                                 uint32_t *__restrict__ wni, uint32_t count) {
+  const uint32_t j = blockIdx.x * NT + threadIdx.x;
+  if (j >= count) return;
+  uint32_t a = ta.one, b = ta.one;
+  for (int t = 0; t < SBA_MAX_LG; ++t)
+    if ((j >> t) & 1u) {
+      a = red1(mont_mul(a, ta.z[t], ta.p, ta.pinv), ta.p);
+      b = red1(mont_mul(b, ta.zi[t], ta.p, ta.pinv), ta.p);
+    }
+  wn[j] = a;
+  wni[j] = b;
+}
+
+// ===========================================================================
+// host side of the thin layer
+// ===========================================================================
+static const uint32_t kPrimes[SBA_K] = {1107296257u, 1711276033u, 1811939329u, 2013265921u,
+                                        2113929217u};
+
+static uint64_t h_powm(uint64_t a, uint64_t e, uint64_t p) {
+  uint64_t r = 1;
+  a %= p;
+  for (; e; e >>= 1, a = a * a % p)
+    if (e & 1) r = r * a % p;
+  return r;
+}
+static uint64_t h_invm(uint64_t a, uint64_t p) { return h_powm(a, p - 2, p); }
+static uint32_t h_mont(uint64_t a, uint64_t p) { return (uint32_t)(((a % p) << 32) % p); }
+static uint64_t h_primroot(uint64_t p) {
+  uint64_t f[16];
+  int nf = 0;
+  uint64_t n = p - 1;
+  for (uint64_t d = 2; d * d <= n; d += (d == 2 ? 1 : 2))
+    if (n % d == 0) {
+      f[nf++] = d;
+      while (n % d == 0) n /= d;
+    }
+  if (n > 1) f[nf++] = n;
+  for (uint64_t g = 2;; ++g) {
+    int ok = 1;
+    for (int i = 0; i < nf && ok; ++i)
+      if (h_powm(g, (p - 1) / f[i], p) == 1) ok = 0;
+    if (ok) return g;
+  }
+}
+
+struct DevCtx {
+  int init = 0;
+  int tw_lg = 0;                       // tables hold 2^(tw_lg-1) node twiddles
+  uint32_t *wn[SBA_K] = {}, *wni[SBA_K] = {};
+  PrimeArgs pa;
+  GarnerArgs ga_base;
+  // grow-only workspace
+  void *ws = nullptr;
+  size_t ws_bytes = 0;
+};
+
+#define SBA_MAX_DEV 16
+static DevCtx g_ctx[SBA_MAX_DEV];
+static std::mutex g_mu;
+
+#define HIPCHK(x)                                                                     \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess) {                                                           \
+      fprintf(stderr, "simd_bigint_amd: %s failed: %s (%s:%d)\n", #x,                 \
+              hipGetErrorString(e_), __FILE__, __LINE__);                             \
+      return SBA_ERR_HIP;                                                             \
+    }                                                                                 \
+  } while (0)
+
+static int ctx_init(DevCtx &c) {
+  if (c.init) return 0;
+  for (int q = 0; q < SBA_K; ++q) {
+    const uint64_t p = kPrimes[q];
+    uint32_t inv = 1;
+    for (int i = 0; i < 5; ++i) inv *= 2u - (uint32_t)p * inv;   // p^-1 mod 2^32
+    c.pa.p[q] = (uint32_t)p;
+    c.pa.pinv[q] = 0u - inv;
+    c.pa.r2[q] = (uint32_t)(((unsigned __int128)1 << 64) % p);
+    c.ga_base.p[q] = (uint32_t)p;
+    c.ga_base.pinv[q] = 0u - inv;
+  }
+  for (int q = 1; q < SBA_K; ++q) {
+    const uint64_t pq = kPrimes[q];
+    uint64_t pp = 1;
+    for (int k = 0; k < q; ++k) {
+      c.ga_base.cm[q][k] = h_mont(pp, pq);
+      pp = pp * (kPrimes[k] % pq) % pq;
+    }
+    c.ga_base.im[q] = h_mont(h_invm(pp, pq), pq);
+  }
+  c.init = 1;
+  return 0;
+}
+
+static int ctx_twiddles(DevCtx &c, int L, hipStream_t st) {
+  if (L <= c.tw_lg) return 0;
+  const int Lt = SBA_MAX_LG;            // build once for the largest transform
+  const uint32_t count = 1u << (Lt - 1);
+  for (int q = 0; q < SBA_K; ++q) {
+    if (c.wn[q]) { HIPCHK(hipFree(c.wn[q])); HIPCHK(hipFree(c.wni[q])); }
+    HIPCHK(hipMalloc(&c.wn[q], (size_t)count * 4));
+    HIPCHK(hipMalloc(&c.wni[q], (size_t)count * 4));
+    const uint64_t p = kPrimes[q];
+    const uint64_t g = h_primroot(p);
+    TwArgs ta;
+    ta.p = (uint32_t)p;
+    ta.pinv = c.pa.pinv[q];
+    ta.one = h_mont(1, p);
+    for (int t = 0; t < SBA_MAX_LG; ++t) {
+      // bit t of a node index carries zeta_{t+2}, a root of order 2^(t+2)
+      const int ord = t + 2;
+      uint64_t z = ord <= SBA_MAX_LG ? h_powm(g, (p - 1) >> ord, p) : 1;
+      ta.z[t] = h_mont(z, p);
+      ta.zi[t] = h_mont(h_invm(z, p), p);
+    }
+    hipLaunchKernelGGL(k_twiddles, dim3((count + NT - 1) / NT), dim3(NT), 0, st, ta, c.wn[q],
+                       c.wni[q], count);
+    HIPCHK(hipGetLastError());
+    c.pa.wn[q] = c.wn[q];
+    c.pa.wni[q] = c.wni[q];
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  c.tw_lg = Lt;
+  return 0;
+}
+
+// workspace layout for transform length N: aspec, bspec (K*N u32 each),
+// w0, w1, w2 (N u64 each), carry maps and carry-ins (N/CARRY_LPB+1 u32 each)
+static size_t ws_need(size_t N) {
+  const size_t nb = N / CARRY_LPB + 2;
+  return 2 * SBA_K * N * 4 + 3 * N * 8 + 2 * nb * 4 + 4096;
+}
+
+static int lg_for(uint64_t zlen) {
+  int L = 10;                           // smallest transform the kernels take
+  while (((uint64_t)1 << L) < zlen) ++L;
+  return L;
+}
+
+extern "C" size_t sba_gpu_workspace_bytes(int64_t an, int64_t bn) {
+  if (an <= 0 || bn <= 0) return 0;
+  const int L = lg_for((uint64_t)an + (uint64_t)bn - 1);
+  if (L > SBA_MAX_LG) return 0;
+  return ws_need((size_t)1 << L);
+}
+
+// CRT cap: largest min(an, bn) with min * (2^64-1)^2 < prod(p)
+extern "C" uint64_t sba_gpu_min_operand_cap(void) {
+  // prod(p) as 3 u64 words, then bisection on q * (2^64-1)^2 <= prod(p)
+  uint64_t w[3] = {1, 0, 0};
+  for (int i = 0; i < SBA_K; ++i) {
+    unsigned __int128 cc = 0;
+    for (int k = 0; k < 3; ++k) {
+      cc += (unsigned __int128)w[k] * kPrimes[i];
+      w[k] = (uint64_t)cc;
+      cc >>= 64;
+    }
+  }
+  uint64_t lo = 0, hi = (uint64_t)1 << 62;
+  while (lo < hi) {
+    const uint64_t mid = lo + (hi - lo + 1) / 2;
+    unsigned __int128 cc = (unsigned __int128)mid * 1u;
+    uint64_t t0 = (uint64_t)cc;
+    cc = (unsigned __int128)mid * ~(uint64_t)1 + (uint64_t)(cc >> 64);
+    uint64_t t1 = (uint64_t)cc, t2 = (uint64_t)(cc >> 64);
+    int le;
+    if (t2 != w[2]) le = t2 < w[2];
+    else if (t1 != w[1]) le = t1 < w[1];
+    else le = t0 <= w[0];
+    if (le) lo = mid; else hi = mid - 1;
+  }
+  return lo;    // lo * F < prod strictly: F = (2^64-1)^2 shares no factor with prod
+}
+
+extern "C" int sba_gpu_max_lg(void) { return SBA_MAX_LG; }
+
+// ---------------------------------------------------------------------------
+// launch helpers (template dispatch on the pass shape)
+// ---------------------------------------------------------------------------
+typedef void (*first_fn)(PrimeArgs, const uint64_t *, uint64_t, uint32_t *, int);
+typedef void (*pass_fn)(PrimeArgs, uint32_t *, int, int);
+typedef void (*fused_fn)(PrimeArgs, const uint64_t *, uint64_t, const uint32_t *, uint32_t *, int);
+
+// first pass of the two-pass plan: ELL = L - 13 levels, TC = 2^(13 - ELL)
+static first_fn first_two_pass(int ell) {
+  switch (ell) {
+    case 1: return k_first_fwd<1, 12>;
+    case 2: return k_first_fwd<2, 11>;
+    case 3: return k_first_fwd<3, 10>;
+    case 4: return k_first_fwd<4, 9>;
+    case 5: return k_first_fwd<5, 8>;
+    case 6: return k_first_fwd<6, 7>;
+    case 7: return k_first_fwd<7, 6>;
+    case 8: return k_first_fwd<8, 5>;
+    case 9: return k_first_fwd<9, 4>;
+    case 10: return k_first_fwd<10, 3>;
+    case 11: return k_first_fwd<11, 2>;
+    case 12: return k_first_fwd<12, 1>;
+  }
+  return nullptr;
+}
+static pass_fn inv_first_two_pass(int ell) {
+  switch (ell) {
+    case 1: return k_pass<1, 12, true>;
+    case 2: return k_pass<2, 11, true>;
+    case 3: return k_pass<3, 10, true>;
+    case 4: return k_pass<4, 9, true>;
+    case 5: return k_pass<5, 8, true>;
+    case 6: return k_pass<6, 7, true>;
+    case 7: return k_pass<7, 6, true>;
+    case 8: return k_pass<8, 5, true>;
+    case 9: return k_pass<9, 4, true>;
+    case 10: return k_pass<10, 3, true>;
+    case 11: return k_pass<11, 2, true>;
+    case 12: return k_pass<12, 1, true>;
+  }
+  return nullptr;
+}
+static first_fn first_single(int L) {
+  switch (L) {
+    case 10: return k_first_fwd<10, 0>;
+    case 11: return k_first_fwd<11, 0>;
+    case 12: return k_first_fwd<12, 0>;
+    case 13: return k_first_fwd<13, 0>;
+  }
+  return nullptr;
+}
+static fused_fn fused_single(int L, int sqr) {
+  switch (L) {
+    case 10: return sqr ? k_fused<10, true, true> : k_fused<10, true, false>;
+    case 11: return sqr ? k_fused<11, true, true> : k_fused<11, true, false>;
+    case 12: return sqr ? k_fused<12, true, true> : k_fused<12, true, false>;
+    case 13: return sqr ? k_fused<13, true, true> : k_fused<13, true, false>;
+  }
+  return nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// optional per-kernel timing: HIP events recorded on the launch stream
+// around each kernel, with the kernel's algorithmic bytes per launch
+// ---------------------------------------------------------------------------
+#define PROF_MAX_REC 16384
+#define PROF_MAX_NAMES 32
+struct ProfRec { int name; double bytes; hipEvent_t e0, e1; };
+static int g_prof_on = 0;
+static ProfRec g_prof_rec[PROF_MAX_REC];
+static int g_prof_n = 0;
+static hipEvent_t g_prof_pool[2 * PROF_MAX_REC];
+static int g_prof_pool_n = 0;
+static const char *g_prof_names[PROF_MAX_NAMES];
+static int g_prof_nnames = 0;
+static double g_prof_ms[PROF_MAX_NAMES], g_prof_bytes[PROF_MAX_NAMES];
+static uint64_t g_prof_cnt[PROF_MAX_NAMES];
+
+static int prof_name(const char *n) {
+  for (int i = 0; i < g_prof_nnames; ++i)
+    if (!strcmp(g_prof_names[i], n)) return i;
+  if (g_prof_nnames == PROF_MAX_NAMES) return PROF_MAX_NAMES - 1;
+  g_prof_names[g_prof_nnames] = n;
+  return g_prof_nnames++;
+}
+static hipEvent_t prof_event(void) {
+  const int i = g_prof_pool_n++;
+  if (!g_prof_pool[i]) (void)hipEventCreate(&g_prof_pool[i]);
+  return g_prof_pool[i];
+}
+struct ProfScope {
+  int rec = -1;
+  hipStream_t st;
+  ProfScope(const char *name, double bytes, hipStream_t s) : st(s) {
+    if (!g_prof_on || g_prof_n >= PROF_MAX_REC) return;
+    rec = g_prof_n++;
+    ProfRec &r = g_prof_rec[rec];
+    r.name = prof_name(name);
+    r.bytes = bytes;
+    r.e0 = prof_event();
+    r.e1 = prof_event();
+    (void)hipEventRecord(r.e0, st);
+  }
+  ~ProfScope() {
+    if (rec >= 0) (void)hipEventRecord(g_prof_rec[rec].e1, st);
+  }
+};
+#define PROF(name, bytes) ProfScope prof_scope_##__LINE__(name, (double)(bytes), st)
+
+extern "C" void sba_gpu_profile_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_prof_on = on;
+}
+// fold recorded events (the caller has synchronized) into per-kernel totals
+extern "C" int sba_gpu_profile_collect(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (int i = 0; i < g_prof_n; ++i) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, g_prof_rec[i].e0, g_prof_rec[i].e1) != hipSuccess) continue;
+    const int k = g_prof_rec[i].name;
+    g_prof_ms[k] += ms;
+    g_prof_bytes[k] += g_prof_rec[i].bytes;
+    g_prof_cnt[k] += 1;
+  }
+  const int n = g_prof_n;
+  g_prof_n = 0;
+  g_prof_pool_n = 0;
+  return n;
+}
+extern "C" void sba_gpu_profile_reset(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (int k = 0; k < PROF_MAX_NAMES; ++k) g_prof_ms[k] = g_prof_bytes[k] = 0, g_prof_cnt[k] = 0;
+  g_prof_n = 0;
+  g_prof_pool_n = 0;
+}
+// kernel idx: name, total ms, total algorithmic bytes, launches; 0 past the end
+extern "C" int sba_gpu_profile_get(int idx, const char **name, double *ms, double *bytes,
+                                   uint64_t *count) {
+  if (idx < 0 || idx >= g_prof_nnames) return 0;
+  *name = g_prof_names[idx];
+  *ms = g_prof_ms[idx];
+  *bytes = g_prof_bytes[idx];
+  *count = g_prof_cnt[idx];
+  return 1;
+}
+
+#define LAST_ELL 13
+
+static int run_mul(DevCtx &c, uint64_t *rp, const uint64_t *ap, uint64_t an, const uint64_t *bp,
+                   uint64_t bn, int sqr, void *ws, hipStream_t st) {
+  const uint64_t zlen = an + bn - 1;
+  const uint64_t Z = an + bn;
+  const int L = lg_for(zlen);
+  const size_t N = (size_t)1 << L;
+  uint32_t *aspec = (uint32_t *)ws;
+  uint32_t *bspec = aspec + SBA_K * N;
+  uint64_t *w0 = (uint64_t *)(bspec + SBA_K * N);
+  uint64_t *w1 = w0 + N, *w2 = w1 + N;
+  const uint32_t nb = (uint32_t)((Z + CARRY_LPB - 1) / CARRY_LPB);
+  uint32_t *bmaps = (uint32_t *)(w2 + N);
+  uint32_t *bcin = bmaps + nb + 1;
+  const PrimeArgs &pa = c.pa;
+
+  const double KN4 = 4.0 * SBA_K * (double)N;
+  if (L <= LAST_ELL) {
+    if (!sqr) {
+      PROF("first_fwd", 8.0 * an + KN4);
+      hipLaunchKernelGGL(first_single(L), dim3(1), dim3(NT), 0, st, pa, ap, an, aspec, L);
+    }
+    {
+      PROF("fused_last", 8.0 * bn + (sqr ? 1 : 2) * KN4);
+      hipLaunchKernelGGL(fused_single(L, sqr), dim3(1, SBA_K), dim3(NT), 0, st, pa, bp, bn,
+                         (const uint32_t *)aspec, bspec, L);
+    }
+  } else {
+    const int ell1 = L - LAST_ELL;
+    const uint32_t ntile1 = (uint32_t)(N >> 13);
+    first_fn f1 = first_two_pass(ell1);
+    if (!sqr) {
+      PROF("first_fwd", 8.0 * an + KN4);
+      hipLaunchKernelGGL(f1, dim3(ntile1), dim3(NT), 0, st, pa, ap, an, aspec, L);
+    }
+    {
+      PROF("first_fwd", 8.0 * bn + KN4);
+      hipLaunchKernelGGL(f1, dim3(ntile1), dim3(NT), 0, st, pa, bp, bn, bspec, L);
+    }
+    const uint32_t ntile2 = (uint32_t)(N >> LAST_ELL);
+    {
+      PROF("fused_last", (sqr ? 2 : 3) * KN4);
+      if (sqr)
+        hipLaunchKernelGGL((k_fused<LAST_ELL, false, true>), dim3(ntile2, SBA_K), dim3(NT), 0,
+                           st, pa, (const uint64_t *)nullptr, (uint64_t)0,
+                           (const uint32_t *)aspec, bspec, L);
+      else
+        hipLaunchKernelGGL((k_fused<LAST_ELL, false, false>), dim3(ntile2, SBA_K), dim3(NT), 0,
+                           st, pa, (const uint64_t *)nullptr, (uint64_t)0,
+                           (const uint32_t *)aspec, bspec, L);
+    }
+    {
+      PROF("inv_first", 2 * KN4);
+      hipLaunchKernelGGL(inv_first_two_pass(ell1), dim3(ntile1, SBA_K), dim3(NT), 0, st, pa,
+                         bspec, L, 0);
+    }
+  }
+  GarnerArgs ga = c.ga_base;
+  for (int q = 0; q < SBA_K; ++q) {
+    const uint64_t p = kPrimes[q];
+    const uint64_t ninv = h_invm(N % p, p);
+    // scale = N^-1 * R^2 mod p
+    ga.scale[q] = (uint32_t)((unsigned __int128)ninv * (((unsigned __int128)1 << 64) % p) % p);
+  }
+  {
+    PROF("garner", (4.0 * SBA_K + 24.0) * zlen);
+    hipLaunchKernelGGL(k_garner, dim3((uint32_t)((zlen + NT - 1) / NT)), dim3(NT), 0, st, ga,
+                       (const uint32_t *)bspec, N, zlen, w0, w1, w2);
+  }
+  {
+    PROF("carry_maps", 24.0 * zlen);
+    hipLaunchKernelGGL(k_carry_maps, dim3(nb), dim3(NT), 0, st, (const uint64_t *)w0,
+                       (const uint64_t *)w1, (const uint64_t *)w2, zlen, Z, bmaps);
+  }
+  {
+    PROF("carry_blocks", 8.0 * nb);
+    hipLaunchKernelGGL(k_carry_blocks, dim3(1), dim3(NT), 0, st, (const uint32_t *)bmaps, nb,
+                       bcin);
+  }
+  {
+    PROF("carry_out", 24.0 * zlen + 8.0 * Z);
+    hipLaunchKernelGGL(k_carry_out, dim3(nb), dim3(NT), 0, st, (const uint64_t *)w0,
+                       (const uint64_t *)w1, (const uint64_t *)w2, zlen, Z,
+                       (const uint32_t *)bcin, rp);
+  }
+  HIPCHK(hipGetLastError());
+  return SBA_OK;
+}
+
+static int get_ctx(DevCtx **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= SBA_MAX_DEV) return SBA_ERR_HIP;
+  DevCtx &c = g_ctx[dev];
+  ctx_init(c);
+  *out = &c;
+  return 0;
+}
+
+static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64_t *bp,
+                      int64_t bn, int sqr, void *ws, size_t ws_bytes, void *stream) {
+  if (an <= 0 || bn <= 0) return SBA_BAND;
+  if (an < bn && !sqr) {
+    const uint64_t *t = ap; ap = bp; bp = t;
+    int64_t tn = an; an = bn; bn = tn;
+  }
+  const uint64_t mn = (uint64_t)(an < bn ? an : bn);
+  if (mn > sba_gpu_min_operand_cap()) return SBA_BAND;
+  const int L = lg_for((uint64_t)an + (uint64_t)bn - 1);
+  if (L > SBA_MAX_LG) return SBA_BAND;
+  hipStream_t st = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lk(g_mu);
+  DevCtx *c;
+  int rc = get_ctx(&c);
+  if (rc) return rc;
+  rc = ctx_twiddles(*c, L, st);
+  if (rc) return rc;
+  const size_t need = ws_need((size_t)1 << L);
+  if (!ws) {
+    if (c->ws_bytes < need) {
+      if (c->ws) HIPCHK(hipFree(c->ws));
+      c->ws = nullptr;
+      c->ws_bytes = 0;
+      HIPCHK(hipMalloc(&c->ws, need));
+      c->ws_bytes = need;
+    }
+    ws = c->ws;
+  } else if (ws_bytes < need) {
+    return SBA_ERR_WORKSPACE;
+  }
+  return run_mul(*c, rp, ap, (uint64_t)an, sqr ? ap : bp, (uint64_t)bn, sqr, ws, st);
+}
+
+extern "C" int sba_gpu_mul_dev(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64_t *bp,
+                               int64_t bn, void *stream) {
+  return mul_common(rp, ap, an, bp, bn, 0, nullptr, 0, stream);
+}
+
+extern "C" int sba_gpu_sqr_dev(uint64_t *rp, const uint64_t *ap, int64_t an, void *stream) {
+  return mul_common(rp, ap, an, ap, an, 1, nullptr, 0, stream);
+}
+
+extern "C" int sba_gpu_mul_dev_ws(uint64_t *rp, const uint64_t *ap, int64_t an,
+                                  const uint64_t *bp, int64_t bn, void *ws, size_t ws_bytes,
+                                  void *stream) {
+  if (!ws) return SBA_ERR_WORKSPACE;
+  return mul_common(rp, ap, an, bp, bn, 0, ws, ws_bytes, stream);
+}
+
+extern "C" int sba_gpu_sqr_dev_ws(uint64_t *rp, const uint64_t *ap, int64_t an, void *ws,
+                                  size_t ws_bytes, void *stream) {
+  if (!ws) return SBA_ERR_WORKSPACE;
+  return mul_common(rp, ap, an, ap, an, 1, ws, ws_bytes, stream);
+}
+
+extern "C" int sba_gpu_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}

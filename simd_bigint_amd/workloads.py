@@ -1,0 +1,126 @@
+"""Synthetic operands for tests and benchmarks.
+
+xorshift64 with shifts (13, 7, 17) -- the reference's generator
+(bench/nt_sweep.c:15-16, tests/p30_test.c:14-15). The state is a
+one-element list so A then B are drawn from one stream as the reference does
+(bench/nt_sweep.c:46). Large fills are vectorised exactly: xorshift64 is a
+linear map T on GF(2)^64, so 2^16 lanes seeded with T^1..T^65536 of the
+state are advanced together by the bit-matrix of T^65536.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_M = (1 << 64) - 1
+_LANES = 1 << 16
+_JUMP = None
+
+
+def _xs(x: int) -> int:
+    x ^= (x << 13) & _M
+    x ^= x >> 7
+    x ^= (x << 17) & _M
+    return x
+
+
+def _xs_vec(v: np.ndarray) -> np.ndarray:
+    v = v ^ (v << np.uint64(13))
+    v = v ^ (v >> np.uint64(7))
+    v = v ^ (v << np.uint64(17))
+    return v
+
+
+def _jump_matrix() -> np.ndarray:
+    """column b = T^_LANES (1 << b)."""
+    global _JUMP
+    if _JUMP is None:
+        cols = np.array([1 << b for b in range(64)], dtype=np.uint64)
+        for _ in range(_LANES):
+            cols = _xs_vec(cols)
+        _JUMP = cols
+    return _JUMP
+
+
+def _apply(mat: np.ndarray, v: np.ndarray) -> np.ndarray:
+    out = np.zeros_like(v)
+    one = np.uint64(1)
+    for b in range(64):
+        bit = (v >> np.uint64(b)) & one
+        out ^= (np.uint64(0) - bit) & mat[b]
+    return out
+
+
+def xorshift_fill(n: int, st: list) -> np.ndarray:
+    """n limbs from state st[0] (updated), identical to n calls of xr()."""
+    out = np.empty(n, dtype=np.uint64)
+    x = int(st[0])
+    if n <= 2 * _LANES:
+        for i in range(n):
+            x = _xs(x)
+            out[i] = x
+        st[0] = x
+        return out
+    head = np.empty(_LANES, dtype=np.uint64)
+    for j in range(_LANES):
+        x = _xs(x)
+        head[j] = x
+    mat = _jump_matrix()
+    cur = head
+    pos = 0
+    while pos < n:
+        take = min(_LANES, n - pos)
+        out[pos:pos + take] = cur[:take]
+        pos += take
+        if pos < n:
+            cur = _apply(mat, cur)
+    st[0] = int(out[-1])
+    return out
+
+
+def pattern_fill(n: int, pat: int, st: list) -> np.ndarray:
+    """pat 0: xorshift limbs; 1: all ones; 2: all ones every 32nd limb, zero
+    elsewhere (tests/p30_test.c:18-22)."""
+    if pat == 1:
+        return np.full(n, _M, dtype=np.uint64)
+    if pat == 2:
+        a = np.zeros(n, dtype=np.uint64)
+        a[::32] = np.uint64(_M)
+        return a
+    return xorshift_fill(n, st)
+
+
+def rlimb64_fill(n: int, st: list) -> np.ndarray:
+    """biased limbs of tests/mpn_mul_u64_test.cpp:19-27: xr() & 7 picks 0,
+    ~0, 1, 2^63 or a fresh xr() limb."""
+    out = np.empty(n, dtype=np.uint64)
+    x = int(st[0])
+    for i in range(n):
+        x = _xs(x)
+        k = x & 7
+        if k == 0:
+            v = 0
+        elif k == 1:
+            v = _M
+        elif k == 2:
+            v = 1
+        elif k == 3:
+            v = 1 << 63
+        else:
+            x = _xs(x)
+            v = x
+        out[i] = v
+    st[0] = x
+    return out
+
+
+def to_int(a) -> int:
+    return int.from_bytes(np.ascontiguousarray(a, dtype=np.uint64).tobytes(), "little")
+
+
+def from_int(v: int, n: int) -> np.ndarray:
+    return np.frombuffer(v.to_bytes(8 * n, "little"), dtype=np.uint64).copy()
+
+
+def all_ones_product(an: int, bn: int) -> np.ndarray:
+    """(2^64a - 1)(2^64b - 1) = 2^64(a+b) - 2^64a - 2^64b + 1, as limbs."""
+    return from_int(((1 << (64 * an)) - 1) * ((1 << (64 * bn)) - 1), an + bn)

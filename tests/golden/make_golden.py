@@ -1,0 +1,176 @@
+"""Writes tests/golden/golden.json from the reference compiled from source.
+
+Run in the build container (where the reference sources are present):
+    make -C oracle && python tests/golden/make_golden.py
+The reference is built by oracle/Makefile into oracle/_ref/libsbref.so from
+the reference's own headers (include/ntmul.h, include/u64cvt.h,
+include/dc_div.h) and assembly (src/x86_64/*basecase_le6.S,
+src/avx512/recip_3by2_seed.s). Only data is written: the generator's
+seeds/shapes (inputs are regenerated by the tests with the same xorshift64
+generator, bench/nt_sweep.c:15-16) and the reference's outputs -- full limb
+vectors for small shapes, SHA-256 plus sampled limbs for large ones.
+"""
+import ctypes
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
+sys.path.insert(0, ROOT)
+from simd_bigint_amd.workloads import xorshift_fill, rlimb64_fill, pattern_fill  # noqa: E402
+
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C")
+
+
+def load_ref():
+    L = ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", "libsbref.so"))
+    for f in ("sbref_nt_mul", "sbref_p30_mul", "sbref_p50_mul"):
+        getattr(L, f).argtypes = [u64p, u64p, ctypes.c_ssize_t, u64p, ctypes.c_ssize_t]
+        getattr(L, f).restype = ctypes.c_int
+    L.sbref_nt_sqr.argtypes = [u64p, u64p, ctypes.c_ssize_t]
+    L.sbref_nt_sqr.restype = ctypes.c_int
+    L.sbref_simd_mpn_mul.argtypes = [u64p, u64p, ctypes.c_uint64, u64p, ctypes.c_uint64]
+    L.sbref_simd_mpn_mul.restype = None
+    for f in ("sbref_divrem_u64", "sbref_divrem_u64_dc"):
+        getattr(L, f).argtypes = [u64p, u64p, u64p, ctypes.c_uint64, u64p, ctypes.c_uint64,
+                                  ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
+        getattr(L, f).restype = None
+    return L
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.uint64).tobytes()).hexdigest()
+
+
+def samples(r):
+    n = len(r)
+    idx = sorted(set([0, 1, 2, n // 3, n // 2, n - 3, n - 2, n - 1]))
+    return {str(i): "%016x" % int(r[i]) for i in idx if 0 <= i < n}
+
+
+def hexlist(a):
+    return ["%016x" % int(x) for x in a]
+
+
+def main():
+    R = load_ref()
+    out = {"generator": "xorshift64 (13,7,17): A filled first, then B from the same state",
+           "small_mul": [], "large_mul": [], "small_mpn": [], "divrem": []}
+
+    def nt(a, b):
+        r = np.zeros(len(a) + len(b), np.uint64)
+        ok = R.sbref_nt_mul(r, a, len(a), b, len(b))
+        assert ok == 1
+        return r
+
+    def sq(a):
+        r = np.zeros(2 * len(a), np.uint64)
+        assert R.sbref_nt_sqr(r, a, len(a)) == 1
+        return r
+
+    # small shapes: full vectors (tests/nt_test.c:67-70 grid, tests/p30_test.c:18-22 patterns)
+    shapes = [(an, bn) for an in range(1, 9) for bn in range(1, 9)]
+    shapes += [(24, 24), (40, 40), (100, 37), (37, 100), (300, 3)]
+    for pat in (0, 1, 2):
+        for (an, bn) in shapes:
+            st = [42 + an * 1000 + bn]
+            a = pattern_fill(an, pat, st)
+            b = pattern_fill(bn, pat, st)
+            r = nt(a, b)
+            out["small_mul"].append({"an": an, "bn": bn, "pat": pat, "seed": 42 + an * 1000 + bn,
+                                     "r": hexlist(r)})
+    # squares
+    for an in (1, 2, 3, 7, 40, 100):
+        for pat in (0, 1, 2):
+            st = [43 + an]
+            a = pattern_fill(an, pat, st)
+            out["small_mul"].append({"an": an, "bn": 0, "pat": pat, "seed": 43 + an, "sqr": 1,
+                                     "r": hexlist(sq(a))})
+
+    # large configs: SHA-256 + samples
+    large = [
+        ("mid_513x512", 513, 512, 47, 0),
+        ("mid_2048_ones", 2048, 2048, 0, 1),
+        ("mid_2048_sqr", 2048, 0, 48, 0),
+        ("cfg1_16384", 16384, 16384, 42, 0),
+        ("cfg1_16384_sqr", 16384, 0, 42, 0),
+        ("metric_4Mi", 1 << 22, 1 << 22, 42, 0),
+        ("cfg2_3x2^20", 3 << 20, 3 << 20, 43, 0),
+        ("cfg2_3x2^20_ones", 3 << 20, 3 << 20, 0, 1),
+        ("cfg2_sqr_2^22", 1 << 22, 0, 43, 0),
+        ("cfg3_2^23x2^18", 1 << 23, 1 << 18, 44, 0),
+        ("p30cap_plus1_ones", 2344426, 2344426, 0, 1),
+        ("half_2^19_ones", 1 << 19, 1 << 19, 0, 1),
+        ("seam_lo", 180224 // 2 - 1, 180224 - 180224 // 2 + 2, 45, 0),
+        ("unbal_2^22m5_x_2^21", (1 << 22) - 5, 1 << 21, 46, 0),
+        ("sparse_65536", 65536, 65536, 0, 2),
+    ]
+    for name, an, bn, seed, pat in large:
+        st = [seed]
+        a = pattern_fill(an, pat, st)
+        if bn == 0:
+            r = sq(a)
+        else:
+            b = pattern_fill(bn, pat, st)
+            r = nt(a, b)
+        out["large_mul"].append({"name": name, "an": an, "bn": bn, "seed": seed, "pat": pat,
+                                 "sqr": int(bn == 0), "sha256": sha(r), "samples": samples(r)})
+        print(name, "done", flush=True)
+
+    # public entry simd_mpn_mul: tests/mpn_mul_u64_test.cpp:18-27 biased limbs
+    st = [0xfeedfacefeedface]
+    for an in range(1, 11):
+        for bn in range(1, an + 1):
+            a = rlimb64_fill(an, st)
+            b = rlimb64_fill(bn, st)
+            r = np.zeros(an + bn, np.uint64)
+            R.sbref_simd_mpn_mul(r, a, an, b, bn)
+            out["small_mpn"].append({"an": an, "bn": bn, "a": hexlist(a), "b": hexlist(b),
+                                     "r": hexlist(r)})
+    for an, bn in ((26, 13), (130, 65), (400, 400), (1300, 800), (620, 90)):
+        a = rlimb64_fill(an, st)
+        b = rlimb64_fill(bn, st)
+        r = np.zeros(an + bn, np.uint64)
+        R.sbref_simd_mpn_mul(r, a, an, b, bn)
+        out["small_mpn"].append({"an": an, "bn": bn, "a": hexlist(a), "b": hexlist(b),
+                                 "r": hexlist(r)})
+
+    # division (reference requires bitlen(D) > 416: dn >= 7 with top bit set)
+    def dr(nn, dn, seed, fn, full):
+        st = [seed]
+        n = xorshift_fill(nn, st)
+        d = xorshift_fill(dn, st)
+        d[-1] |= np.uint64(1 << 63)
+        q = np.zeros(max(nn - dn + 1, 1), np.uint64)
+        r = np.zeros(dn, np.uint64)
+        qn = ctypes.c_uint64()
+        rn = ctypes.c_uint64()
+        getattr(R, fn)(q, r, n, nn, d, dn, ctypes.byref(qn), ctypes.byref(rn))
+        q = q[: qn.value]
+        r = r[: rn.value]
+        e = {"nn": nn, "dn": dn, "seed": seed, "fn": fn, "qn": qn.value, "rn": rn.value}
+        if full:
+            e["q"] = hexlist(q)
+            e["r"] = hexlist(r)
+        else:
+            e["q_sha256"] = sha(q)
+            e["r_sha256"] = sha(r)
+        out["divrem"].append(e)
+
+    for nn, dn in ((14, 7), (20, 7), (40, 13), (100, 50), (300, 64), (513, 256)):
+        dr(nn, dn, 50 + nn, "sbref_divrem_u64", True)
+        dr(nn, dn, 50 + nn, "sbref_divrem_u64_dc", True)
+    dr(1 << 14, 1 << 13, 51, "sbref_divrem_u64_dc", False)
+    dr(1 << 21, 1 << 20, 44, "sbref_divrem_u64_dc", False)
+    print("divrem done", flush=True)
+
+    with open(os.path.join(HERE, "golden.json"), "w") as f:
+        json.dump(out, f, indent=0, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

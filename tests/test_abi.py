@@ -1,0 +1,69 @@
+"""The C-ABI library loads and exports every symbol include/*.h declares;
+band and size queries answer without a GPU (CPU only, no compute calls)."""
+import glob
+import os
+import re
+import subprocess
+
+import pytest
+
+import simd_bigint_amd as sba
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+PRIMES = [1107296257, 1711276033, 1811939329, 2013265921, 2113929217]
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        txt = open(h).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \t\*]*?\b([A-Za-z_]\w*)\s*\(", txt, flags=re.M):
+            name = m.group(1)
+            if name not in ("defined",):
+                names.add(name)
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    L = sba.lib()
+    names = declared_functions()
+    assert "nt_mul" in names and "divrem_u64" in names and "simd_mpn_mul" in names
+    out = subprocess.run(["nm", "-D", "--defined-only", sba.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    for n in names:
+        getattr(L, n)
+
+
+def test_band_queries():
+    P = 1
+    for p in PRIMES:
+        P *= p
+    F = (2**64 - 1) ** 2
+    assert sba.min_operand_cap() == P // F
+    assert sba.max_lg() == 25
+    for p in PRIMES:
+        assert (p - 1) % (1 << 25) == 0
+    assert PRIMES == sorted(PRIMES)
+    L = sba.lib()
+    assert L.sba_gpu_workspace_bytes(0, 5) == 0
+    assert L.sba_gpu_workspace_bytes(1 << 22, 1 << 22) > 0
+    assert L.sba_gpu_workspace_bytes(1 << 25, 1 << 25) == 0      # beyond 2^25 coefficients
+
+
+def test_out_of_band_returns_zero_without_device():
+    import numpy as np
+    a = np.ones(4, np.uint64)
+    e = np.zeros(0, np.uint64)
+    L = sba.lib()
+    r = np.zeros(8, np.uint64)
+    p = sba._arr
+    assert L.nt_mul(p(r), p(a), 0, p(a), 4) == 0
+    assert L.nt_mul(p(r), p(a), 4, p(a), -1) == 0
+    assert L.nt_sqr(p(r), p(a), 0) == 0
+    # p30 band (include/p30.h:1851-1858): min operand cap 2,344,425
+    assert L.p30_mul(p(r), p(a), 2344426, p(a), 2344426) == 0
+    del e
