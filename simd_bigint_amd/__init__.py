@@ -44,6 +44,14 @@ def lib() -> ctypes.CDLL:
         raise ImportError(
             f"simd_bigint_amd: {LIB_PATH} is missing; build it first "
             "(python -c 'import __graft_entry__ as g; g.build()')")
+    # One HIP runtime per process: when torch (ROCm build) is importable, load
+    # it first so this library binds to the same libamdhip64.so.7 instance as
+    # torch's device tensors and streams (both carry that SONAME; loading ours
+    # first would leave torch with a second runtime that sees no GPU).
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     sp = ctypes.c_ssize_t
     u64 = ctypes.c_uint64
@@ -171,13 +179,20 @@ def _ptr(t) -> int:
     return int(t.data_ptr())
 
 
+def _check_dev(r, need):
+    if not r.is_contiguous() or r.numel() < need:
+        raise ValueError("result tensor must be contiguous with >= %d limbs" % need)
+
+
 def mul_dev(r, a, b, stream=None) -> int:
     """Enqueue r = a * b on device tensors (uint64/int64 limbs)."""
+    _check_dev(r, a.numel() + b.numel())
     s = None if stream is None else ctypes.c_void_p(int(stream.cuda_stream))
     return lib().sba_gpu_mul_dev(_ptr(r), _ptr(a), a.numel(), _ptr(b), b.numel(), s)
 
 
 def sqr_dev(r, a, stream=None) -> int:
+    _check_dev(r, 2 * a.numel())
     s = None if stream is None else ctypes.c_void_p(int(stream.cuda_stream))
     return lib().sba_gpu_sqr_dev(_ptr(r), _ptr(a), a.numel(), s)
 

@@ -50,8 +50,15 @@ static void grow(uint64_t **p, size_t *cap, size_t n) {
 }
 
 /* H2D, multiply on the default stream, D2H. Returns the engine's code. */
+static int gpu_in_band(size_t an, size_t bn) {
+  uint64_t mn = an < bn ? an : bn;
+  return an > 0 && bn > 0 && mn <= sba_gpu_min_operand_cap() &&
+         (uint64_t)an + bn - 1 <= ((uint64_t)1 << sba_gpu_max_lg());
+}
+
 static int gpu_mul_host(uint64_t *rp, const uint64_t *ap, size_t an,
                         const uint64_t *bp, size_t bn, int sqr) {
+  if (!gpu_in_band(an, bn)) return SBA_BAND;   /* before touching any buffer */
   if (!device_ok()) die("device lookup (no HIP device)", SBA_ERR_NODEV);
   pthread_mutex_lock(&g_stage_mu);
   grow(&g_da, &g_na, an);

@@ -835,6 +835,11 @@ static int run_mul(DevCtx &c, uint64_t *rp, const uint64_t *ap, uint64_t an, con
       hipLaunchKernelGGL(f1, dim3(ntile1), dim3(NT), 0, st, pa, bp, bn, bspec, L);
     }
     const uint32_t ntile2 = (uint32_t)(N >> LAST_ELL);
+    if (!sqr) {
+      PROF("fwd_last", 2 * KN4);
+      hipLaunchKernelGGL((k_pass<LAST_ELL, 0, false>), dim3(ntile2, SBA_K), dim3(NT), 0, st, pa,
+                         aspec, L, L - LAST_ELL);
+    }
     {
       PROF("fused_last", (sqr ? 2 : 3) * KN4);
       if (sqr)

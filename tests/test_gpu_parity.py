@@ -118,13 +118,20 @@ def test_sparse_and_zero_operands():
 def test_band_edges():
     cap = sba.min_operand_cap()
     assert cap > (1 << 25)
-    a = np.ones(3, np.uint64)
-    # beyond 2^25 coefficients: band rejection, nothing launched
-    big = np.zeros(1, np.uint64)
     L = sba.lib()
+    a = np.ones(3, np.uint64)
     r = np.zeros(4, np.uint64)
-    assert L.nt_mul(sba._arr(r), sba._arr(a), (1 << 24) + 1, sba._arr(a), 1 << 24) == 0
-    del big
+    # beyond 2^25 coefficients: rejected before any buffer is read
+    assert L.nt_mul(sba._arr(r), sba._arr(a), (1 << 24) + 2, sba._arr(a), 1 << 24) == 0
+    # exactly 2^25 coefficients is in band (largest transform)
+    st = [3]
+    x = xorshift_fill(1 << 24, st)
+    y = xorshift_fill((1 << 24) + 1, st)
+    ok, r = sba.nt_mul(x, y)
+    assert ok == 1
+    qs = [(1 << 61) - 1]
+    for q in qs:
+        assert orc.fingerprint(r, q) == orc.fingerprint(x, q) * orc.fingerprint(y, q) % q
 
 
 def test_public_entry_default_dispatch():
@@ -181,7 +188,8 @@ def test_device_resident_matches_host():
         torch.cuda.synchronize()
         ok, r = sba.nt_mul(a, b)
         assert ok == 1 and (tr.cpu().numpy().view(np.uint64) == r).all()
-        assert sba.sqr_dev(tr[: 2 * an], ta) == 1
+        ts = torch.empty(2 * an, dtype=torch.int64, device="cuda")
+        assert sba.sqr_dev(ts, ta) == 1
         torch.cuda.synchronize()
         ok, r = sba.nt_sqr(a)
-        assert (tr[: 2 * an].cpu().numpy().view(np.uint64) == r).all()
+        assert (ts.cpu().numpy().view(np.uint64) == r).all()
