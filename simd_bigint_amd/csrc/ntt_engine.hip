@@ -73,14 +73,28 @@ __device__ __forceinline__ uint32_t limb_residue(uint64_t x, uint32_t p, uint32_
 
 // ---------------------------------------------------------------------------
 // one chunk of B levels (local levels T0 .. T0+B-1 of an ELL-level pass)
+//
+// LDS index of tile element (k, c) is (k << LOGTC) | c, padded by one word
+// per 32 (lpad) against bank conflicts. A chunk's group of 2^B elements has
+// stride D = 2^(LB + LOGTC) words; when D >= 32, or when the whole group
+// sits inside one 32-word row (D * 2^B <= 32), the padded addresses are
+// base + m * const, so one address register serves the group (ds offsets).
 // ---------------------------------------------------------------------------
+template <int ELL, int LOGTC, int T0, int B> struct ChunkGeom {
+  static constexpr int LB = ELL - T0 - B;
+  static constexpr int LOGD = LB + LOGTC;
+  static constexpr bool LINEAR = LOGD >= 5 || LOGD + B <= 5;
+  static constexpr int DP = LOGD >= 5 ? (1 << LOGD) + (1 << (LOGD - 5)) : (1 << LOGD);
+};
+
 template <int ELL, int LOGTC, int T0, int B, bool INV>
 __device__ __forceinline__ void tile_chunk(uint32_t *lds, const uint32_t *__restrict__ tw,
                                            uint32_t H, uint32_t p, uint32_t pinv) {
   constexpr int TC = 1 << LOGTC;
   constexpr int NE = 1 << (ELL + LOGTC);
   constexpr int GPT = NE / ((1 << B) * NT);
-  constexpr int LB = ELL - T0 - B;
+  using G = ChunkGeom<ELL, LOGTC, T0, B>;
+  constexpr int LB = G::LB;
   static_assert(GPT >= 1, "chunk too wide for the tile");
   __syncthreads();
 #pragma unroll
@@ -92,10 +106,12 @@ __device__ __forceinline__ void tile_chunk(uint32_t *lds, const uint32_t *__rest
     const uint32_t hb = rest >> LB;
     const uint32_t kbase = (hb << (ELL - T0)) | lo;
     const uint32_t hj = (H << T0) | hb;
+    const uint32_t i0 = (kbase << LOGTC) | c;
+    uint32_t *l0 = lds + lpad(i0);
     uint32_t v[1 << B];
 #pragma unroll
     for (int m = 0; m < (1 << B); ++m)
-      v[m] = lds[lpad(((kbase | ((uint32_t)m << LB)) << LOGTC) | c)];
+      v[m] = G::LINEAR ? l0[m * G::DP] : lds[lpad(i0 + ((uint32_t)m << G::LOGD))];
     if (!INV) {
 #pragma unroll
       for (int u = 0; u < B; ++u) {
@@ -126,23 +142,35 @@ __device__ __forceinline__ void tile_chunk(uint32_t *lds, const uint32_t *__rest
       }
     }
 #pragma unroll
-    for (int m = 0; m < (1 << B); ++m)
-      lds[lpad(((kbase | ((uint32_t)m << LB)) << LOGTC) | c)] = v[m];
+    for (int m = 0; m < (1 << B); ++m) {
+      if (G::LINEAR) l0[m * G::DP] = v[m];
+      else lds[lpad(i0 + ((uint32_t)m << G::LOGD))] = v[m];
+    }
   }
 }
 
+// chunk sizes: the largest B <= BM whose group addressing is linear,
+// falling back to BM when no size is
 template <int ELL, int LOGTC> struct ChunkPlan {
   static constexpr int E = (1 << (ELL + LOGTC)) / NT;
   static constexpr int LE = ILog2<E>::v;
   static constexpr int BM = LE < 5 ? LE : 5;
+  static constexpr int pick(int t0) {
+    int r = ELL - t0;
+    int bmax = r < BM ? r : BM;
+    for (int b = bmax; b >= 1; --b) {
+      int logd = ELL - t0 - b + LOGTC;
+      if (logd >= 5 || logd + b <= 5) return b;
+    }
+    return bmax;
+  }
 };
 
 template <int ELL, int LOGTC, int T0>
 __device__ __forceinline__ void tile_fwd(uint32_t *lds, const uint32_t *tw, uint32_t H,
                                          uint32_t p, uint32_t pinv) {
   if constexpr (T0 < ELL) {
-    constexpr int BM = ChunkPlan<ELL, LOGTC>::BM;
-    constexpr int B = (ELL - T0) < BM ? (ELL - T0) : BM;
+    constexpr int B = ChunkPlan<ELL, LOGTC>::pick(T0);
     tile_chunk<ELL, LOGTC, T0, B, false>(lds, tw, H, p, pinv);
     tile_fwd<ELL, LOGTC, T0 + B>(lds, tw, H, p, pinv);
   }
@@ -152,8 +180,7 @@ template <int ELL, int LOGTC, int T0>
 __device__ __forceinline__ void tile_inv(uint32_t *lds, const uint32_t *tw, uint32_t H,
                                          uint32_t p, uint32_t pinv) {
   if constexpr (T0 < ELL) {
-    constexpr int BM = ChunkPlan<ELL, LOGTC>::BM;
-    constexpr int B = (ELL - T0) < BM ? (ELL - T0) : BM;
+    constexpr int B = ChunkPlan<ELL, LOGTC>::pick(T0);
     tile_inv<ELL, LOGTC, T0 + B>(lds, tw, H, p, pinv);
     tile_chunk<ELL, LOGTC, T0, B, true>(lds, tw, H, p, pinv);
   }
@@ -172,25 +199,22 @@ __global__ __launch_bounds__(NT) void k_first_fwd(PrimeArgs pa, const uint64_t *
   constexpr int NE = 1 << (ELL + LOGTC);
   constexpr int E = NE / NT;
   __shared__ uint32_t lds[NE + NE / 32];
-  const size_t N = (size_t)1 << L;
+  const uint32_t N = 1u << L;             // N <= 2^25: 32-bit offsets throughout
   const uint32_t S = 1u << (L - ELL);
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t col0 = tile << LOGTC;
-  uint64_t x[E];
-#pragma unroll
-  for (int i = 0; i < E; ++i) {
-    const uint32_t e = threadIdx.x + i * NT;
-    const size_t gidx = (size_t)(e >> LOGTC) * S + col0 + (e & (TC - 1));
-    x[i] = gidx < nlimbs ? limbs[gidx] : 0;
-  }
+  const uint32_t nl = nlimbs < N ? (uint32_t)nlimbs : N;
+  uint32_t *lt = lds + lpad(threadIdx.x);
 #pragma unroll 1
   for (int q = 0; q < SBA_K; ++q) {
     const uint32_t p = pa.p[q], pinv = pa.pinv[q], r2 = pa.r2[q];
     __syncthreads();
+    // the tile's limbs are re-read per prime (L2-resident after the first)
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const uint32_t e = threadIdx.x + i * NT;
-      lds[lpad(e)] = limb_residue(x[i], p, pinv, r2);
+      const uint32_t gidx = (e >> LOGTC) * S + col0 + (e & (TC - 1));
+      lt[i * (NT + NT / 32)] = limb_residue(gidx < nl ? limbs[gidx] : 0, p, pinv, r2);
     }
     tile_fwd<ELL, LOGTC, 0>(lds, pa.wn[q], 0, p, pinv);
     __syncthreads();
@@ -198,8 +222,8 @@ __global__ __launch_bounds__(NT) void k_first_fwd(PrimeArgs pa, const uint64_t *
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const uint32_t e = threadIdx.x + i * NT;
-      const size_t gidx = (size_t)(e >> LOGTC) * S + col0 + (e & (TC - 1));
-      out[gidx] = lds[lpad(e)];
+      const uint32_t gidx = (e >> LOGTC) * S + col0 + (e & (TC - 1));
+      out[gidx] = lt[i * (NT + NT / 32)];
     }
   }
 }
@@ -214,19 +238,20 @@ __global__ __launch_bounds__(NT) void k_pass(PrimeArgs pa, uint32_t *__restrict_
   constexpr int E = NE / NT;
   __shared__ uint32_t lds[NE + NE / 32];
   const int q = blockIdx.y;
-  const size_t N = (size_t)1 << L;
+  const uint32_t N = 1u << L;
   const uint32_t S = 1u << (L - s0 - ELL);
   const uint32_t ctiles = S >> LOGTC;
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t H = tile / ctiles;
   const uint32_t col0 = (tile % ctiles) << LOGTC;
-  const size_t base = ((size_t)H << (L - s0)) + col0;
+  const uint32_t base = (H << (L - s0)) + col0;
   uint32_t *buf = spec + q * N;
   const uint32_t p = pa.p[q], pinv = pa.pinv[q];
+  uint32_t *lt = lds + lpad(threadIdx.x);
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const uint32_t e = threadIdx.x + i * NT;
-    lds[lpad(e)] = buf[base + (size_t)(e >> LOGTC) * S + (e & (TC - 1))];
+    lt[i * (NT + NT / 32)] = buf[base + (e >> LOGTC) * S + (e & (TC - 1))];
   }
   if (!INV) tile_fwd<ELL, LOGTC, 0>(lds, pa.wn[q], H, p, pinv);
   else tile_inv<ELL, LOGTC, 0>(lds, pa.wni[q], H, p, pinv);
@@ -234,7 +259,7 @@ __global__ __launch_bounds__(NT) void k_pass(PrimeArgs pa, uint32_t *__restrict_
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const uint32_t e = threadIdx.x + i * NT;
-    buf[base + (size_t)(e >> LOGTC) * S + (e & (TC - 1))] = lds[lpad(e)];
+    buf[base + (e >> LOGTC) * S + (e & (TC - 1))] = lt[i * (NT + NT / 32)];
   }
 }
 
@@ -249,51 +274,46 @@ __global__ __launch_bounds__(NT) void k_fused(PrimeArgs pa, const uint64_t *__re
   constexpr int E = NE / NT;
   __shared__ uint32_t lds[NE + NE / 32];
   const int q = blockIdx.y;
-  const size_t N = (size_t)1 << L;
+  const uint32_t N = 1u << L;
   const uint32_t H = blockIdx.x;
-  const size_t base = (size_t)H << ELL;
+  const uint32_t base = H << ELL;
+  const uint32_t nb = bn < N ? (uint32_t)bn : N;
   const uint32_t p = pa.p[q], pinv = pa.pinv[q];
   uint32_t *buf = bspec + q * N;
+  uint32_t *lt = lds + lpad(threadIdx.x);
+  const uint32_t tb = base + threadIdx.x;
 #pragma unroll
   for (int i = 0; i < E; ++i) {
-    const uint32_t e = threadIdx.x + i * NT;
     uint32_t v;
     if (LIMB_IN) {
-      const size_t gidx = base + e;
-      v = limb_residue(gidx < bn ? blimbs[gidx] : 0, p, pinv, pa.r2[q]);
+      const uint32_t gidx = tb + i * NT;
+      v = limb_residue(gidx < nb ? blimbs[gidx] : 0, p, pinv, pa.r2[q]);
     } else {
-      v = buf[base + e];
+      v = buf[tb + i * NT];
     }
-    lds[lpad(e)] = v;
+    lt[i * (NT + NT / 32)] = v;
   }
   tile_fwd<ELL, 0, 0>(lds, pa.wn[q], H, p, pinv);
   __syncthreads();
   const uint32_t *as = aspec + q * N;
 #pragma unroll
   for (int i = 0; i < E; ++i) {
-    const uint32_t e = threadIdx.x + i * NT;
-    const uint32_t v = lds[lpad(e)];
-    const uint32_t o = SQR ? red1(v, p) : red1(as[base + e], p);
-    lds[lpad(e)] = mont_mul(v, o, p, pinv);
+    const uint32_t v = lt[i * (NT + NT / 32)];
+    const uint32_t o = SQR ? red1(v, p) : red1(as[tb + i * NT], p);
+    lt[i * (NT + NT / 32)] = mont_mul(v, o, p, pinv);
   }
   tile_inv<ELL, 0, 0>(lds, pa.wni[q], H, p, pinv);
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < E; ++i) {
-    const uint32_t e = threadIdx.x + i * NT;
-    buf[base + e] = lds[lpad(e)];
-  }
+  for (int i = 0; i < E; ++i) buf[tb + i * NT] = lt[i * (NT + NT / 32)];
 }
 
 // ---------------------------------------------------------------------------
-// Garner: five residues -> coefficient as 3 u64 words (SoA: w0, w1, w2)
+// Garner: the five residues of coefficient i -> 3 u64 words
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_garner(GarnerArgs ga, const uint32_t *__restrict__ spec,
-                                               size_t N, uint64_t zlen, uint64_t *__restrict__ w0,
-                                               uint64_t *__restrict__ w1,
-                                               uint64_t *__restrict__ w2) {
-  const uint64_t i = (uint64_t)blockIdx.x * NT + threadIdx.x;
-  if (i >= zlen) return;
+__device__ __forceinline__ void garner_coef(const GarnerArgs &ga, const uint32_t *__restrict__ spec,
+                                            uint32_t N, uint32_t i, uint64_t &c0, uint64_t &c1,
+                                            uint64_t &c2) {
   uint32_t u[SBA_K], x[SBA_K];
 #pragma unroll
   for (int q = 0; q < SBA_K; ++q)
@@ -319,19 +339,22 @@ __global__ __launch_bounds__(NT) void k_garner(GarnerArgs ga, const uint32_t *__
   const unsigned __int128 t0 = (unsigned __int128)(uint64_t)c * ga.p[0] + x[0];
   const unsigned __int128 t1 =
       (unsigned __int128)(uint64_t)(c >> 64) * ga.p[0] + (uint64_t)(t0 >> 64);
-  w0[i] = (uint64_t)t0;
-  w1[i] = (uint64_t)t1;
-  w2[i] = (uint64_t)(t1 >> 64);
+  c0 = (uint64_t)t0;
+  c1 = (uint64_t)t1;
+  c2 = (uint64_t)(t1 >> 64);
 }
 
 // ---------------------------------------------------------------------------
-// carry propagation. Limb i receives s_i = w0[i] + w1[i-1] + w2[i-2]
-// (< 3 * 2^64); with carry c in {0..3}: out_i = (s_i + c) mod 2^64,
-// c' = floor((s_i + c) / 2^64) <= 3. A chunk of limbs is a map
-// {0..3} -> {0..3} (2 bits per entry, 8 bits); maps compose
-// associatively, so the carries are an exclusive scan of maps.
+// carry propagation. Limb j receives s_j = w0[j] + w1[j-1] + w2[j-2]
+// (< 3 * 2^64) = v_j + 2^64 h_j. With incoming carry c in {0..3}:
+// out_j = (v_j + c) mod 2^64 and c' = h_j + [c >= 2^64 - v_j] <= 3. A limb
+// is thus a map {0..3} -> {0..3} (2 bits per entry, 8 bits); maps compose
+// associatively, so the carries are an exclusive scan of maps:
+//   k_garner_maps  Garner + limb sums + limb maps + block maps (LDS-staged)
+//   k_carry_blocks scan of block maps -> carry into each block
+//   k_carry_fix    per-limb carries from the maps, out_j = v_j + c_j
 // ---------------------------------------------------------------------------
-#define CARRY_LPT 8                     // limbs per thread
+#define CARRY_LPT 8                     // limbs per thread in the map scan
 #define CARRY_LPB (CARRY_LPT * NT)      // limbs per block
 #define MAP_ID 0xE4u                    // c -> c
 
@@ -343,35 +366,12 @@ __device__ __forceinline__ uint32_t map_then(uint32_t f, uint32_t g) {
   for (uint32_t c = 0; c < 4; ++c) r |= map_apply(g, map_apply(f, c)) << (2 * c);
   return r;
 }
-
-__device__ __forceinline__ void limb_sum(const uint64_t *__restrict__ w0,
-                                         const uint64_t *__restrict__ w1,
-                                         const uint64_t *__restrict__ w2, uint64_t zlen,
-                                         uint64_t i, uint64_t &v, uint32_t &h) {
-  const uint64_t a = i < zlen ? w0[i] : 0;
-  const uint64_t b = (i >= 1 && i - 1 < zlen) ? w1[i - 1] : 0;
-  const uint64_t c = (i >= 2 && i - 2 < zlen) ? w2[i - 2] : 0;
-  const uint64_t s = a + b;
-  uint32_t hh = s < a;
-  const uint64_t s2 = s + c;
-  hh += s2 < s;
-  v = s2;
-  h = hh;
-}
-
-// thread's chunk map (also returns the chunk's sums for reuse)
-__device__ __forceinline__ uint32_t chunk_map(const uint64_t *v, const uint32_t *h) {
+__device__ __forceinline__ uint32_t limb_map(uint64_t v, uint32_t h) {
+  const uint64_t room = ~v;                       // 2^64 - 1 - v
+  const uint32_t z = room < 3 ? (uint32_t)room + 1 : 4u;   // overflow iff c >= z
   uint32_t f = 0;
 #pragma unroll
-  for (uint32_t c0 = 0; c0 < 4; ++c0) {
-    uint32_t c = c0;
-#pragma unroll
-    for (int k = 0; k < CARRY_LPT; ++k) {
-      const uint64_t t = v[k] + c;
-      c = h[k] + (t < v[k] ? 1u : 0u);
-    }
-    f |= c << (2 * c0);
-  }
+  for (uint32_t c = 0; c < 4; ++c) f |= (h + (c >= z ? 1u : 0u)) << (2 * c);
   return f;
 }
 
@@ -399,21 +399,60 @@ __device__ uint32_t block_scan_maps(uint32_t f, uint32_t *total) {
   return map_then(pre, exc);
 }
 
-__global__ __launch_bounds__(NT) void k_carry_maps(const uint64_t *__restrict__ w0,
-                                                   const uint64_t *__restrict__ w1,
-                                                   const uint64_t *__restrict__ w2,
-                                                   uint64_t zlen, uint64_t Z,
-                                                   uint32_t *__restrict__ bmaps) {
-  const uint64_t i0 = (uint64_t)blockIdx.x * CARRY_LPB + (uint64_t)threadIdx.x * CARRY_LPT;
-  uint64_t v[CARRY_LPT];
-  uint32_t h[CARRY_LPT];
+__global__ __launch_bounds__(NT) void k_garner_maps(GarnerArgs ga, const uint32_t *__restrict__ spec,
+                                                    uint32_t N, uint32_t zlen, uint32_t Z,
+                                                    uint64_t *__restrict__ rp,
+                                                    uint8_t *__restrict__ lmaps,
+                                                    uint32_t *__restrict__ bmaps) {
+  __shared__ uint64_t ws0[CARRY_LPB + 2], ws1[CARRY_LPB + 2], ws2[CARRY_LPB + 2];
+  __shared__ uint64_t mp64[CARRY_LPB / 8];
+  uint8_t *mp = (uint8_t *)mp64;
+  const uint32_t B0 = blockIdx.x * CARRY_LPB;
+  // coefficients B0-2 .. B0+CARRY_LPB-1 -> LDS slot j+2 (coalesced residue reads)
+#pragma unroll 2
+  for (int k = 0; k < CARRY_LPT; ++k) {
+    const uint32_t j = threadIdx.x + k * NT;
+    const uint32_t i = B0 + j;
+    uint64_t c0 = 0, c1 = 0, c2 = 0;
+    if (i < zlen) garner_coef(ga, spec, N, i, c0, c1, c2);
+    ws0[j + 2] = c0; ws1[j + 2] = c1; ws2[j + 2] = c2;
+  }
+  if (threadIdx.x < 2) {
+    const uint32_t j = threadIdx.x;            // slot j holds coefficient B0 - 2 + j
+    uint64_t c0 = 0, c1 = 0, c2 = 0;
+    if (B0 >= 2 - j) {
+      const uint32_t i = B0 - 2 + j;
+      if (i < zlen) garner_coef(ga, spec, N, i, c0, c1, c2);
+    }
+    ws0[j] = c0; ws1[j] = c1; ws2[j] = c2;
+  }
+  __syncthreads();
+  // limb sums (coalesced writes of v_j into the product), limb maps in LDS
 #pragma unroll
   for (int k = 0; k < CARRY_LPT; ++k) {
-    if (i0 + k < Z) limb_sum(w0, w1, w2, zlen, i0 + k, v[k], h[k]);
-    else { v[k] = 0; h[k] = 0; }
+    const uint32_t j = threadIdx.x + k * NT;
+    const uint32_t L = B0 + j;
+    uint32_t f = MAP_ID;
+    if (L < Z) {
+      const uint64_t a = ws0[j + 2], b = ws1[j + 1], c = ws2[j];
+      const uint64_t s1 = a + b;
+      uint32_t h = s1 < a ? 1u : 0u;
+      const uint64_t s2 = s1 + c;
+      h += s2 < s1 ? 1u : 0u;
+      rp[L] = s2;
+      f = limb_map(s2, h);
+    }
+    mp[j] = (uint8_t)f;
   }
+  __syncthreads();
+  // each thread composes its 8 consecutive limb maps; block total
+  const uint64_t m8 = mp64[threadIdx.x];
+  *(uint64_t *)(lmaps + (size_t)B0 + threadIdx.x * CARRY_LPT) = m8;
+  uint32_t f = MAP_ID;
+#pragma unroll
+  for (int k = 0; k < CARRY_LPT; ++k) f = map_then(f, (uint32_t)(m8 >> (8 * k)) & 0xFFu);
   uint32_t tot;
-  block_scan_maps(chunk_map(v, h), &tot);
+  block_scan_maps(f, &tot);
   if (threadIdx.x == 0) bmaps[blockIdx.x] = tot;
 }
 
@@ -435,28 +474,31 @@ __global__ __launch_bounds__(NT) void k_carry_blocks(const uint32_t *__restrict_
     }
 }
 
-__global__ __launch_bounds__(NT) void k_carry_out(const uint64_t *__restrict__ w0,
-                                                  const uint64_t *__restrict__ w1,
-                                                  const uint64_t *__restrict__ w2,
-                                                  uint64_t zlen, uint64_t Z,
-                                                  const uint32_t *__restrict__ bcin,
-                                                  uint64_t *__restrict__ rp) {
-  const uint64_t i0 = (uint64_t)blockIdx.x * CARRY_LPB + (uint64_t)threadIdx.x * CARRY_LPT;
-  uint64_t v[CARRY_LPT];
-  uint32_t h[CARRY_LPT];
+__global__ __launch_bounds__(NT) void k_carry_fix(uint64_t *__restrict__ rp,
+                                                  const uint8_t *__restrict__ lmaps, uint32_t Z,
+                                                  const uint32_t *__restrict__ bcin) {
+  __shared__ uint64_t cj64[CARRY_LPB / 8];
+  uint8_t *cj = (uint8_t *)cj64;
+  const uint32_t B0 = blockIdx.x * CARRY_LPB;
+  const uint64_t m8 = *(const uint64_t *)(lmaps + (size_t)B0 + threadIdx.x * CARRY_LPT);
+  uint32_t f = MAP_ID;
 #pragma unroll
-  for (int k = 0; k < CARRY_LPT; ++k) {
-    if (i0 + k < Z) limb_sum(w0, w1, w2, zlen, i0 + k, v[k], h[k]);
-    else { v[k] = 0; h[k] = 0; }
-  }
+  for (int k = 0; k < CARRY_LPT; ++k) f = map_then(f, (uint32_t)(m8 >> (8 * k)) & 0xFFu);
   uint32_t tot;
-  const uint32_t pre = block_scan_maps(chunk_map(v, h), &tot);
+  const uint32_t pre = block_scan_maps(f, &tot);
   uint32_t c = map_apply(pre, bcin[blockIdx.x]);
 #pragma unroll
   for (int k = 0; k < CARRY_LPT; ++k) {
-    const uint64_t t = v[k] + c;
-    c = h[k] + (t < v[k] ? 1u : 0u);
-    if (i0 + k < Z) rp[i0 + k] = t;
+    cj[threadIdx.x * CARRY_LPT + k] = (uint8_t)c;
+    c = map_apply((uint32_t)(m8 >> (8 * k)) & 0xFFu, c);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < CARRY_LPT; ++k) {
+    const uint32_t j = threadIdx.x + k * NT;
+    const uint32_t L = B0 + j;
+    const uint32_t cc = cj[j];
+    if (L < Z && cc) rp[L] += cc;
   }
 }
 
@@ -599,10 +641,11 @@ static int ctx_twiddles(DevCtx &c, int L, hipStream_t st) {
 }
 
 // workspace layout for transform length N: aspec, bspec (K*N u32 each),
-// w0, w1, w2 (N u64 each), carry maps and carry-ins (N/CARRY_LPB+1 u32 each)
+// limb carry maps (one byte per limb, padded to whole blocks), block maps and
+// block carry-ins (one u32 per block of CARRY_LPB limbs each)
 static size_t ws_need(size_t N) {
   const size_t nb = N / CARRY_LPB + 2;
-  return 2 * SBA_K * N * 4 + 3 * N * 8 + 2 * nb * 4 + 4096;
+  return 2 * SBA_K * N * 4 + nb * CARRY_LPB + 2 * nb * 4 + 4096;
 }
 
 static int lg_for(uint64_t zlen) {
@@ -804,10 +847,9 @@ static int run_mul(DevCtx &c, uint64_t *rp, const uint64_t *ap, uint64_t an, con
   const size_t N = (size_t)1 << L;
   uint32_t *aspec = (uint32_t *)ws;
   uint32_t *bspec = aspec + SBA_K * N;
-  uint64_t *w0 = (uint64_t *)(bspec + SBA_K * N);
-  uint64_t *w1 = w0 + N, *w2 = w1 + N;
   const uint32_t nb = (uint32_t)((Z + CARRY_LPB - 1) / CARRY_LPB);
-  uint32_t *bmaps = (uint32_t *)(w2 + N);
+  uint8_t *lmaps = (uint8_t *)(bspec + SBA_K * N);
+  uint32_t *bmaps = (uint32_t *)(lmaps + (size_t)nb * CARRY_LPB);
   uint32_t *bcin = bmaps + nb + 1;
   const PrimeArgs &pa = c.pa;
 
@@ -865,14 +907,9 @@ static int run_mul(DevCtx &c, uint64_t *rp, const uint64_t *ap, uint64_t an, con
     ga.scale[q] = (uint32_t)((unsigned __int128)ninv * (((unsigned __int128)1 << 64) % p) % p);
   }
   {
-    PROF("garner", (4.0 * SBA_K + 24.0) * zlen);
-    hipLaunchKernelGGL(k_garner, dim3((uint32_t)((zlen + NT - 1) / NT)), dim3(NT), 0, st, ga,
-                       (const uint32_t *)bspec, N, zlen, w0, w1, w2);
-  }
-  {
-    PROF("carry_maps", 24.0 * zlen);
-    hipLaunchKernelGGL(k_carry_maps, dim3(nb), dim3(NT), 0, st, (const uint64_t *)w0,
-                       (const uint64_t *)w1, (const uint64_t *)w2, zlen, Z, bmaps);
+    PROF("garner_maps", 4.0 * SBA_K * zlen + 9.0 * Z);
+    hipLaunchKernelGGL(k_garner_maps, dim3(nb), dim3(NT), 0, st, ga, (const uint32_t *)bspec,
+                       (uint32_t)N, (uint32_t)zlen, (uint32_t)Z, rp, lmaps, bmaps);
   }
   {
     PROF("carry_blocks", 8.0 * nb);
@@ -880,10 +917,9 @@ static int run_mul(DevCtx &c, uint64_t *rp, const uint64_t *ap, uint64_t an, con
                        bcin);
   }
   {
-    PROF("carry_out", 24.0 * zlen + 8.0 * Z);
-    hipLaunchKernelGGL(k_carry_out, dim3(nb), dim3(NT), 0, st, (const uint64_t *)w0,
-                       (const uint64_t *)w1, (const uint64_t *)w2, zlen, Z,
-                       (const uint32_t *)bcin, rp);
+    PROF("carry_fix", 17.0 * Z);
+    hipLaunchKernelGGL(k_carry_fix, dim3(nb), dim3(NT), 0, st, rp, (const uint8_t *)lmaps,
+                       (uint32_t)Z, (const uint32_t *)bcin);
   }
   HIPCHK(hipGetLastError());
   return SBA_OK;
