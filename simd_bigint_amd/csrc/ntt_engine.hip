@@ -47,6 +47,14 @@ struct GarnerArgs {
 };
 
 #define NT 256
+// occupancy hint for the transform kernels (waves per SIMD) and the widest
+// register chunk (radix 2^SBA_BMAX), fixed by measurement
+#ifndef SBA_WAVES
+#define SBA_WAVES 1
+#endif
+#ifndef SBA_BMAX
+#define SBA_BMAX 5
+#endif
 __device__ __forceinline__ uint32_t lpad(uint32_t i) { return i + (i >> 5); }
 
 // blockIdx -> tile so that consecutive tiles (which share cache lines and
@@ -154,7 +162,7 @@ __device__ __forceinline__ void tile_chunk(uint32_t *lds, const uint32_t *__rest
 template <int ELL, int LOGTC> struct ChunkPlan {
   static constexpr int E = (1 << (ELL + LOGTC)) / NT;
   static constexpr int LE = ILog2<E>::v;
-  static constexpr int BM = LE < 5 ? LE : 5;
+  static constexpr int BM = LE < SBA_BMAX ? LE : SBA_BMAX;
   static constexpr int pick(int t0) {
     int r = ELL - t0;
     int bmax = r < BM ? r : BM;
@@ -192,7 +200,7 @@ __device__ __forceinline__ void tile_inv(uint32_t *lds, const uint32_t *tw, uint
 // First forward pass (global levels [0, ELL)): limbs -> residues of all K
 // primes -> ELL levels -> spec[q][.]. Tiles = column tiles (H = 0).
 template <int ELL, int LOGTC>
-__global__ __launch_bounds__(NT) void k_first_fwd(PrimeArgs pa, const uint64_t *__restrict__ limbs,
+__global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd(PrimeArgs pa, const uint64_t *__restrict__ limbs,
                                                   uint64_t nlimbs, uint32_t *__restrict__ spec,
                                                   int L) {
   constexpr int TC = 1 << LOGTC;
@@ -205,6 +213,9 @@ __global__ __launch_bounds__(NT) void k_first_fwd(PrimeArgs pa, const uint64_t *
   const uint32_t col0 = tile << LOGTC;
   const uint32_t nl = nlimbs < N ? (uint32_t)nlimbs : N;
   uint32_t *lt = lds + lpad(threadIdx.x);
+  // element e = tid + i*NT sits at col0 + toff + uoff(i): a per-thread part
+  // and a uniform part (scalar base + 32-bit lane offset addressing)
+  const uint32_t toff = (threadIdx.x >> LOGTC) * S + (threadIdx.x & (TC - 1));
 #pragma unroll 1
   for (int q = 0; q < SBA_K; ++q) {
     const uint32_t p = pa.p[q], pinv = pa.pinv[q], r2 = pa.r2[q];
@@ -212,18 +223,17 @@ __global__ __launch_bounds__(NT) void k_first_fwd(PrimeArgs pa, const uint64_t *
     // the tile's limbs are re-read per prime (L2-resident after the first)
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      const uint32_t e = threadIdx.x + i * NT;
-      const uint32_t gidx = (e >> LOGTC) * S + col0 + (e & (TC - 1));
-      lt[i * (NT + NT / 32)] = limb_residue(gidx < nl ? limbs[gidx] : 0, p, pinv, r2);
+      const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+      const uint64_t *lp = limbs + uoff;
+      lt[i * (NT + NT / 32)] = limb_residue(uoff + toff < nl ? lp[toff] : 0, p, pinv, r2);
     }
     tile_fwd<ELL, LOGTC, 0>(lds, pa.wn[q], 0, p, pinv);
     __syncthreads();
     uint32_t *out = spec + q * N;
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      const uint32_t e = threadIdx.x + i * NT;
-      const uint32_t gidx = (e >> LOGTC) * S + col0 + (e & (TC - 1));
-      out[gidx] = lt[i * (NT + NT / 32)];
+      const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+      (out + uoff)[toff] = lt[i * (NT + NT / 32)];
     }
   }
 }
@@ -231,7 +241,7 @@ __global__ __launch_bounds__(NT) void k_first_fwd(PrimeArgs pa, const uint64_t *
 // In-place pass over global levels [s0, s0 + ELL) of one prime
 // (blockIdx.y), forward or inverse.
 template <int ELL, int LOGTC, bool INV>
-__global__ __launch_bounds__(NT) void k_pass(PrimeArgs pa, uint32_t *__restrict__ spec, int L,
+__global__ __launch_bounds__(NT, SBA_WAVES) void k_pass(PrimeArgs pa, uint32_t *__restrict__ spec, int L,
                                              int s0) {
   constexpr int TC = 1 << LOGTC;
   constexpr int NE = 1 << (ELL + LOGTC);
@@ -245,21 +255,22 @@ __global__ __launch_bounds__(NT) void k_pass(PrimeArgs pa, uint32_t *__restrict_
   const uint32_t H = tile / ctiles;
   const uint32_t col0 = (tile % ctiles) << LOGTC;
   const uint32_t base = (H << (L - s0)) + col0;
-  uint32_t *buf = spec + q * N;
+  uint32_t *buf = spec + q * N + base;
   const uint32_t p = pa.p[q], pinv = pa.pinv[q];
   uint32_t *lt = lds + lpad(threadIdx.x);
+  const uint32_t toff = (threadIdx.x >> LOGTC) * S + (threadIdx.x & (TC - 1));
 #pragma unroll
   for (int i = 0; i < E; ++i) {
-    const uint32_t e = threadIdx.x + i * NT;
-    lt[i * (NT + NT / 32)] = buf[base + (e >> LOGTC) * S + (e & (TC - 1))];
+    const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+    lt[i * (NT + NT / 32)] = (buf + uoff)[toff];
   }
   if (!INV) tile_fwd<ELL, LOGTC, 0>(lds, pa.wn[q], H, p, pinv);
   else tile_inv<ELL, LOGTC, 0>(lds, pa.wni[q], H, p, pinv);
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < E; ++i) {
-    const uint32_t e = threadIdx.x + i * NT;
-    buf[base + (e >> LOGTC) * S + (e & (TC - 1))] = lt[i * (NT + NT / 32)];
+    const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+    (buf + uoff)[toff] = lt[i * (NT + NT / 32)];
   }
 }
 
@@ -267,7 +278,7 @@ __global__ __launch_bounds__(NT) void k_pass(PrimeArgs pa, uint32_t *__restrict_
 // square) + first inverse levels, on contiguous tiles (S = 1, TC = 1).
 // LIMB_IN: the pass is the whole transform (s0 = 0) and reads b's limbs.
 template <int ELL, bool LIMB_IN, bool SQR>
-__global__ __launch_bounds__(NT) void k_fused(PrimeArgs pa, const uint64_t *__restrict__ blimbs,
+__global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, const uint64_t *__restrict__ blimbs,
                                               uint64_t bn, const uint32_t *__restrict__ aspec,
                                               uint32_t *__restrict__ bspec, int L) {
   constexpr int NE = 1 << ELL;
@@ -279,33 +290,33 @@ __global__ __launch_bounds__(NT) void k_fused(PrimeArgs pa, const uint64_t *__re
   const uint32_t base = H << ELL;
   const uint32_t nb = bn < N ? (uint32_t)bn : N;
   const uint32_t p = pa.p[q], pinv = pa.pinv[q];
-  uint32_t *buf = bspec + q * N;
+  uint32_t *buf = bspec + q * N + base;
+  const uint32_t *as = aspec + q * N + base;
+  const uint64_t *bl = blimbs + base;
   uint32_t *lt = lds + lpad(threadIdx.x);
-  const uint32_t tb = base + threadIdx.x;
+  const uint32_t tid = threadIdx.x;
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     uint32_t v;
     if (LIMB_IN) {
-      const uint32_t gidx = tb + i * NT;
-      v = limb_residue(gidx < nb ? blimbs[gidx] : 0, p, pinv, pa.r2[q]);
+      v = limb_residue(base + i * NT + tid < nb ? (bl + i * NT)[tid] : 0, p, pinv, pa.r2[q]);
     } else {
-      v = buf[tb + i * NT];
+      v = (buf + i * NT)[tid];
     }
     lt[i * (NT + NT / 32)] = v;
   }
   tile_fwd<ELL, 0, 0>(lds, pa.wn[q], H, p, pinv);
   __syncthreads();
-  const uint32_t *as = aspec + q * N;
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const uint32_t v = lt[i * (NT + NT / 32)];
-    const uint32_t o = SQR ? red1(v, p) : red1(as[tb + i * NT], p);
+    const uint32_t o = SQR ? red1(v, p) : red1((as + i * NT)[tid], p);
     lt[i * (NT + NT / 32)] = mont_mul(v, o, p, pinv);
   }
   tile_inv<ELL, 0, 0>(lds, pa.wni[q], H, p, pinv);
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < E; ++i) buf[tb + i * NT] = lt[i * (NT + NT / 32)];
+  for (int i = 0; i < E; ++i) (buf + i * NT)[tid] = lt[i * (NT + NT / 32)];
 }
 
 // ---------------------------------------------------------------------------
