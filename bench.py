@@ -85,11 +85,175 @@ def cpu_baseline(a, b, budget_s):
                       "(mean %.3f ns/limb)" % (what, len(times), len(times), mean * 1e9 / n)}, r
 
 
+def batch_leg(njobs, L):
+    """4096 independent multiplies, an and bn log-uniform in [64, 65536]
+    limbs (10% sparse, 5% all-ones), resident in HBM: one hipGraph replay of
+    the batch vs the plain per-call GPU path over the same jobs."""
+    import numpy as np
+    import torch
+    import simd_bigint_amd as sba
+    from simd_bigint_amd.workloads import batch_workload, to_int
+
+    pairs = batch_workload(njobs, seed=4)
+    dev = torch.device("cuda:0")
+    jobs, jobs2 = [], []
+    for a, b in pairs:
+        ta = torch.from_numpy(a.view(np.int64)).to(dev)
+        tb = torch.from_numpy(b.view(np.int64)).to(dev)
+        jobs.append((torch.empty(len(a) + len(b), dtype=torch.int64, device=dev), ta, tb))
+        jobs2.append((torch.empty(len(a) + len(b), dtype=torch.int64, device=dev), ta, tb))
+    limbs = sum(len(a) + len(b) for a, b in pairs)
+    stream = torch.cuda.Stream(device=dev)
+    bt = sba.Batch(jobs, use_graph=True)
+    with torch.cuda.stream(stream):
+        bt.run(stream)
+    torch.cuda.synchronize()
+    reps = 5
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        bt.run(stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_batch = e0.elapsed_time(e1) / reps
+    log("bench: batch of %d in %.3f ms (%d size classes)" % (njobs, ms_batch, bt.classes))
+    rc = sba.mul_loop(jobs2, stream)
+    torch.cuda.synchronize()
+    if rc != 1:
+        raise RuntimeError("per-call loop returned %d" % rc)
+    e0.record(stream)
+    t0 = time.perf_counter()
+    for _ in range(2):
+        sba.mul_loop(jobs2, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_loop = max(e0.elapsed_time(e1), (time.perf_counter() - t0) * 1e3) / 2
+    log("bench: per-call loop %.3f ms" % ms_loop)
+    same = all(torch.equal(j1[0], j2[0]) for j1, j2 in zip(jobs, jobs2))
+    sample_ok = True
+    for k in range(0, njobs, max(njobs // 32, 1)):
+        a, b = pairs[k]
+        want = to_int(a) * to_int(b)
+        got = to_int(jobs[k][0].cpu().numpy().view(np.uint64))
+        sample_ok &= got == want
+    bt.close()
+    return {"jobs": njobs, "limbs_total": int(limbs), "size_classes": bt.classes,
+            "batched_ms": round(ms_batch, 4), "per_call_ms": round(ms_loop, 4),
+            "batched_muls_per_s": round(njobs / (ms_batch * 1e-3), 1),
+            "per_call_muls_per_s": round(njobs / (ms_loop * 1e-3), 1),
+            "speedup": round(ms_loop / ms_batch, 3),
+            "bit_exact": bool(same and sample_ok),
+            "check": "all batched products == per-call products; 32 sampled products == "
+                     "Python integer products"}
+
+
+def serve_leg(nreq, rate):
+    """Serving: Poisson arrivals (80% multiplies / 20% squares), operand sizes
+    log-uniform in [256, 2^20] limbs drawn as slices of a device-resident
+    random pool, admission by a fixed HBM budget, continuous batching over 3
+    streams. Every result is checked on the CPU by fingerprints mod two
+    61-bit primes; 16 sampled results are compared with the same request run
+    alone."""
+    import numpy as np
+    import torch
+    import simd_bigint_amd as sba
+    from simd_bigint_amd import serving
+    from simd_bigint_amd.workloads import to_int
+
+    rng = np.random.default_rng(17)
+    pool_np = rng.integers(0, 2**64, 1 << 21, dtype=np.uint64)
+    pool = torch.from_numpy(pool_np.view(np.int64)).cuda()
+    budget = 2 << 30
+    reqs = serving.poisson_requests(nreq, rate, pool, seed=18)
+    srv = serving.Server(budget, nstreams=3, max_batch=128)
+    # warm the size classes once (plans, twiddles, allocator)
+    serving.Server(budget, nstreams=3, max_batch=128).run(
+        serving.poisson_requests(32, 1e6, pool, seed=19))
+    stats = srv.run(reqs)
+    out = stats.summary()
+    qs = [(1 << 61) - 1, 2305843009213693921]
+    ok = True
+    for q in reqs:
+        r = to_int(q.r.cpu().numpy().view(np.uint64))
+        a = to_int(q.a.cpu().numpy().view(np.uint64))
+        b = a if q.b is None else to_int(q.b.cpu().numpy().view(np.uint64))
+        for m in qs:
+            ok &= (r % m) == (a % m) * (b % m) % m
+    same = True
+    for q in reqs[:: max(nreq // 16, 1)]:
+        alone = torch.empty(q.limbs, dtype=torch.int64, device="cuda")
+        if q.b is None:
+            sba.sqr_dev(alone, q.a)
+        else:
+            sba.mul_dev(alone, q.a, q.b)
+        torch.cuda.synchronize()
+        same &= torch.equal(alone, q.r)
+    out.update({"rate_per_s": rate, "hbm_budget_bytes": budget,
+                "sizes": "log-uniform [256, 2^20] limbs, 80% mul / 20% sqr",
+                "bit_exact": bool(ok and same),
+                "check": "all results: fingerprints mod 2 primes; 16 sampled: == alone"})
+    log("bench: serving", out)
+    return out
+
+
+def configs_leg(L, reps=10):
+    """The other BASELINE shapes on the same engine (operands resident in
+    HBM): 3*2^20 balanced, 2^22 squaring, 2^23 x 2^18, and the 16384^2
+    reference-CPU shape; each product checked against its golden SHA-256."""
+    import numpy as np
+    import torch
+    from simd_bigint_amd.workloads import pattern_fill
+
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
+    gl = {e["name"]: e for e in gold["large_mul"]}
+    out = {}
+    for name in ("cfg2_3x2^20", "cfg2_3x2^20_ones", "cfg2_sqr_2^22", "cfg3_2^23x2^18",
+                 "cfg1_16384"):
+        e = gl[name]
+        st = [e["seed"]]
+        a = pattern_fill(e["an"], e["pat"], st)
+        ta = torch.from_numpy(a.view(np.int64)).cuda()
+        if e["sqr"]:
+            tr = torch.empty(2 * e["an"], dtype=torch.int64, device="cuda")
+            f = lambda: L.sba_gpu_sqr_dev(tr.data_ptr(), ta.data_ptr(), e["an"], None)
+        else:
+            b = pattern_fill(e["bn"], e["pat"], st)
+            tb = torch.from_numpy(b.view(np.int64)).cuda()
+            tr = torch.empty(e["an"] + e["bn"], dtype=torch.int64, device="cuda")
+            f = lambda: L.sba_gpu_mul_dev(tr.data_ptr(), ta.data_ptr(), e["an"], tb.data_ptr(),
+                                          e["bn"], None)
+        f()
+        torch.cuda.synchronize()
+        exact = hashlib.sha256(tr.cpu().numpy().tobytes()).hexdigest() == e["sha256"]
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        n = max(e["an"], e["bn"])
+        out[name] = {"an": e["an"], "bn": e["bn"] or e["an"], "ms": round(ms, 4),
+                     "ns_per_limb": round(ms * 1e6 / n, 5), "bit_exact": bool(exact)}
+        log("bench:", name, out[name])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch-jobs", type=int, default=4096,
+                    help="jobs in the batched vs per-call comparison (0 = skip)")
+    ap.add_argument("--configs", type=int, default=1,
+                    help="also time the other BASELINE shapes (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=1000,
+                    help="requests in the serving run (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=5000.0,
+                    help="Poisson arrival rate of the serving run, requests/s")
     ap.add_argument("--cpu-budget", type=float, default=12.0,
                     help="seconds of CPU work for the cpu_baseline sample (0 = skip)")
     args = ap.parse_args()
@@ -183,6 +347,14 @@ def main():
     b_alg = Z * (16 + 32 * K_model)
     pipe_gbs = b_alg / (ms * 1e-3) / 1e9
 
+    others = configs_leg(L) if args.configs else None
+    batched = None
+    if args.batch_jobs > 0:
+        batched = batch_leg(args.batch_jobs, L)
+    served = None
+    if args.serve_requests > 0:
+        served = serve_leg(args.serve_requests, args.serve_rate)
+
     cpu = None
     if args.cpu_budget > 0:
         log("bench: cpu baseline")
@@ -213,6 +385,9 @@ def main():
                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(pipe_gbs / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
+        "other_configs": others,
+        "batched": batched,
+        "serving": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -113,6 +113,38 @@ int sba_gpu_max_lg(void);
 int sba_gpu_device_count(void);
 
 /* ---------------------------------------------------------------------- */
+/* Batches (no counterpart in the reference, which has no batch entry):    */
+/* many independent multiplies of uneven sizes on device buffers. A job    */
+/* with bp == NULL is a square of ap. sba_batch_create groups the jobs in  */
+/* size classes, allocates their workspace once and (use_graph != 0)       */
+/* captures the whole launch sequence into a hipGraph; sba_batch_run       */
+/* replays it on a stream (enqueue only). Jobs keep their pointers: rerun  */
+/* after rewriting the operand buffers. *status: SBA_OK, SBA_BAND (a job   */
+/* outside the band) or < 0.                                               */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+  uint64_t *rp;            /* an+bn limbs (2an for a square) */
+  const uint64_t *ap;
+  int64_t an;
+  const uint64_t *bp;      /* NULL: square of ap */
+  int64_t bn;
+} sba_job;
+typedef struct sba_batch sba_batch;
+sba_batch *sba_batch_create(const sba_job *jobs, int njobs, int use_graph, int *status);
+int sba_batch_run(sba_batch *b, void *stream);
+int sba_batch_classes(const sba_batch *b);
+void sba_batch_destroy(sba_batch *b);
+/* one-shot launch of a batch into a caller workspace of >=
+ * sba_batch_ws_bytes(jobs, n) bytes: no allocation, no graph (the serving
+ * loop's entry). Returns after enqueueing; the workspace is in use until the
+ * stream passes this point. */
+size_t sba_batch_ws_bytes(const sba_job *jobs, int njobs);
+int sba_batch_launch_ws(const sba_job *jobs, int njobs, void *ws, size_t ws_bytes,
+                        void *stream);
+/* the same jobs through the plain per-call entries, one after another */
+int sba_gpu_mul_loop(const sba_job *jobs, int njobs, void *stream);
+
+/* ---------------------------------------------------------------------- */
 /* Tuning and test hooks (no counterpart in the reference, whose tier      */
 /* thresholds are compile-time macros, include/u52-mparam.h:1-19).         */
 /* ---------------------------------------------------------------------- */

@@ -89,6 +89,20 @@ def lib() -> ctypes.CDLL:
     L.sba_gpu_max_lg.restype = ctypes.c_int
     L.sba_gpu_device_count.argtypes = []
     L.sba_gpu_device_count.restype = ctypes.c_int
+    L.sba_batch_create.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.sba_batch_create.restype = vp
+    L.sba_batch_run.argtypes = [vp, vp]
+    L.sba_batch_run.restype = ctypes.c_int
+    L.sba_batch_classes.argtypes = [vp]
+    L.sba_batch_classes.restype = ctypes.c_int
+    L.sba_batch_destroy.argtypes = [vp]
+    L.sba_batch_destroy.restype = None
+    L.sba_batch_ws_bytes.argtypes = [vp, ctypes.c_int]
+    L.sba_batch_ws_bytes.restype = ctypes.c_size_t
+    L.sba_batch_launch_ws.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t, vp]
+    L.sba_batch_launch_ws.restype = ctypes.c_int
+    L.sba_gpu_mul_loop.argtypes = [vp, ctypes.c_int, vp]
+    L.sba_gpu_mul_loop.restype = ctypes.c_int
     L.sba_set_gpu_threshold.argtypes = [u64]
     L.sba_set_gpu_threshold.restype = None
     L.sba_get_gpu_threshold.argtypes = []
@@ -203,3 +217,62 @@ def min_operand_cap() -> int:
 
 def max_lg() -> int:
     return int(lib().sba_gpu_max_lg())
+
+
+# ---------------------------------------------------------------------------
+# batches
+# ---------------------------------------------------------------------------
+class SbaJob(ctypes.Structure):
+    _fields_ = [("rp", ctypes.c_void_p), ("ap", ctypes.c_void_p), ("an", ctypes.c_int64),
+                ("bp", ctypes.c_void_p), ("bn", ctypes.c_int64)]
+
+
+def job_array(jobs):
+    """(r, a, b) torch tensors -> ctypes array of sba_job (b None: square)."""
+    arr = (SbaJob * len(jobs))()
+    for i, (r, a, b) in enumerate(jobs):
+        need = a.numel() + (b.numel() if b is not None else a.numel())
+        _check_dev(r, need)
+        arr[i] = SbaJob(_ptr(r), _ptr(a), a.numel(), None if b is None else _ptr(b),
+                        0 if b is None else b.numel())
+    return arr
+
+
+def mul_loop(jobs, stream=None) -> int:
+    """The plain per-call path over the jobs (one GPU call each)."""
+    arr = job_array(jobs)
+    s = None if stream is None else ctypes.c_void_p(int(stream.cuda_stream))
+    return lib().sba_gpu_mul_loop(ctypes.cast(arr, ctypes.c_void_p), len(jobs), s)
+
+
+class Batch:
+    """Many independent device-resident multiplies (or squares) of uneven
+    sizes, launched together (sba_batch_create / sba_batch_run). jobs is a
+    list of (r, a, b) torch tensors; b = None means a square of a."""
+
+    def __init__(self, jobs, use_graph: bool = True):
+        self._keep = jobs
+        arr = job_array(jobs)
+        st = ctypes.c_int()
+        self._h = lib().sba_batch_create(ctypes.cast(arr, ctypes.c_void_p), len(jobs),
+                                         1 if use_graph else 0, ctypes.byref(st))
+        if not self._h:
+            raise RuntimeError("sba_batch_create failed with status %d" % st.value)
+        self.classes = lib().sba_batch_classes(self._h)
+
+    def run(self, stream=None) -> None:
+        s = None if stream is None else ctypes.c_void_p(int(stream.cuda_stream))
+        rc = lib().sba_batch_run(self._h, s)
+        if rc != SBA_OK:
+            raise RuntimeError("sba_batch_run returned %d" % rc)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().sba_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

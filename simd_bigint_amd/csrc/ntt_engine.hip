@@ -195,23 +195,78 @@ __device__ __forceinline__ void tile_inv(uint32_t *lds, const uint32_t *tw, uint
 }
 
 // ---------------------------------------------------------------------------
+// job descriptors: every kernel serves one multiply (JobRef.one) or a whole
+// size class of a batch (JobRef.arr[blockIdx.z]); all jobs of one launch
+// share the transform length N = 2^L.
+// ---------------------------------------------------------------------------
+struct Job {
+  const uint64_t *a, *b;          // operand limbs (device)
+  uint64_t *r;                    // product limbs (device), an+bn of them
+  uint32_t *aspec, *bspec;        // K x N residues each
+  uint8_t *lmaps;                 // one carry map per limb, padded to blocks
+  uint32_t *bmaps, *bcin;         // per carry block
+  uint32_t an, bn, zlen, Z, nb;   // limbs, coefficients, product limbs, carry blocks
+};
+// global-address-space views of descriptor pointers: loads through a pointer
+// read from memory are otherwise generic (flat) and serialised against LDS
+typedef __attribute__((address_space(1))) uint32_t g_u32;
+typedef __attribute__((address_space(1))) const uint32_t gc_u32;
+typedef __attribute__((address_space(1))) uint64_t g_u64;
+typedef __attribute__((address_space(1))) const uint64_t gc_u64;
+typedef __attribute__((address_space(1))) uint8_t g_u8;
+
+struct JobRef {
+  const Job *arr;
+  Job one;
+};
+// The descriptor is uniform across the block; readfirstlane moves every field
+// into scalar registers so tile addresses stay scalar-base + lane-offset.
+template <typename T> __device__ __forceinline__ T *uni_ptr(T *p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (T *)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ uint32_t uni_u32(uint32_t x) {
+  return __builtin_amdgcn_readfirstlane(x);
+}
+__device__ __forceinline__ Job get_job(const JobRef &jr) {
+  Job j;
+  if (jr.arr) {
+    j = jr.arr[blockIdx.z];
+  } else {
+    j = jr.one;
+  }
+  j.a = uni_ptr(j.a); j.b = uni_ptr(j.b); j.r = uni_ptr(j.r);
+  j.aspec = uni_ptr(j.aspec); j.bspec = uni_ptr(j.bspec); j.lmaps = uni_ptr(j.lmaps);
+  j.bmaps = uni_ptr(j.bmaps); j.bcin = uni_ptr(j.bcin);
+  j.an = uni_u32(j.an); j.bn = uni_u32(j.bn); j.zlen = uni_u32(j.zlen); j.Z = uni_u32(j.Z);
+  j.nb = uni_u32(j.nb);
+  return j;
+}
+
+// ---------------------------------------------------------------------------
 // pass kernels
 // ---------------------------------------------------------------------------
 // First forward pass (global levels [0, ELL)): limbs -> residues of all K
 // primes -> ELL levels -> spec[q][.]. Tiles = column tiles (H = 0).
+// which = 0: operand a into aspec; 1: operand b into bspec.
 template <int ELL, int LOGTC>
-__global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd(PrimeArgs pa, const uint64_t *__restrict__ limbs,
-                                                  uint64_t nlimbs, uint32_t *__restrict__ spec,
-                                                  int L) {
+__global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd(PrimeArgs pa, JobRef jr, int which,
+                                                             int L) {
   constexpr int TC = 1 << LOGTC;
   constexpr int NE = 1 << (ELL + LOGTC);
   constexpr int E = NE / NT;
   __shared__ uint32_t lds[NE + NE / 32];
+  const Job jb = get_job(jr);
+  gc_u64 *limbs = (gc_u64 *)(which ? jb.b : jb.a);
+  g_u32 *spec = (g_u32 *)(which ? jb.bspec : jb.aspec);
+  const uint32_t nlimbs = which ? jb.bn : jb.an;
   const uint32_t N = 1u << L;             // N <= 2^25: 32-bit offsets throughout
   const uint32_t S = 1u << (L - ELL);
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t col0 = tile << LOGTC;
-  const uint32_t nl = nlimbs < N ? (uint32_t)nlimbs : N;
+  const uint32_t nl = nlimbs < N ? nlimbs : N;
   uint32_t *lt = lds + lpad(threadIdx.x);
   // element e = tid + i*NT sits at col0 + toff + uoff(i): a per-thread part
   // and a uniform part (scalar base + 32-bit lane offset addressing)
@@ -224,12 +279,12 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd(PrimeArgs pa, const
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
-      const uint64_t *lp = limbs + uoff;
+      gc_u64 *lp = limbs + uoff;
       lt[i * (NT + NT / 32)] = limb_residue(uoff + toff < nl ? lp[toff] : 0, p, pinv, r2);
     }
     tile_fwd<ELL, LOGTC, 0>(lds, pa.wn[q], 0, p, pinv);
     __syncthreads();
-    uint32_t *out = spec + q * N;
+    g_u32 *out = spec + q * N;
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
@@ -239,14 +294,16 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd(PrimeArgs pa, const
 }
 
 // In-place pass over global levels [s0, s0 + ELL) of one prime
-// (blockIdx.y), forward or inverse.
+// (blockIdx.y), forward or inverse, on aspec (which = 0) or bspec (1).
 template <int ELL, int LOGTC, bool INV>
-__global__ __launch_bounds__(NT, SBA_WAVES) void k_pass(PrimeArgs pa, uint32_t *__restrict__ spec, int L,
-                                             int s0) {
+__global__ __launch_bounds__(NT, SBA_WAVES) void k_pass(PrimeArgs pa, JobRef jr, int which, int L,
+                                                        int s0) {
   constexpr int TC = 1 << LOGTC;
   constexpr int NE = 1 << (ELL + LOGTC);
   constexpr int E = NE / NT;
   __shared__ uint32_t lds[NE + NE / 32];
+  const Job jb = get_job(jr);
+  g_u32 *spec = (g_u32 *)(which ? jb.bspec : jb.aspec);
   const int q = blockIdx.y;
   const uint32_t N = 1u << L;
   const uint32_t S = 1u << (L - s0 - ELL);
@@ -255,7 +312,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_pass(PrimeArgs pa, uint32_t *
   const uint32_t H = tile / ctiles;
   const uint32_t col0 = (tile % ctiles) << LOGTC;
   const uint32_t base = (H << (L - s0)) + col0;
-  uint32_t *buf = spec + q * N + base;
+  g_u32 *buf = spec + q * N + base;
   const uint32_t p = pa.p[q], pinv = pa.pinv[q];
   uint32_t *lt = lds + lpad(threadIdx.x);
   const uint32_t toff = (threadIdx.x >> LOGTC) * S + (threadIdx.x & (TC - 1));
@@ -278,21 +335,20 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_pass(PrimeArgs pa, uint32_t *
 // square) + first inverse levels, on contiguous tiles (S = 1, TC = 1).
 // LIMB_IN: the pass is the whole transform (s0 = 0) and reads b's limbs.
 template <int ELL, bool LIMB_IN, bool SQR>
-__global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, const uint64_t *__restrict__ blimbs,
-                                              uint64_t bn, const uint32_t *__restrict__ aspec,
-                                              uint32_t *__restrict__ bspec, int L) {
+__global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr, int L) {
   constexpr int NE = 1 << ELL;
   constexpr int E = NE / NT;
   __shared__ uint32_t lds[NE + NE / 32];
+  const Job jb = get_job(jr);
   const int q = blockIdx.y;
   const uint32_t N = 1u << L;
   const uint32_t H = blockIdx.x;
   const uint32_t base = H << ELL;
-  const uint32_t nb = bn < N ? (uint32_t)bn : N;
+  const uint32_t nb = jb.bn < N ? jb.bn : N;
   const uint32_t p = pa.p[q], pinv = pa.pinv[q];
-  uint32_t *buf = bspec + q * N + base;
-  const uint32_t *as = aspec + q * N + base;
-  const uint64_t *bl = blimbs + base;
+  g_u32 *buf = (g_u32 *)jb.bspec + q * N + base;
+  gc_u32 *as = (gc_u32 *)jb.aspec + q * N + base;
+  gc_u64 *bl = (gc_u64 *)jb.b + base;
   uint32_t *lt = lds + lpad(threadIdx.x);
   const uint32_t tid = threadIdx.x;
 #pragma unroll
@@ -322,7 +378,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, const uin
 // ---------------------------------------------------------------------------
 // Garner: the five residues of coefficient i -> 3 u64 words
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void garner_coef(const GarnerArgs &ga, const uint32_t *__restrict__ spec,
+__device__ __forceinline__ void garner_coef(const GarnerArgs &ga, gc_u32 *spec,
                                             uint32_t N, uint32_t i, uint64_t &c0, uint64_t &c1,
                                             uint64_t &c2) {
   uint32_t u[SBA_K], x[SBA_K];
@@ -410,13 +466,14 @@ __device__ uint32_t block_scan_maps(uint32_t f, uint32_t *total) {
   return map_then(pre, exc);
 }
 
-__global__ __launch_bounds__(NT) void k_garner_maps(GarnerArgs ga, const uint32_t *__restrict__ spec,
-                                                    uint32_t N, uint32_t zlen, uint32_t Z,
-                                                    uint64_t *__restrict__ rp,
-                                                    uint8_t *__restrict__ lmaps,
-                                                    uint32_t *__restrict__ bmaps) {
+__global__ __launch_bounds__(NT) void k_garner_maps(GarnerArgs ga, JobRef jr, uint32_t N) {
   __shared__ uint64_t ws0[CARRY_LPB + 2], ws1[CARRY_LPB + 2], ws2[CARRY_LPB + 2];
   __shared__ uint64_t mp64[CARRY_LPB / 8];
+  const Job jb = get_job(jr);
+  if (blockIdx.x >= jb.nb) return;                 // this job has fewer carry blocks
+  gc_u32 *spec = (gc_u32 *)jb.bspec;
+  const uint32_t zlen = jb.zlen, Z = jb.Z;
+  g_u64 *rp = (g_u64 *)jb.r;
   uint8_t *mp = (uint8_t *)mp64;
   const uint32_t B0 = blockIdx.x * CARRY_LPB;
   // coefficients B0-2 .. B0+CARRY_LPB-1 -> LDS slot j+2 (coalesced residue reads)
@@ -442,15 +499,15 @@ __global__ __launch_bounds__(NT) void k_garner_maps(GarnerArgs ga, const uint32_
 #pragma unroll
   for (int k = 0; k < CARRY_LPT; ++k) {
     const uint32_t j = threadIdx.x + k * NT;
-    const uint32_t L = B0 + j;
+    const uint32_t Lb = B0 + j;
     uint32_t f = MAP_ID;
-    if (L < Z) {
+    if (Lb < Z) {
       const uint64_t a = ws0[j + 2], b = ws1[j + 1], c = ws2[j];
       const uint64_t s1 = a + b;
       uint32_t h = s1 < a ? 1u : 0u;
       const uint64_t s2 = s1 + c;
       h += s2 < s1 ? 1u : 0u;
-      rp[L] = s2;
+      rp[Lb] = s2;
       f = limb_map(s2, h);
     }
     mp[j] = (uint8_t)f;
@@ -458,18 +515,21 @@ __global__ __launch_bounds__(NT) void k_garner_maps(GarnerArgs ga, const uint32_
   __syncthreads();
   // each thread composes its 8 consecutive limb maps; block total
   const uint64_t m8 = mp64[threadIdx.x];
-  *(uint64_t *)(lmaps + (size_t)B0 + threadIdx.x * CARRY_LPT) = m8;
+  *(g_u64 *)((g_u8 *)jb.lmaps + (size_t)B0 + threadIdx.x * CARRY_LPT) = m8;
   uint32_t f = MAP_ID;
 #pragma unroll
   for (int k = 0; k < CARRY_LPT; ++k) f = map_then(f, (uint32_t)(m8 >> (8 * k)) & 0xFFu);
   uint32_t tot;
   block_scan_maps(f, &tot);
-  if (threadIdx.x == 0) bmaps[blockIdx.x] = tot;
+  if (threadIdx.x == 0) ((g_u32 *)jb.bmaps)[blockIdx.x] = tot;
 }
 
-// one block: carry into every block = exclusive scan of block maps applied to 0
-__global__ __launch_bounds__(NT) void k_carry_blocks(const uint32_t *__restrict__ bmaps,
-                                                     uint32_t nb, uint32_t *__restrict__ bcin) {
+// one block per job: carry into every block = exclusive scan of block maps
+__global__ __launch_bounds__(NT) void k_carry_blocks(JobRef jr) {
+  const Job jb = get_job(jr);
+  const uint32_t nb = jb.nb;
+  gc_u32 *bmaps = (gc_u32 *)jb.bmaps;
+  g_u32 *bcin = (g_u32 *)jb.bcin;
   const uint32_t per = (nb + NT - 1) / NT;
   const uint32_t b0 = threadIdx.x * per;
   uint32_t f = MAP_ID;
@@ -485,19 +545,21 @@ __global__ __launch_bounds__(NT) void k_carry_blocks(const uint32_t *__restrict_
     }
 }
 
-__global__ __launch_bounds__(NT) void k_carry_fix(uint64_t *__restrict__ rp,
-                                                  const uint8_t *__restrict__ lmaps, uint32_t Z,
-                                                  const uint32_t *__restrict__ bcin) {
+__global__ __launch_bounds__(NT) void k_carry_fix(JobRef jr) {
   __shared__ uint64_t cj64[CARRY_LPB / 8];
+  const Job jb = get_job(jr);
+  if (blockIdx.x >= jb.nb) return;
+  g_u64 *rp = (g_u64 *)jb.r;
+  const uint32_t Z = jb.Z;
   uint8_t *cj = (uint8_t *)cj64;
   const uint32_t B0 = blockIdx.x * CARRY_LPB;
-  const uint64_t m8 = *(const uint64_t *)(lmaps + (size_t)B0 + threadIdx.x * CARRY_LPT);
+  const uint64_t m8 = *(gc_u64 *)((g_u8 *)jb.lmaps + (size_t)B0 + threadIdx.x * CARRY_LPT);
   uint32_t f = MAP_ID;
 #pragma unroll
   for (int k = 0; k < CARRY_LPT; ++k) f = map_then(f, (uint32_t)(m8 >> (8 * k)) & 0xFFu);
   uint32_t tot;
   const uint32_t pre = block_scan_maps(f, &tot);
-  uint32_t c = map_apply(pre, bcin[blockIdx.x]);
+  uint32_t c = map_apply(pre, ((gc_u32 *)jb.bcin)[blockIdx.x]);
 #pragma unroll
   for (int k = 0; k < CARRY_LPT; ++k) {
     cj[threadIdx.x * CARRY_LPT + k] = (uint8_t)c;
@@ -507,9 +569,9 @@ __global__ __launch_bounds__(NT) void k_carry_fix(uint64_t *__restrict__ rp,
 #pragma unroll
   for (int k = 0; k < CARRY_LPT; ++k) {
     const uint32_t j = threadIdx.x + k * NT;
-    const uint32_t L = B0 + j;
+    const uint32_t Lb = B0 + j;
     const uint32_t cc = cj[j];
-    if (L < Z && cc) rp[L] += cc;
+    if (Lb < Z && cc) rp[Lb] += cc;
   }
 }
 
@@ -705,9 +767,9 @@ extern "C" int sba_gpu_max_lg(void) { return SBA_MAX_LG; }
 // ---------------------------------------------------------------------------
 // launch helpers (template dispatch on the pass shape)
 // ---------------------------------------------------------------------------
-typedef void (*first_fn)(PrimeArgs, const uint64_t *, uint64_t, uint32_t *, int);
-typedef void (*pass_fn)(PrimeArgs, uint32_t *, int, int);
-typedef void (*fused_fn)(PrimeArgs, const uint64_t *, uint64_t, const uint32_t *, uint32_t *, int);
+typedef void (*first_fn)(PrimeArgs, JobRef, int, int);
+typedef void (*pass_fn)(PrimeArgs, JobRef, int, int, int);
+typedef void (*fused_fn)(PrimeArgs, JobRef, int);
 
 // first pass of the two-pass plan: ELL = L - 13 levels, TC = 2^(13 - ELL)
 static first_fn first_two_pass(int ell) {
@@ -850,64 +912,58 @@ extern "C" int sba_gpu_profile_get(int idx, const char **name, double *ms, doubl
 
 #define LAST_ELL 13
 
-static int run_mul(DevCtx &c, uint64_t *rp, const uint64_t *ap, uint64_t an, const uint64_t *bp,
-                   uint64_t bn, int sqr, void *ws, hipStream_t st) {
-  const uint64_t zlen = an + bn - 1;
-  const uint64_t Z = an + bn;
-  const int L = lg_for(zlen);
-  const size_t N = (size_t)1 << L;
-  uint32_t *aspec = (uint32_t *)ws;
-  uint32_t *bspec = aspec + SBA_K * N;
-  const uint32_t nb = (uint32_t)((Z + CARRY_LPB - 1) / CARRY_LPB);
-  uint8_t *lmaps = (uint8_t *)(bspec + SBA_K * N);
-  uint32_t *bmaps = (uint32_t *)(lmaps + (size_t)nb * CARRY_LPB);
-  uint32_t *bcin = bmaps + nb + 1;
-  const PrimeArgs &pa = c.pa;
+// carry blocks of a product of Z limbs
+static uint32_t carry_blocks(uint64_t Z) { return (uint32_t)((Z + CARRY_LPB - 1) / CARRY_LPB); }
 
-  const double KN4 = 4.0 * SBA_K * (double)N;
+// Enqueue the whole multiply for `njobs` jobs sharing N = 2^L (jr.arr ==
+// nullptr: the single job jr.one). nbmax = the largest carry-block count;
+// zsum / asum / bsum feed the profiler's algorithmic byte counts.
+static int enqueue_class(DevCtx &c, int L, int sqr, const JobRef &jr, uint32_t njobs,
+                         uint32_t nbmax, double asum, double bsum, double zsum, hipStream_t st) {
+  const uint32_t N = 1u << L;
+  const PrimeArgs &pa = c.pa;
+  const double KN4 = 4.0 * SBA_K * (double)N * njobs;
   if (L <= LAST_ELL) {
     if (!sqr) {
-      PROF("first_fwd", 8.0 * an + KN4);
-      hipLaunchKernelGGL(first_single(L), dim3(1), dim3(NT), 0, st, pa, ap, an, aspec, L);
+      PROF("first_fwd", 8.0 * asum + KN4);
+      hipLaunchKernelGGL(first_single(L), dim3(1, 1, njobs), dim3(NT), 0, st, pa, jr, 0, L);
     }
     {
-      PROF("fused_last", 8.0 * bn + (sqr ? 1 : 2) * KN4);
-      hipLaunchKernelGGL(fused_single(L, sqr), dim3(1, SBA_K), dim3(NT), 0, st, pa, bp, bn,
-                         (const uint32_t *)aspec, bspec, L);
+      PROF("fused_last", 8.0 * bsum + (sqr ? 1 : 2) * KN4);
+      hipLaunchKernelGGL(fused_single(L, sqr), dim3(1, SBA_K, njobs), dim3(NT), 0, st, pa, jr,
+                         L);
     }
   } else {
     const int ell1 = L - LAST_ELL;
-    const uint32_t ntile1 = (uint32_t)(N >> 13);
+    const uint32_t ntile1 = N >> 13;
+    const uint32_t ntile2 = N >> LAST_ELL;
     first_fn f1 = first_two_pass(ell1);
     if (!sqr) {
-      PROF("first_fwd", 8.0 * an + KN4);
-      hipLaunchKernelGGL(f1, dim3(ntile1), dim3(NT), 0, st, pa, ap, an, aspec, L);
+      PROF("first_fwd", 8.0 * asum + KN4);
+      hipLaunchKernelGGL(f1, dim3(ntile1, 1, njobs), dim3(NT), 0, st, pa, jr, 0, L);
     }
     {
-      PROF("first_fwd", 8.0 * bn + KN4);
-      hipLaunchKernelGGL(f1, dim3(ntile1), dim3(NT), 0, st, pa, bp, bn, bspec, L);
+      PROF("first_fwd", 8.0 * bsum + KN4);
+      hipLaunchKernelGGL(f1, dim3(ntile1, 1, njobs), dim3(NT), 0, st, pa, jr, 1, L);
     }
-    const uint32_t ntile2 = (uint32_t)(N >> LAST_ELL);
     if (!sqr) {
       PROF("fwd_last", 2 * KN4);
-      hipLaunchKernelGGL((k_pass<LAST_ELL, 0, false>), dim3(ntile2, SBA_K), dim3(NT), 0, st, pa,
-                         aspec, L, L - LAST_ELL);
+      hipLaunchKernelGGL((k_pass<LAST_ELL, 0, false>), dim3(ntile2, SBA_K, njobs), dim3(NT), 0,
+                         st, pa, jr, 0, L, L - LAST_ELL);
     }
     {
       PROF("fused_last", (sqr ? 2 : 3) * KN4);
       if (sqr)
-        hipLaunchKernelGGL((k_fused<LAST_ELL, false, true>), dim3(ntile2, SBA_K), dim3(NT), 0,
-                           st, pa, (const uint64_t *)nullptr, (uint64_t)0,
-                           (const uint32_t *)aspec, bspec, L);
+        hipLaunchKernelGGL((k_fused<LAST_ELL, false, true>), dim3(ntile2, SBA_K, njobs),
+                           dim3(NT), 0, st, pa, jr, L);
       else
-        hipLaunchKernelGGL((k_fused<LAST_ELL, false, false>), dim3(ntile2, SBA_K), dim3(NT), 0,
-                           st, pa, (const uint64_t *)nullptr, (uint64_t)0,
-                           (const uint32_t *)aspec, bspec, L);
+        hipLaunchKernelGGL((k_fused<LAST_ELL, false, false>), dim3(ntile2, SBA_K, njobs),
+                           dim3(NT), 0, st, pa, jr, L);
     }
     {
       PROF("inv_first", 2 * KN4);
-      hipLaunchKernelGGL(inv_first_two_pass(ell1), dim3(ntile1, SBA_K), dim3(NT), 0, st, pa,
-                         bspec, L, 0);
+      hipLaunchKernelGGL(inv_first_two_pass(ell1), dim3(ntile1, SBA_K, njobs), dim3(NT), 0, st,
+                         pa, jr, 1, L, 0);
     }
   }
   GarnerArgs ga = c.ga_base;
@@ -918,22 +974,29 @@ static int run_mul(DevCtx &c, uint64_t *rp, const uint64_t *ap, uint64_t an, con
     ga.scale[q] = (uint32_t)((unsigned __int128)ninv * (((unsigned __int128)1 << 64) % p) % p);
   }
   {
-    PROF("garner_maps", 4.0 * SBA_K * zlen + 9.0 * Z);
-    hipLaunchKernelGGL(k_garner_maps, dim3(nb), dim3(NT), 0, st, ga, (const uint32_t *)bspec,
-                       (uint32_t)N, (uint32_t)zlen, (uint32_t)Z, rp, lmaps, bmaps);
+    PROF("garner_maps", 4.0 * SBA_K * zsum + 9.0 * zsum);
+    hipLaunchKernelGGL(k_garner_maps, dim3(nbmax, 1, njobs), dim3(NT), 0, st, ga, jr, N);
   }
   {
-    PROF("carry_blocks", 8.0 * nb);
-    hipLaunchKernelGGL(k_carry_blocks, dim3(1), dim3(NT), 0, st, (const uint32_t *)bmaps, nb,
-                       bcin);
+    PROF("carry_blocks", 8.0 * zsum / CARRY_LPB);
+    hipLaunchKernelGGL(k_carry_blocks, dim3(1, 1, njobs), dim3(NT), 0, st, jr);
   }
   {
-    PROF("carry_fix", 17.0 * Z);
-    hipLaunchKernelGGL(k_carry_fix, dim3(nb), dim3(NT), 0, st, rp, (const uint8_t *)lmaps,
-                       (uint32_t)Z, (const uint32_t *)bcin);
+    PROF("carry_fix", 17.0 * zsum);
+    hipLaunchKernelGGL(k_carry_fix, dim3(nbmax, 1, njobs), dim3(NT), 0, st, jr);
   }
   HIPCHK(hipGetLastError());
   return SBA_OK;
+}
+
+// carve one job's buffers out of a workspace region of ws_need(N) bytes
+static void job_layout(Job &j, void *ws, uint32_t N) {
+  j.aspec = (uint32_t *)ws;
+  j.bspec = j.aspec + (size_t)SBA_K * N;
+  j.lmaps = (uint8_t *)(j.bspec + (size_t)SBA_K * N);
+  const size_t nbcap = N / CARRY_LPB + 2;
+  j.bmaps = (uint32_t *)(j.lmaps + nbcap * CARRY_LPB);
+  j.bcin = j.bmaps + nbcap;
 }
 
 static int get_ctx(DevCtx **out) {
@@ -946,17 +1009,21 @@ static int get_ctx(DevCtx **out) {
   return 0;
 }
 
+static int in_band(int64_t an, int64_t bn) {
+  if (an <= 0 || bn <= 0) return 0;
+  const uint64_t mn = (uint64_t)(an < bn ? an : bn);
+  if (mn > sba_gpu_min_operand_cap()) return 0;
+  return lg_for((uint64_t)an + (uint64_t)bn - 1) <= SBA_MAX_LG;
+}
+
 static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64_t *bp,
                       int64_t bn, int sqr, void *ws, size_t ws_bytes, void *stream) {
-  if (an <= 0 || bn <= 0) return SBA_BAND;
+  if (!in_band(an, bn)) return SBA_BAND;
   if (an < bn && !sqr) {
     const uint64_t *t = ap; ap = bp; bp = t;
     int64_t tn = an; an = bn; bn = tn;
   }
-  const uint64_t mn = (uint64_t)(an < bn ? an : bn);
-  if (mn > sba_gpu_min_operand_cap()) return SBA_BAND;
   const int L = lg_for((uint64_t)an + (uint64_t)bn - 1);
-  if (L > SBA_MAX_LG) return SBA_BAND;
   hipStream_t st = (hipStream_t)stream;
   std::lock_guard<std::mutex> lk(g_mu);
   DevCtx *c;
@@ -977,7 +1044,19 @@ static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64
   } else if (ws_bytes < need) {
     return SBA_ERR_WORKSPACE;
   }
-  return run_mul(*c, rp, ap, (uint64_t)an, sqr ? ap : bp, (uint64_t)bn, sqr, ws, st);
+  JobRef jr;
+  jr.arr = nullptr;
+  Job &j = jr.one;
+  j.a = ap;
+  j.b = sqr ? ap : bp;
+  j.r = rp;
+  j.an = (uint32_t)an;
+  j.bn = (uint32_t)bn;
+  j.zlen = (uint32_t)(an + bn - 1);
+  j.Z = (uint32_t)(an + bn);
+  j.nb = carry_blocks(j.Z);
+  job_layout(j, ws, 1u << L);
+  return enqueue_class(*c, L, sqr, jr, 1, j.nb, (double)an, (double)bn, (double)j.Z, st);
 }
 
 extern "C" int sba_gpu_mul_dev(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64_t *bp,
@@ -1000,6 +1079,265 @@ extern "C" int sba_gpu_sqr_dev_ws(uint64_t *rp, const uint64_t *ap, int64_t an, 
                                   size_t ws_bytes, void *stream) {
   if (!ws) return SBA_ERR_WORKSPACE;
   return mul_common(rp, ap, an, ap, an, 1, ws, ws_bytes, stream);
+}
+
+// ---------------------------------------------------------------------------
+// batches: many independent multiplies of uneven sizes. Jobs are grouped in
+// size classes (same N = 2^L, same mul/sqr kind); one launch sequence per
+// class covers all its jobs (grid z = job). The whole batch is captured once
+// into a hipGraph and replayed by sba_batch_run.
+// ---------------------------------------------------------------------------
+struct BatchClass {
+  int L, sqr;
+  uint32_t njobs, nbmax;
+  Job *djobs;                      // device array
+  double asum, bsum, zsum;
+};
+
+struct sba_batch {
+  int dev;
+  void *ws;                        // all jobs' workspaces
+  Job *djobs_all;
+  int nclass;
+  BatchClass cls[2 * (SBA_MAX_LG + 1)];
+  hipStream_t cap_stream;          // private stream used for capture
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  int use_graph;
+};
+
+static int batch_enqueue(sba_batch *b, hipStream_t st) {
+  DevCtx *c;
+  int rc = get_ctx(&c);
+  if (rc) return rc;
+  for (int k = 0; k < b->nclass; ++k) {
+    BatchClass &bc = b->cls[k];
+    JobRef jr;
+    jr.arr = bc.djobs;
+    memset(&jr.one, 0, sizeof jr.one);
+    rc = enqueue_class(*c, bc.L, bc.sqr, jr, bc.njobs, bc.nbmax, bc.asum, bc.bsum, bc.zsum, st);
+    if (rc != SBA_OK) return rc;
+  }
+  return SBA_OK;
+}
+
+extern "C" void sba_batch_destroy(sba_batch *b) {
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+  if (b->ws) (void)hipFree(b->ws);
+  if (b->djobs_all) (void)hipFree(b->djobs_all);
+  free(b);
+}
+
+extern "C" sba_batch *sba_batch_create(const sba_job *jobs, int njobs, int use_graph,
+                                       int *status) {
+  int st_dummy;
+  if (!status) status = &st_dummy;
+  *status = SBA_OK;
+  if (njobs <= 0) { *status = SBA_BAND; return nullptr; }
+  for (int i = 0; i < njobs; ++i)
+    if (!in_band(jobs[i].an, jobs[i].bp ? jobs[i].bn : jobs[i].an)) {
+      *status = SBA_BAND;
+      return nullptr;
+    }
+  std::lock_guard<std::mutex> lk(g_mu);
+  DevCtx *c;
+  if ((*status = get_ctx(&c)) != 0) return nullptr;
+  sba_batch *b = (sba_batch *)calloc(1, sizeof(sba_batch));
+  if (!b) { *status = SBA_ERR_HIP; return nullptr; }
+  b->use_graph = use_graph;
+  (void)hipGetDevice(&b->dev);
+  // classify: key (L, sqr); jobs ordered by class
+  int Lmax = 0;
+  size_t total = 0;
+  int count[2][SBA_MAX_LG + 1] = {};
+  for (int i = 0; i < njobs; ++i) {
+    const int sq = jobs[i].bp == nullptr;
+    const int L = lg_for((uint64_t)jobs[i].an + (uint64_t)(sq ? jobs[i].an : jobs[i].bn) - 1);
+    count[sq][L]++;
+    if (L > Lmax) Lmax = L;
+    total += (ws_need((size_t)1 << L) + 255) & ~(size_t)255;
+  }
+  int rc = ctx_twiddles(*c, Lmax, 0);
+  if (rc) { free(b); *status = rc; return nullptr; }
+  if (hipMalloc(&b->ws, total) != hipSuccess ||
+      hipMalloc(&b->djobs_all, sizeof(Job) * (size_t)njobs) != hipSuccess) {
+    sba_batch_destroy(b);
+    *status = SBA_ERR_HIP;
+    return nullptr;
+  }
+  Job *hj = (Job *)calloc((size_t)njobs, sizeof(Job));
+  int pos = 0;
+  size_t off = 0;
+  for (int sq = 0; sq < 2; ++sq)
+    for (int L = 0; L <= SBA_MAX_LG; ++L) {
+      if (!count[sq][L]) continue;
+      BatchClass &bc = b->cls[b->nclass++];
+      bc.L = L;
+      bc.sqr = sq;
+      bc.njobs = 0;
+      bc.nbmax = 0;
+      bc.djobs = b->djobs_all + pos;
+      bc.asum = bc.bsum = bc.zsum = 0;
+      for (int i = 0; i < njobs; ++i) {
+        const int isq = jobs[i].bp == nullptr;
+        const uint64_t an = (uint64_t)jobs[i].an, bn = isq ? an : (uint64_t)jobs[i].bn;
+        if (isq != sq || lg_for(an + bn - 1) != L) continue;
+        Job &j = hj[pos++];
+        // larger operand first (the first pass of b then covers the shorter one)
+        const bool swap = !isq && an < bn;
+        j.a = swap ? jobs[i].bp : jobs[i].ap;
+        j.b = isq ? jobs[i].ap : (swap ? jobs[i].ap : jobs[i].bp);
+        j.an = (uint32_t)(swap ? bn : an);
+        j.bn = (uint32_t)(swap ? an : bn);
+        j.r = jobs[i].rp;
+        j.zlen = (uint32_t)(an + bn - 1);
+        j.Z = (uint32_t)(an + bn);
+        j.nb = carry_blocks(j.Z);
+        job_layout(j, (char *)b->ws + off, 1u << L);
+        off += (ws_need((size_t)1 << L) + 255) & ~(size_t)255;
+        bc.njobs++;
+        if (j.nb > bc.nbmax) bc.nbmax = j.nb;
+        bc.asum += j.an;
+        bc.bsum += j.bn;
+        bc.zsum += j.Z;
+      }
+    }
+  hipError_t e = hipMemcpy(b->djobs_all, hj, sizeof(Job) * (size_t)njobs, hipMemcpyHostToDevice);
+  free(hj);
+  if (e != hipSuccess) { sba_batch_destroy(b); *status = SBA_ERR_HIP; return nullptr; }
+  if (use_graph) {
+    if (hipStreamCreate(&b->cap_stream) != hipSuccess ||
+        hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      sba_batch_destroy(b);
+      *status = SBA_ERR_HIP;
+      return nullptr;
+    }
+    const int saved = g_prof_on;
+    g_prof_on = 0;                               // no event records inside the graph
+    rc = batch_enqueue(b, b->cap_stream);
+    g_prof_on = saved;
+    hipError_t e2 = hipStreamEndCapture(b->cap_stream, &b->graph);
+    if (rc != SBA_OK || e2 != hipSuccess ||
+        hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+      sba_batch_destroy(b);
+      *status = rc != SBA_OK ? rc : SBA_ERR_HIP;
+      return nullptr;
+    }
+  }
+  return b;
+}
+
+extern "C" int sba_batch_run(sba_batch *b, void *stream) {
+  if (!b) return SBA_ERR_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  if (b->use_graph) {
+    HIPCHK(hipGraphLaunch(b->exec, st));
+    return SBA_OK;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  return batch_enqueue(b, st);
+}
+
+extern "C" int sba_batch_classes(const sba_batch *b) { return b ? b->nclass : 0; }
+
+// ---------------------------------------------------------------------------
+// one-shot batch launch into a caller workspace (serving: no allocation, no
+// graph; the job table is staged through the workspace's tail)
+// ---------------------------------------------------------------------------
+static size_t job_ws(const sba_job &j) {
+  const uint64_t an = (uint64_t)j.an, bn = j.bp ? (uint64_t)j.bn : an;
+  return (ws_need((size_t)1 << lg_for(an + bn - 1)) + 255) & ~(size_t)255;
+}
+
+extern "C" size_t sba_batch_ws_bytes(const sba_job *jobs, int njobs) {
+  size_t t = 0;
+  for (int i = 0; i < njobs; ++i) {
+    if (!in_band(jobs[i].an, jobs[i].bp ? jobs[i].bn : jobs[i].an)) return 0;
+    t += job_ws(jobs[i]);
+  }
+  return t + (((size_t)njobs * sizeof(Job) + 255) & ~(size_t)255);
+}
+
+extern "C" int sba_batch_launch_ws(const sba_job *jobs, int njobs, void *ws, size_t ws_bytes,
+                                   void *stream) {
+  if (njobs <= 0) return SBA_BAND;
+  const size_t need = sba_batch_ws_bytes(jobs, njobs);
+  if (!need) return SBA_BAND;
+  if (!ws || ws_bytes < need) return SBA_ERR_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lk(g_mu);
+  DevCtx *c;
+  int rc = get_ctx(&c);
+  if (rc) return rc;
+  int Lmax = 0;
+  for (int i = 0; i < njobs; ++i) {
+    const uint64_t an = (uint64_t)jobs[i].an, bn = jobs[i].bp ? (uint64_t)jobs[i].bn : an;
+    const int L = lg_for(an + bn - 1);
+    if (L > Lmax) Lmax = L;
+  }
+  if ((rc = ctx_twiddles(*c, Lmax, st)) != 0) return rc;
+  // job table at the tail of the workspace
+  Job *dj = (Job *)((char *)ws + need - (((size_t)njobs * sizeof(Job) + 255) & ~(size_t)255));
+  Job *hj = (Job *)calloc((size_t)njobs, sizeof(Job));
+  if (!hj) return SBA_ERR_HIP;
+  struct Cls { int L, sqr, start, n; uint32_t nbmax; double as, bs, zs; } cls[2 * (SBA_MAX_LG + 1)];
+  int ncls = 0, pos = 0;
+  size_t off = 0;
+  for (int sq = 0; sq < 2; ++sq)
+    for (int L = 0; L <= SBA_MAX_LG; ++L) {
+      Cls k = {L, sq, pos, 0, 0, 0, 0, 0};
+      for (int i = 0; i < njobs; ++i) {
+        const int isq = jobs[i].bp == nullptr;
+        const uint64_t an = (uint64_t)jobs[i].an, bn = isq ? an : (uint64_t)jobs[i].bn;
+        if (isq != sq || lg_for(an + bn - 1) != L) continue;
+        Job &j = hj[pos++];
+        const bool swap = !isq && an < bn;
+        j.a = swap ? jobs[i].bp : jobs[i].ap;
+        j.b = isq ? jobs[i].ap : (swap ? jobs[i].ap : jobs[i].bp);
+        j.an = (uint32_t)(swap ? bn : an);
+        j.bn = (uint32_t)(swap ? an : bn);
+        j.r = jobs[i].rp;
+        j.zlen = (uint32_t)(an + bn - 1);
+        j.Z = (uint32_t)(an + bn);
+        j.nb = carry_blocks(j.Z);
+        job_layout(j, (char *)ws + off, 1u << L);
+        off += job_ws(jobs[i]);
+        k.n++;
+        if (j.nb > k.nbmax) k.nbmax = j.nb;
+        k.as += j.an;
+        k.bs += j.bn;
+        k.zs += j.Z;
+      }
+      if (k.n) cls[ncls++] = k;
+    }
+  hipError_t e = hipMemcpyAsync(dj, hj, sizeof(Job) * (size_t)njobs, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);   // hj is pageable and freed below
+  free(hj);
+  if (e != hipSuccess) return SBA_ERR_HIP;
+  for (int k = 0; k < ncls; ++k) {
+    JobRef jr;
+    jr.arr = dj + cls[k].start;
+    memset(&jr.one, 0, sizeof jr.one);
+    rc = enqueue_class(*c, cls[k].L, cls[k].sqr, jr, (uint32_t)cls[k].n, cls[k].nbmax, cls[k].as,
+                       cls[k].bs, cls[k].zs, st);
+    if (rc != SBA_OK) return rc;
+  }
+  return SBA_OK;
+}
+
+// the plain per-call path over the same jobs: one sba_gpu_mul_dev /
+// sba_gpu_sqr_dev call per job, in order, on one stream
+extern "C" int sba_gpu_mul_loop(const sba_job *jobs, int njobs, void *stream) {
+  for (int i = 0; i < njobs; ++i) {
+    const sba_job &j = jobs[i];
+    const int rc = j.bp ? sba_gpu_mul_dev(j.rp, j.ap, j.an, j.bp, j.bn, stream)
+                        : sba_gpu_sqr_dev(j.rp, j.ap, j.an, stream);
+    if (rc != SBA_OK) return rc;
+  }
+  return SBA_OK;
 }
 
 extern "C" int sba_gpu_device_count(void) {

@@ -124,3 +124,29 @@ def from_int(v: int, n: int) -> np.ndarray:
 def all_ones_product(an: int, bn: int) -> np.ndarray:
     """(2^64a - 1)(2^64b - 1) = 2^64(a+b) - 2^64a - 2^64b + 1, as limbs."""
     return from_int(((1 << (64 * an)) - 1) * ((1 << (64 * bn)) - 1), an + bn)
+
+
+def batch_workload(n: int = 4096, seed: int = 4, lo: int = 64, hi: int = 65536,
+                   sparse: float = 0.10, ones: float = 0.05):
+    """The batched mixed-size workload: n independent pairs, an and bn drawn
+    independently log-uniform in [lo, hi] limbs; a `sparse` fraction of pairs
+    with 90% zero limbs, a `ones` fraction all-ones, the rest uniform random
+    u64. Returns a list of (a, b) uint64 arrays."""
+    rng = np.random.default_rng(seed)
+    span = np.log2(hi / lo)
+    out = []
+    for _ in range(n):
+        an = int(lo * 2 ** (rng.random() * span))
+        bn = int(lo * 2 ** (rng.random() * span))
+        u = rng.random()
+        if u < ones:
+            a = np.full(an, _M, dtype=np.uint64)
+            b = np.full(bn, _M, dtype=np.uint64)
+        else:
+            a = rng.integers(0, 2**64, an, dtype=np.uint64)
+            b = rng.integers(0, 2**64, bn, dtype=np.uint64)
+            if u < ones + sparse:
+                a[rng.random(an) < 0.9] = 0
+                b[rng.random(bn) < 0.9] = 0
+        out.append((a, b))
+    return out

@@ -193,3 +193,63 @@ def test_device_resident_matches_host():
         torch.cuda.synchronize()
         ok, r = sba.nt_sqr(a)
         assert (ts.cpu().numpy().view(np.uint64) == r).all()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_mixed_sizes_vs_oracle(use_graph):
+    torch = pytest.importorskip("torch")
+    from simd_bigint_amd.workloads import batch_workload
+    pairs = batch_workload(150, 123, hi=20000)
+    jobs, want = [], []
+    for i, (a, b) in enumerate(pairs):
+        ta = torch.from_numpy(a.view(np.int64)).cuda()
+        if i % 7 == 3:                       # a few squares
+            tr = torch.zeros(2 * len(a), dtype=torch.int64, device="cuda")
+            jobs.append((tr, ta, None))
+            want.append(orc.nt_sqr(a))
+        else:
+            tb = torch.from_numpy(b.view(np.int64)).cuda()
+            tr = torch.zeros(len(a) + len(b), dtype=torch.int64, device="cuda")
+            jobs.append((tr, ta, tb))
+            want.append(orc.nt_mul(a, b))
+    bt = sba.Batch(jobs, use_graph=use_graph)
+    assert bt.classes >= 4
+    for rep in range(3):                     # replays (graph) agree every time
+        for (tr, _, _) in jobs:
+            tr.zero_()
+        bt.run()
+        torch.cuda.synchronize()
+        for k, (tr, _, _) in enumerate(jobs):
+            assert (tr.cpu().numpy().view(np.uint64) == want[k]).all(), (rep, k)
+    bt.close()
+
+
+def test_serving_results_same_alone_and_in_mix():
+    """Continuous batching: each result of the served mix (Poisson arrivals,
+    80% multiplies / 20% squares, admission by an HBM budget) is bit-identical
+    to the same request run alone, and to the Python integer product."""
+    torch = pytest.importorskip("torch")
+    from simd_bigint_amd import serving
+    rng = np.random.default_rng(8)
+    pool_np = rng.integers(0, 2**64, 1 << 18, dtype=np.uint64)
+    pool = torch.from_numpy(pool_np.view(np.int64)).cuda()
+    reqs = serving.poisson_requests(120, 4000.0, pool, seed=9, lo=256, hi=1 << 15)
+    budget = 48 << 20
+    srv = serving.Server(budget, nstreams=3, max_batch=32)
+    stats = srv.run(reqs)
+    s = stats.summary()
+    say("serve", s)
+    assert s["requests"] == len(reqs) and stats.peak_hbm_bytes <= budget
+    assert stats.batches < len(reqs)          # requests were batched together
+    for q in reqs:
+        alone = torch.empty(q.limbs, dtype=torch.int64, device="cuda")
+        if q.b is None:
+            assert sba.sqr_dev(alone, q.a) == 1
+        else:
+            assert sba.mul_dev(alone, q.a, q.b) == 1
+        torch.cuda.synchronize()
+        assert torch.equal(alone, q.r), q.rid
+    for q in reqs[::10]:
+        a = to_int(q.a.cpu().numpy().view(np.uint64))
+        b = a if q.b is None else to_int(q.b.cpu().numpy().view(np.uint64))
+        assert to_int(q.r.cpu().numpy().view(np.uint64)) == a * b
