@@ -55,7 +55,25 @@ struct GarnerArgs {
 #ifndef SBA_BMAX
 #define SBA_BMAX 5
 #endif
-__device__ __forceinline__ uint32_t lpad(uint32_t i) { return i + (i >> 5); }
+// 1: the first forward pass runs one prime per block (grid y = prime)
+#ifndef SBA_FIRST_SPLIT
+#define SBA_FIRST_SPLIT 1
+#endif
+// 1: widest chunks even where group addressing is not linear
+#ifndef SBA_CHUNK_GREEDY
+#define SBA_CHUNK_GREEDY 0
+#endif
+// LDS padding: PW words after every 32-word row. A column tile (TC columns
+// of 2^ELL rows) pads by TC words for TC <= 16 -- the last chunk then puts
+// the TC columns x 32/TC groups of a half-wave on 32 distinct banks -- and
+// by 1 word otherwise (the contiguous tiles).
+template <int LOGTC> struct Pad {
+  static constexpr int PW = LOGTC <= 4 ? (1 << LOGTC) : 1;
+  static constexpr int words(int ne) { return ne + (ne / 32) * PW; }
+};
+template <int LOGTC> __device__ __forceinline__ uint32_t lpadt(uint32_t i) {
+  return i + (i >> 5) * Pad<LOGTC>::PW;
+}
 
 // blockIdx -> tile so that consecutive tiles (which share cache lines and
 // twiddle rows) run on the same XCD: the dispatcher hands block b to XCD
@@ -92,7 +110,8 @@ template <int ELL, int LOGTC, int T0, int B> struct ChunkGeom {
   static constexpr int LB = ELL - T0 - B;
   static constexpr int LOGD = LB + LOGTC;
   static constexpr bool LINEAR = LOGD >= 5 || LOGD + B <= 5;
-  static constexpr int DP = LOGD >= 5 ? (1 << LOGD) + (1 << (LOGD - 5)) : (1 << LOGD);
+  static constexpr int DP =
+      LOGD >= 5 ? (1 << LOGD) + (1 << (LOGD - 5)) * Pad<LOGTC>::PW : (1 << LOGD);
 };
 
 template <int ELL, int LOGTC, int T0, int B, bool INV>
@@ -115,11 +134,11 @@ __device__ __forceinline__ void tile_chunk(uint32_t *lds, const uint32_t *__rest
     const uint32_t kbase = (hb << (ELL - T0)) | lo;
     const uint32_t hj = (H << T0) | hb;
     const uint32_t i0 = (kbase << LOGTC) | c;
-    uint32_t *l0 = lds + lpad(i0);
+    uint32_t *l0 = lds + lpadt<LOGTC>(i0);
     uint32_t v[1 << B];
 #pragma unroll
     for (int m = 0; m < (1 << B); ++m)
-      v[m] = G::LINEAR ? l0[m * G::DP] : lds[lpad(i0 + ((uint32_t)m << G::LOGD))];
+      v[m] = G::LINEAR ? l0[m * G::DP] : lds[lpadt<LOGTC>(i0 + ((uint32_t)m << G::LOGD))];
     if (!INV) {
 #pragma unroll
       for (int u = 0; u < B; ++u) {
@@ -152,7 +171,7 @@ __device__ __forceinline__ void tile_chunk(uint32_t *lds, const uint32_t *__rest
 #pragma unroll
     for (int m = 0; m < (1 << B); ++m) {
       if (G::LINEAR) l0[m * G::DP] = v[m];
-      else lds[lpad(i0 + ((uint32_t)m << G::LOGD))] = v[m];
+      else lds[lpadt<LOGTC>(i0 + ((uint32_t)m << G::LOGD))] = v[m];
     }
   }
 }
@@ -166,6 +185,7 @@ template <int ELL, int LOGTC> struct ChunkPlan {
   static constexpr int pick(int t0) {
     int r = ELL - t0;
     int bmax = r < BM ? r : BM;
+    if (SBA_CHUNK_GREEDY) return bmax;
     for (int b = bmax; b >= 1; --b) {
       int logd = ELL - t0 - b + LOGTC;
       if (logd >= 5 || logd + b <= 5) return b;
@@ -257,7 +277,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd(PrimeArgs pa, JobRe
   constexpr int TC = 1 << LOGTC;
   constexpr int NE = 1 << (ELL + LOGTC);
   constexpr int E = NE / NT;
-  __shared__ uint32_t lds[NE + NE / 32];
+  __shared__ uint32_t lds[Pad<LOGTC>::words(NE)];
   const Job jb = get_job(jr);
   gc_u64 *limbs = (gc_u64 *)(which ? jb.b : jb.a);
   g_u32 *spec = (g_u32 *)(which ? jb.bspec : jb.aspec);
@@ -267,20 +287,23 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd(PrimeArgs pa, JobRe
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t col0 = tile << LOGTC;
   const uint32_t nl = nlimbs < N ? nlimbs : N;
-  uint32_t *lt = lds + lpad(threadIdx.x);
+  uint32_t *lt = lds + lpadt<LOGTC>(threadIdx.x);
   // element e = tid + i*NT sits at col0 + toff + uoff(i): a per-thread part
   // and a uniform part (scalar base + 32-bit lane offset addressing)
   const uint32_t toff = (threadIdx.x >> LOGTC) * S + (threadIdx.x & (TC - 1));
+  // SBA_FIRST_SPLIT: one prime per block (grid y), else all primes per block
+  const int q0 = SBA_FIRST_SPLIT ? (int)blockIdx.y : 0;
+  const int q1 = SBA_FIRST_SPLIT ? q0 + 1 : SBA_K;
 #pragma unroll 1
-  for (int q = 0; q < SBA_K; ++q) {
+  for (int q = q0; q < q1; ++q) {
     const uint32_t p = pa.p[q], pinv = pa.pinv[q], r2 = pa.r2[q];
     __syncthreads();
-    // the tile's limbs are re-read per prime (L2-resident after the first)
+    // the tile's limbs are re-read per prime (cache-resident after the first)
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
       gc_u64 *lp = limbs + uoff;
-      lt[i * (NT + NT / 32)] = limb_residue(uoff + toff < nl ? lp[toff] : 0, p, pinv, r2);
+      lt[i * Pad<LOGTC>::words(NT)] = limb_residue(uoff + toff < nl ? lp[toff] : 0, p, pinv, r2);
     }
     tile_fwd<ELL, LOGTC, 0>(lds, pa.wn[q], 0, p, pinv);
     __syncthreads();
@@ -288,7 +311,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd(PrimeArgs pa, JobRe
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
-      (out + uoff)[toff] = lt[i * (NT + NT / 32)];
+      (out + uoff)[toff] = lt[i * Pad<LOGTC>::words(NT)];
     }
   }
 }
@@ -301,7 +324,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_pass(PrimeArgs pa, JobRef jr,
   constexpr int TC = 1 << LOGTC;
   constexpr int NE = 1 << (ELL + LOGTC);
   constexpr int E = NE / NT;
-  __shared__ uint32_t lds[NE + NE / 32];
+  __shared__ uint32_t lds[Pad<LOGTC>::words(NE)];
   const Job jb = get_job(jr);
   g_u32 *spec = (g_u32 *)(which ? jb.bspec : jb.aspec);
   const int q = blockIdx.y;
@@ -314,12 +337,12 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_pass(PrimeArgs pa, JobRef jr,
   const uint32_t base = (H << (L - s0)) + col0;
   g_u32 *buf = spec + q * N + base;
   const uint32_t p = pa.p[q], pinv = pa.pinv[q];
-  uint32_t *lt = lds + lpad(threadIdx.x);
+  uint32_t *lt = lds + lpadt<LOGTC>(threadIdx.x);
   const uint32_t toff = (threadIdx.x >> LOGTC) * S + (threadIdx.x & (TC - 1));
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
-    lt[i * (NT + NT / 32)] = (buf + uoff)[toff];
+    lt[i * Pad<LOGTC>::words(NT)] = (buf + uoff)[toff];
   }
   if (!INV) tile_fwd<ELL, LOGTC, 0>(lds, pa.wn[q], H, p, pinv);
   else tile_inv<ELL, LOGTC, 0>(lds, pa.wni[q], H, p, pinv);
@@ -327,7 +350,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_pass(PrimeArgs pa, JobRef jr,
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
-    (buf + uoff)[toff] = lt[i * (NT + NT / 32)];
+    (buf + uoff)[toff] = lt[i * Pad<LOGTC>::words(NT)];
   }
 }
 
@@ -338,7 +361,7 @@ template <int ELL, bool LIMB_IN, bool SQR>
 __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr, int L) {
   constexpr int NE = 1 << ELL;
   constexpr int E = NE / NT;
-  __shared__ uint32_t lds[NE + NE / 32];
+  __shared__ uint32_t lds[Pad<0>::words(NE)];
   const Job jb = get_job(jr);
   const int q = blockIdx.y;
   const uint32_t N = 1u << L;
@@ -349,7 +372,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
   g_u32 *buf = (g_u32 *)jb.bspec + q * N + base;
   gc_u32 *as = (gc_u32 *)jb.aspec + q * N + base;
   gc_u64 *bl = (gc_u64 *)jb.b + base;
-  uint32_t *lt = lds + lpad(threadIdx.x);
+  uint32_t *lt = lds + lpadt<0>(threadIdx.x);
   const uint32_t tid = threadIdx.x;
 #pragma unroll
   for (int i = 0; i < E; ++i) {
@@ -359,20 +382,20 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
     } else {
       v = (buf + i * NT)[tid];
     }
-    lt[i * (NT + NT / 32)] = v;
+    lt[i * Pad<0>::words(NT)] = v;
   }
   tile_fwd<ELL, 0, 0>(lds, pa.wn[q], H, p, pinv);
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < E; ++i) {
-    const uint32_t v = lt[i * (NT + NT / 32)];
+    const uint32_t v = lt[i * Pad<0>::words(NT)];
     const uint32_t o = SQR ? red1(v, p) : red1((as + i * NT)[tid], p);
-    lt[i * (NT + NT / 32)] = mont_mul(v, o, p, pinv);
+    lt[i * Pad<0>::words(NT)] = mont_mul(v, o, p, pinv);
   }
   tile_inv<ELL, 0, 0>(lds, pa.wni[q], H, p, pinv);
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < E; ++i) (buf + i * NT)[tid] = lt[i * (NT + NT / 32)];
+  for (int i = 0; i < E; ++i) (buf + i * NT)[tid] = lt[i * Pad<0>::words(NT)];
 }
 
 // ---------------------------------------------------------------------------
@@ -926,7 +949,8 @@ static int enqueue_class(DevCtx &c, int L, int sqr, const JobRef &jr, uint32_t n
   if (L <= LAST_ELL) {
     if (!sqr) {
       PROF("first_fwd", 8.0 * asum + KN4);
-      hipLaunchKernelGGL(first_single(L), dim3(1, 1, njobs), dim3(NT), 0, st, pa, jr, 0, L);
+      hipLaunchKernelGGL(first_single(L), dim3(1, SBA_FIRST_SPLIT ? SBA_K : 1, njobs), dim3(NT),
+                         0, st, pa, jr, 0, L);
     }
     {
       PROF("fused_last", 8.0 * bsum + (sqr ? 1 : 2) * KN4);
@@ -940,11 +964,13 @@ static int enqueue_class(DevCtx &c, int L, int sqr, const JobRef &jr, uint32_t n
     first_fn f1 = first_two_pass(ell1);
     if (!sqr) {
       PROF("first_fwd", 8.0 * asum + KN4);
-      hipLaunchKernelGGL(f1, dim3(ntile1, 1, njobs), dim3(NT), 0, st, pa, jr, 0, L);
+      hipLaunchKernelGGL(f1, dim3(ntile1, SBA_FIRST_SPLIT ? SBA_K : 1, njobs), dim3(NT), 0, st,
+                         pa, jr, 0, L);
     }
     {
       PROF("first_fwd", 8.0 * bsum + KN4);
-      hipLaunchKernelGGL(f1, dim3(ntile1, 1, njobs), dim3(NT), 0, st, pa, jr, 1, L);
+      hipLaunchKernelGGL(f1, dim3(ntile1, SBA_FIRST_SPLIT ? SBA_K : 1, njobs), dim3(NT), 0, st,
+                         pa, jr, 1, L);
     }
     if (!sqr) {
       PROF("fwd_last", 2 * KN4);
