@@ -85,13 +85,31 @@ def cpu_baseline(a, b, budget_s):
                       "(mean %.3f ns/limb)" % (what, len(times), len(times), mean * 1e9 / n)}, r
 
 
+def _time_batch(bt, stream, reps=5):
+    import torch
+    bt.run(stream)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        bt.run(stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
 def batch_leg(njobs, L):
     """4096 independent multiplies, an and bn log-uniform in [64, 65536]
     limbs (10% sparse, 5% all-ones), resident in HBM: one hipGraph replay of
-    the batch vs the plain per-call GPU path over the same jobs."""
+    the batch vs the plain per-call GPU path over the same jobs. The batch
+    is timed under four plan tables: the hand-written rule, the shipped
+    (MI355X-measured) table, the autotuner's exhaustive choice on these jobs
+    and its cost-model-guided choice (2 timings per size class)."""
     import numpy as np
     import torch
     import simd_bigint_amd as sba
+    from simd_bigint_amd import tuning
     from simd_bigint_amd.workloads import batch_workload, to_int
 
     pairs = batch_workload(njobs, seed=4)
@@ -104,24 +122,63 @@ def batch_leg(njobs, L):
         jobs2.append((torch.empty(len(a) + len(b), dtype=torch.int64, device=dev), ta, tb))
     limbs = sum(len(a) + len(b) for a, b in pairs)
     stream = torch.cuda.Stream(device=dev)
+    keys = set()
+    for r, a, b in jobs:
+        L_, sq = ctypes.c_int(), ctypes.c_int()
+        arr = sba.job_array([(r, a, b)])
+        sba.lib().sba_job_class(ctypes.cast(arr, ctypes.c_void_p), ctypes.byref(L_),
+                                ctypes.byref(sq))
+        keys.add((L_.value, sq.value))
+    plans = {}
+
+    def timed(label):
+        bt = sba.Batch(jobs, use_graph=True)
+        with torch.cuda.stream(stream):
+            ms = _time_batch(bt, stream)
+        ncls = bt.classes
+        bt.close()
+        plans[label] = round(ms, 4)
+        log("bench: batch of %d under %s plans: %.3f ms (%d size classes)" %
+            (njobs, label, ms, ncls))
+        return ms, ncls
+
+    # the hand-written rule for every class of the batch
+    for L_, sq in keys:
+        for bk in range(sba.lib().sba_plan_bucket(1 << 30) + 1):
+            tuning.plan_set(L_, sq, bk, tuning.plan_heuristic(L_))
+    timed("heuristic")
+    tuning.plan_reset()
+    timed("shipped")
+    t0 = time.perf_counter()
+    n_ex = tuning.autotune(jobs, mode=tuning.EXHAUSTIVE, reps=3, stream=stream)
+    t_ex = time.perf_counter() - t0
+    timed("exhaustive")
+    tuning.plan_reset()
+    n_model = None
+    t_model = None
+    if os.path.exists(tuning.MODEL_PATH):
+        tuning.CostModel.load().install()
+        t0 = time.perf_counter()
+        n_model = tuning.autotune(jobs, mode=tuning.MODEL, topk=2, reps=3, stream=stream)
+        t_model = time.perf_counter() - t0
+        ms_batch, ncls = timed("model_top2")
+    else:
+        ms_batch, ncls = timed("shipped")
+    plans.update({"exhaustive_timings": n_ex, "exhaustive_tune_s": round(t_ex, 3),
+                  "model_timings": n_model,
+                  "model_tune_s": None if t_model is None else round(t_model, 3)})
+    # leave the batch in the products of the final (model-tuned) plans
     bt = sba.Batch(jobs, use_graph=True)
-    with torch.cuda.stream(stream):
-        bt.run(stream)
+    bt.run(stream)
     torch.cuda.synchronize()
-    reps = 5
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(reps):
-        bt.run(stream)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms_batch = e0.elapsed_time(e1) / reps
-    log("bench: batch of %d in %.3f ms (%d size classes)" % (njobs, ms_batch, bt.classes))
+    bt.close()
+    tuning.plan_reset()
     rc = sba.mul_loop(jobs2, stream)
     torch.cuda.synchronize()
     if rc != 1:
         raise RuntimeError("per-call loop returned %d" % rc)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     t0 = time.perf_counter()
     for _ in range(2):
@@ -137,15 +194,15 @@ def batch_leg(njobs, L):
         want = to_int(a) * to_int(b)
         got = to_int(jobs[k][0].cpu().numpy().view(np.uint64))
         sample_ok &= got == want
-    bt.close()
-    return {"jobs": njobs, "limbs_total": int(limbs), "size_classes": bt.classes,
+    return {"jobs": njobs, "limbs_total": int(limbs), "size_classes": ncls,
             "batched_ms": round(ms_batch, 4), "per_call_ms": round(ms_loop, 4),
             "batched_muls_per_s": round(njobs / (ms_batch * 1e-3), 1),
             "per_call_muls_per_s": round(njobs / (ms_loop * 1e-3), 1),
             "speedup": round(ms_loop / ms_batch, 3),
+            "plans_ms": plans,
             "bit_exact": bool(same and sample_ok),
-            "check": "all batched products == per-call products; 32 sampled products == "
-                     "Python integer products"}
+            "check": "all batched products (cost-model-tuned plans) == per-call products "
+                     "(shipped plans); 32 sampled products == Python integer products"}
 
 
 def serve_leg(nreq, rate):

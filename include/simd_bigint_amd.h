@@ -108,6 +108,16 @@ int sba_gpu_mul_dev_ws(uint64_t *rp, const uint64_t *ap, int64_t an,
 int sba_gpu_sqr_dev_ws(uint64_t *rp, const uint64_t *ap, int64_t an, void *ws,
                        size_t ws_bytes, void *stream);
 size_t sba_gpu_workspace_bytes(int64_t an, int64_t bn);
+/* 1 when the device entries accept (an, bn): min(an, bn) within the CRT
+ * cap and either an + bn - 1 <= 2^25 (one transform) or a chunked product
+ * (the longer operand cut into chunks against the shorter one's spectrum,
+ * an + bn <= 2^29). Batches take only the one-transform band. */
+int sba_gpu_in_band(int64_t an, int64_t bn);
+/* how a single product of (an, bn) runs: *nchunks == 0 -> one transform;
+ * else chunks of *chunk limbs of the longer operand against the shorter
+ * one's spectrum of length 2^*Lc */
+int sba_gpu_chunking(int64_t an, int64_t bn, int sqr, int *Lc, uint64_t *chunk,
+                     uint64_t *nchunks);
 uint64_t sba_gpu_min_operand_cap(void);
 int sba_gpu_max_lg(void);
 int sba_gpu_device_count(void);
@@ -145,6 +155,89 @@ int sba_batch_launch_ws(const sba_job *jobs, int njobs, void *ws, size_t ws_byte
 int sba_gpu_mul_loop(const sba_job *jobs, int njobs, void *stream);
 
 /* ---------------------------------------------------------------------- */
+/* Plans and autotuning (no counterpart in the reference: its transform    */
+/* shape is fixed per size by code, include/p30.h:1847-1938). A plan cuts  */
+/* a transform of length 2^L into passes:                                  */
+/*   last   levels of the last pass, on contiguous tiles of 2^last         */
+/*          residues; last == L (L <= 13) is a single-tile transform,      */
+/*          else a first pass over column tiles of 2^13 residues takes the */
+/*          other L - last levels (last in 11..13, 1 <= L-last <= 12)      */
+/*   bm     widest register chunk in a tile: radix-2^bm stages (3..5)      */
+/*   split  first forward pass one prime per block (1) or all five        */
+/*          primes per block (0)                                           */
+/* The library keeps one plan per (L, square?, bucket of the class's job   */
+/* count); every launch (single call, batch class, serving batch) uses the */
+/* plan stored for its class at that time. The table starts as (and        */
+/* sba_plan_reset restores) the shipped one: the fastest plan per key      */
+/* measured on MI355X, the hand-written rule (sba_plan_heuristic: last =   */
+/* min(L, 13), bm 5, split 1) where a key was not measured. sba_plan_set   */
+/* returns SBA_BAND for an invalid plan.                                   */
+/* ---------------------------------------------------------------------- */
+#define SBA_PLAN_BUCKETS 6
+int sba_plan_bucket(int njobs);    /* 1-3 jobs: 0, 4-15: 1, ... >= 1024: 5 */
+int sba_plan_set(int L, int sqr, int bucket, int last, int bm, int split);
+int sba_plan_get(int L, int sqr, int bucket, int *last, int *bm, int *split);
+int sba_plan_heuristic(int L, int *last, int *bm, int *split);
+void sba_plan_reset(void);
+/* valid plans for length 2^L as (last, bm, split) triples; returns the
+ * count (writes at most max triples) */
+int sba_plan_candidates(int L, int *plans, int max);
+/* time one size class (all jobs the same L and kind) under a plan: one
+ * warm-up, then reps launches between HIP events on the stream; *ms = mean
+ * per launch. Workspace as sba_batch_launch_ws. Synchronizes the stream. */
+int sba_class_time(const sba_job *jobs, int njobs, int last, int bm, int split, int reps,
+                   void *ws, size_t ws_bytes, void *stream, float *ms);
+/* the size class of a job: *L = log2 of its transform length, *sqr = 1 for
+ * a square; SBA_BAND outside the band */
+int sba_job_class(const sba_job *j, int *L, int *sqr);
+
+/* Autotuner: for every size class of the jobs, time candidate plans on the
+ * class's own jobs (reps launches each, products written to the jobs' rp)
+ * and store the fastest for (L, kind, bucket of the class's job count).
+ *   SBA_TUNE_DEFAULT     no change
+ *   SBA_TUNE_EXHAUSTIVE  every valid plan is timed
+ *   SBA_TUNE_MODEL       the loaded cost model ranks the plans; the topk
+ *                        best-ranked are timed (topk = 1: none is, the
+ *                        model's first choice is stored)
+ * Workspace as sba_batch_launch_ws. *measured = plan timings made.
+ * SBA_ERR_WORKSPACE: workspace too small, or no model loaded for MODEL. */
+#define SBA_TUNE_DEFAULT 0
+#define SBA_TUNE_EXHAUSTIVE 1
+#define SBA_TUNE_MODEL 2
+int sba_autotune(const sba_job *jobs, int njobs, int mode, int topk, int reps, void *ws,
+                 size_t ws_bytes, void *stream, int *measured);
+
+/* Cost model: an MLP (SBA_CM_IN -> 32 -> 32 -> 1, ReLU) over plan
+ * features, predicting the standardised log launch time of a size class.
+ * Parameters: W1[32][16], b1[32], W2[32][32], b2[32], w3[32], b3
+ * (SBA_CM_NPARAM floats). The training entries take device pointers and
+ * enqueue on the stream:
+ *   sba_cm_forward   y[n] = f(x[n][SBA_CM_IN])
+ *   sba_cm_backward  grad = d/dparams of (1/n) sum (f(x) - t)^2; ws holds
+ *                    sba_cm_ws_floats(n) floats, predictions in ws[0..n)
+ *   sba_cm_adam      one Adam step (b1 0.9, b2 0.999, eps 1e-8), step >= 1
+ * sba_cm_features writes the raw feature row of (class, plan) (host);
+ * sba_cm_load installs host parameters plus the feature / target
+ * standardisation for the autotuner; sba_cm_predict returns its estimate
+ * in milliseconds. */
+#define SBA_CM_IN 16
+#define SBA_CM_NPARAM 1633
+int sba_cm_nparams(void);
+size_t sba_cm_ws_floats(int n);
+int sba_cm_forward(const float *params, const float *x, int n, float *y, void *stream);
+int sba_cm_backward(const float *params, const float *x, const float *t, int n, float *grad,
+                    float *ws, void *stream);
+int sba_cm_adam(float *params, const float *grad, float *m, float *v, int step, float lr,
+                void *stream);
+int sba_cm_features(int L, int sqr, int njobs, double fill, int last, int bm, int split,
+                    float *f);
+int sba_cm_load(const float *params, const float *fmean, const float *fstd, float ymean,
+                float ystd);
+int sba_cm_loaded(void);
+int sba_cm_predict(int L, int sqr, int njobs, double fill, int last, int bm, int split,
+                   void *stream, double *ms);
+
+/* ---------------------------------------------------------------------- */
 /* Tuning and test hooks (no counterpart in the reference, whose tier      */
 /* thresholds are compile-time macros, include/u52-mparam.h:1-19).         */
 /* ---------------------------------------------------------------------- */
@@ -161,6 +254,10 @@ void sba_gpu_profile_reset(void);
 int sba_gpu_profile_get(int idx, const char **name, double *ms, double *bytes,
                         uint64_t *count);
 /* the CPU tier alone (basecase + Karatsuba), either operand order */
+/* 1 when the reference's nt_mul would accept (an, bn) (its p50 band,
+ * include/p50.h:1020-1046); nt_mul / nt_sqr / p50_* return 0 exactly
+ * outside it. */
+int sba_ref_nt_band(int64_t an, int64_t bn);
 void sba_cpu_mul(uint64_t *rp, const uint64_t *ap, uint64_t an,
                  const uint64_t *bp, uint64_t bn);
 

@@ -103,6 +103,56 @@ def lib() -> ctypes.CDLL:
     L.sba_batch_launch_ws.restype = ctypes.c_int
     L.sba_gpu_mul_loop.argtypes = [vp, ctypes.c_int, vp]
     L.sba_gpu_mul_loop.restype = ctypes.c_int
+    ip = ctypes.POINTER(ctypes.c_int)
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.sba_plan_bucket.argtypes = [ctypes.c_int]
+    L.sba_plan_bucket.restype = ctypes.c_int
+    L.sba_plan_set.argtypes = [ctypes.c_int] * 6
+    L.sba_plan_set.restype = ctypes.c_int
+    L.sba_plan_get.argtypes = [ctypes.c_int] * 3 + [ip] * 3
+    L.sba_plan_get.restype = ctypes.c_int
+    L.sba_plan_heuristic.argtypes = [ctypes.c_int] + [ip] * 3
+    L.sba_plan_heuristic.restype = ctypes.c_int
+    L.sba_plan_reset.argtypes = []
+    L.sba_plan_reset.restype = None
+    L.sba_plan_candidates.argtypes = [ctypes.c_int, ip, ctypes.c_int]
+    L.sba_plan_candidates.restype = ctypes.c_int
+    L.sba_class_time.argtypes = [vp, ctypes.c_int] + [ctypes.c_int] * 4 + [vp, ctypes.c_size_t,
+                                                                            vp, fp]
+    L.sba_class_time.restype = ctypes.c_int
+    L.sba_job_class.argtypes = [vp, ip, ip]
+    L.sba_job_class.restype = ctypes.c_int
+    L.sba_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp,
+                               ctypes.c_size_t, vp, ip]
+    L.sba_autotune.restype = ctypes.c_int
+    L.sba_cm_nparams.argtypes = []
+    L.sba_cm_nparams.restype = ctypes.c_int
+    L.sba_cm_ws_floats.argtypes = [ctypes.c_int]
+    L.sba_cm_ws_floats.restype = ctypes.c_size_t
+    L.sba_cm_forward.argtypes = [vp, vp, ctypes.c_int, vp, vp]
+    L.sba_cm_forward.restype = ctypes.c_int
+    L.sba_cm_backward.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, vp]
+    L.sba_cm_backward.restype = ctypes.c_int
+    L.sba_cm_adam.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_float, vp]
+    L.sba_cm_adam.restype = ctypes.c_int
+    L.sba_cm_features.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, fp]
+    L.sba_cm_features.restype = ctypes.c_int
+    L.sba_cm_load.argtypes = [fp, fp, fp, ctypes.c_float, ctypes.c_float]
+    L.sba_cm_load.restype = ctypes.c_int
+    L.sba_cm_loaded.argtypes = []
+    L.sba_cm_loaded.restype = ctypes.c_int
+    L.sba_cm_predict.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, vp,
+                                 ctypes.POINTER(ctypes.c_double)]
+    L.sba_cm_predict.restype = ctypes.c_int
+    L.sba_gpu_in_band.argtypes = [ctypes.c_int64, ctypes.c_int64]
+    L.sba_gpu_in_band.restype = ctypes.c_int
+    L.sba_ref_nt_band.argtypes = [ctypes.c_int64, ctypes.c_int64]
+    L.sba_ref_nt_band.restype = ctypes.c_int
+    L.sba_gpu_chunking.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ip,
+                                   ctypes.POINTER(u64), ctypes.POINTER(u64)]
+    L.sba_gpu_chunking.restype = ctypes.c_int
     L.sba_set_gpu_threshold.argtypes = [u64]
     L.sba_set_gpu_threshold.restype = None
     L.sba_get_gpu_threshold.argtypes = []
@@ -209,6 +259,16 @@ def sqr_dev(r, a, stream=None) -> int:
     _check_dev(r, 2 * a.numel())
     s = None if stream is None else ctypes.c_void_p(int(stream.cuda_stream))
     return lib().sba_gpu_sqr_dev(_ptr(r), _ptr(a), a.numel(), s)
+
+
+def chunking(an: int, bn: int, sqr: bool = False):
+    """(Lc, chunk limbs, chunks) of a single product; chunks == 0: one
+    transform. None outside the device band."""
+    Lc, C, n = ctypes.c_int(), ctypes.c_uint64(), ctypes.c_uint64()
+    if lib().sba_gpu_chunking(an, bn, 1 if sqr else 0, ctypes.byref(Lc), ctypes.byref(C),
+                              ctypes.byref(n)) != SBA_OK:
+        return None
+    return Lc.value, C.value, n.value
 
 
 def min_operand_cap() -> int:

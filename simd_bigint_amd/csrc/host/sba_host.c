@@ -51,9 +51,7 @@ static void grow(uint64_t **p, size_t *cap, size_t n) {
 
 /* H2D, multiply on the default stream, D2H. Returns the engine's code. */
 static int gpu_in_band(size_t an, size_t bn) {
-  uint64_t mn = an < bn ? an : bn;
-  return an > 0 && bn > 0 && mn <= sba_gpu_min_operand_cap() &&
-         (uint64_t)an + bn - 1 <= ((uint64_t)1 << sba_gpu_max_lg());
+  return sba_gpu_in_band((int64_t)an, (int64_t)bn);
 }
 
 static int gpu_mul_host(uint64_t *rp, const uint64_t *ap, size_t an,
@@ -81,16 +79,48 @@ static int gpu_mul_host(uint64_t *rp, const uint64_t *ap, size_t an,
 /* ------------------------------------------------------------------ */
 /* NTT dispatcher entries: include/ntmul.h:47-79 (1 ok / 0 beyond band) */
 /* ------------------------------------------------------------------ */
+/* The reference's nt_* return 0 only beyond the p50 band
+ * (include/ntmul.h:19,52-57; include/p50.h:1020-1046): the widest trunk
+ * T in {88, 84, 80} whose CRT capacity floor(p0 p1 / 2^T) * floor(p2 p3 /
+ * 2^T) (include/p50.h:161-169) admits ceil(64 min(an,bn) / T) trunks, and
+ * ceil((ceil(64 an / T) + ceil(64 bn / T) - 1) / 8) <= 2^22 vectors. The
+ * four 50-bit primes are include/p50.h:51-56; the capacities only depend on
+ * them. The GPU band contains this band. */
+static const uint64_t kP50Primes[4] = {1025844348715009ull, 1072023837081601ull,
+                                       1086317488242689ull, 1108307720798209ull};
+static int p50_band(ptrdiff_t an, ptrdiff_t bn) {
+  if (an <= 0 || bn <= 0) return 0;
+  const uint64_t mn = (uint64_t)(an < bn ? an : bn);
+  const unsigned __int128 a = (unsigned __int128)kP50Primes[0] * kP50Primes[1];
+  const unsigned __int128 b = (unsigned __int128)kP50Primes[2] * kP50Primes[3];
+  static const int kT[3] = {80, 84, 88};
+  int T = 0;
+  for (int i = 2; i >= 0 && !T; --i) {
+    const int t = kT[i];
+    const unsigned __int128 cap = (a >> t) * (b >> t);
+    const uint64_t capt = cap > (unsigned __int128)~0ull ? ~0ull : (uint64_t)cap;
+    if ((mn * 64 + (uint64_t)t - 1) / (uint64_t)t <= capt) T = t;
+  }
+  if (!T) return 0;
+  const uint64_t nat = ((uint64_t)an * 64 + T - 1) / T, nbt = ((uint64_t)bn * 64 + T - 1) / T;
+  return (nat + nbt - 1 + 7) / 8 <= ((uint64_t)1 << 22);
+}
+int sba_ref_nt_band(int64_t an, int64_t bn) { return p50_band((ptrdiff_t)an, (ptrdiff_t)bn); }
+
 int nt_mul_r(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, const uint64_t *bp,
              ptrdiff_t bn, struct scratch *sc) {
   (void)sc;
-  if (an <= 0 || bn <= 0) return 0;
-  return gpu_mul_host(rp, ap, (size_t)an, bp, (size_t)bn, 0) == SBA_OK;
+  if (!p50_band(an, bn)) return 0;
+  if (gpu_mul_host(rp, ap, (size_t)an, bp, (size_t)bn, 0) != SBA_OK)
+    die("GPU multiply inside the nt_mul band", SBA_BAND);
+  return 1;
 }
 int nt_sqr_r(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, struct scratch *sc) {
   (void)sc;
-  if (an <= 0) return 0;
-  return gpu_mul_host(rp, ap, (size_t)an, ap, (size_t)an, 1) == SBA_OK;
+  if (!p50_band(an, an)) return 0;
+  if (gpu_mul_host(rp, ap, (size_t)an, ap, (size_t)an, 1) != SBA_OK)
+    die("GPU square inside the nt_sqr band", SBA_BAND);
+  return 1;
 }
 int nt_mul(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, const uint64_t *bp,
            ptrdiff_t bn) {

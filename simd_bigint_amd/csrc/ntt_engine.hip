@@ -1,5 +1,7 @@
 // ntt_engine.hip -- gfx950 kernels of the multi-prime NTT multiply and the
-// thin C-ABI layer that launches them (declared in sba_gpu.h).
+// thin C-ABI layer that launches them (declared in include/simd_bigint_amd.h).
+// The pass kernels live in ntt_tiles.h; this file holds Garner, the carry
+// scan, the twiddle tables and all host code.
 //
 // Math (one prime p, transform length N = 2^L):
 //   * Root table in the FLINT / reference convention (include/p30.h:312-329,
@@ -22,381 +24,13 @@
 //   * Garner over the five residues (ascending primes) composes each
 //     coefficient into 3 u64 words; a 3-kernel carry-lookahead scan over a
 //     4-state carry map adds the words into limbs bit-exactly.
-#include "gpu_arith.h"
-#include "simd_bigint_amd.h"
+#include "ntt_tiles.h"
+#include "plan_table.h"
 
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <mutex>
-
-// ---------------------------------------------------------------------------
-// kernel argument block (by value)
-// ---------------------------------------------------------------------------
-struct PrimeArgs {
-  uint32_t p[SBA_K], pinv[SBA_K], r2[SBA_K];
-  const uint32_t *wn[SBA_K];
-  const uint32_t *wni[SBA_K];
-};
-
-struct GarnerArgs {
-  uint32_t p[SBA_K], pinv[SBA_K];
-  uint32_t scale[SBA_K];          // N^-1 * R^2 mod p (undoes N and the pointwise R^-1)
-  uint32_t cm[SBA_K][SBA_K];      // (prod_{j<k} p_j mod p_q) * R mod p_q
-  uint32_t im[SBA_K];             // (prod_{j<q} p_j)^-1 * R mod p_q
-};
-
-#define NT 256
-// occupancy hint for the transform kernels (waves per SIMD) and the widest
-// register chunk (radix 2^SBA_BMAX), fixed by measurement
-#ifndef SBA_WAVES
-#define SBA_WAVES 1
-#endif
-#ifndef SBA_BMAX
-#define SBA_BMAX 5
-#endif
-// 1: the first forward pass runs one prime per block (grid y = prime)
-#ifndef SBA_FIRST_SPLIT
-#define SBA_FIRST_SPLIT 1
-#endif
-// 1: widest chunks even where group addressing is not linear
-#ifndef SBA_CHUNK_GREEDY
-#define SBA_CHUNK_GREEDY 0
-#endif
-// LDS padding: PW words after every 32-word row. A column tile (TC columns
-// of 2^ELL rows) pads by TC words for TC <= 16 -- the last chunk then puts
-// the TC columns x 32/TC groups of a half-wave on 32 distinct banks -- and
-// by 1 word otherwise (the contiguous tiles).
-template <int LOGTC> struct Pad {
-  static constexpr int PW = LOGTC <= 4 ? (1 << LOGTC) : 1;
-  static constexpr int words(int ne) { return ne + (ne / 32) * PW; }
-};
-template <int LOGTC> __device__ __forceinline__ uint32_t lpadt(uint32_t i) {
-  return i + (i >> 5) * Pad<LOGTC>::PW;
-}
-
-// blockIdx -> tile so that consecutive tiles (which share cache lines and
-// twiddle rows) run on the same XCD: the dispatcher hands block b to XCD
-// b % 8.
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
-  if (n & 7) return b;
-  return (b & 7) * (n >> 3) + (b >> 3);
-}
-
-template <int X> struct ILog2 { static constexpr int v = 1 + ILog2<X / 2>::v; };
-template <> struct ILog2<1> { static constexpr int v = 0; };
-
-// residue of a limb, lazy in [0, 2p): (hi * 2^32 mod p) + (lo mod p).
-// p > 2^30 so lo < 4p: two folds bring lo into [0, p).
-__device__ __forceinline__ uint32_t limb_residue(uint64_t x, uint32_t p, uint32_t pinv,
-                                                 uint32_t r2) {
-  uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-  uint32_t h = red1(mont_mul(hi, r2, p, pinv), p);
-  uint32_t l2 = lo - 2u * p;
-  uint32_t l = l2 < lo ? l2 : lo;
-  return h + red1(l, p);
-}
-
-// ---------------------------------------------------------------------------
-// one chunk of B levels (local levels T0 .. T0+B-1 of an ELL-level pass)
-//
-// LDS index of tile element (k, c) is (k << LOGTC) | c, padded by one word
-// per 32 (lpad) against bank conflicts. A chunk's group of 2^B elements has
-// stride D = 2^(LB + LOGTC) words; when D >= 32, or when the whole group
-// sits inside one 32-word row (D * 2^B <= 32), the padded addresses are
-// base + m * const, so one address register serves the group (ds offsets).
-// ---------------------------------------------------------------------------
-template <int ELL, int LOGTC, int T0, int B> struct ChunkGeom {
-  static constexpr int LB = ELL - T0 - B;
-  static constexpr int LOGD = LB + LOGTC;
-  static constexpr bool LINEAR = LOGD >= 5 || LOGD + B <= 5;
-  static constexpr int DP =
-      LOGD >= 5 ? (1 << LOGD) + (1 << (LOGD - 5)) * Pad<LOGTC>::PW : (1 << LOGD);
-};
-
-template <int ELL, int LOGTC, int T0, int B, bool INV>
-__device__ __forceinline__ void tile_chunk(uint32_t *lds, const uint32_t *__restrict__ tw,
-                                           uint32_t H, uint32_t p, uint32_t pinv) {
-  constexpr int TC = 1 << LOGTC;
-  constexpr int NE = 1 << (ELL + LOGTC);
-  constexpr int GPT = NE / ((1 << B) * NT);
-  using G = ChunkGeom<ELL, LOGTC, T0, B>;
-  constexpr int LB = G::LB;
-  static_assert(GPT >= 1, "chunk too wide for the tile");
-  __syncthreads();
-#pragma unroll
-  for (int gi = 0; gi < GPT; ++gi) {
-    const uint32_t g = threadIdx.x + gi * NT;
-    const uint32_t c = g & (TC - 1);
-    const uint32_t rest = g >> LOGTC;
-    const uint32_t lo = rest & ((1u << LB) - 1);
-    const uint32_t hb = rest >> LB;
-    const uint32_t kbase = (hb << (ELL - T0)) | lo;
-    const uint32_t hj = (H << T0) | hb;
-    const uint32_t i0 = (kbase << LOGTC) | c;
-    uint32_t *l0 = lds + lpadt<LOGTC>(i0);
-    uint32_t v[1 << B];
-#pragma unroll
-    for (int m = 0; m < (1 << B); ++m)
-      v[m] = G::LINEAR ? l0[m * G::DP] : lds[lpadt<LOGTC>(i0 + ((uint32_t)m << G::LOGD))];
-    if (!INV) {
-#pragma unroll
-      for (int u = 0; u < B; ++u) {
-        const int half = 1 << (B - u - 1);
-#pragma unroll
-        for (int m2 = 0; m2 < (1 << u); ++m2) {
-          const uint32_t w = tw[(hj << u) | (uint32_t)m2];
-#pragma unroll
-          for (int r = 0; r < half; ++r) {
-            const int m = (m2 << (B - u)) | r;
-            bfly_fwd(v[m], v[m + half], w, p, pinv);
-          }
-        }
-      }
-    } else {
-#pragma unroll
-      for (int u = B - 1; u >= 0; --u) {
-        const int half = 1 << (B - u - 1);
-#pragma unroll
-        for (int m2 = 0; m2 < (1 << u); ++m2) {
-          const uint32_t w = tw[(hj << u) | (uint32_t)m2];
-#pragma unroll
-          for (int r = 0; r < half; ++r) {
-            const int m = (m2 << (B - u)) | r;
-            bfly_inv(v[m], v[m + half], w, p, pinv);
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < (1 << B); ++m) {
-      if (G::LINEAR) l0[m * G::DP] = v[m];
-      else lds[lpadt<LOGTC>(i0 + ((uint32_t)m << G::LOGD))] = v[m];
-    }
-  }
-}
-
-// chunk sizes: the largest B <= BM whose group addressing is linear,
-// falling back to BM when no size is
-template <int ELL, int LOGTC> struct ChunkPlan {
-  static constexpr int E = (1 << (ELL + LOGTC)) / NT;
-  static constexpr int LE = ILog2<E>::v;
-  static constexpr int BM = LE < SBA_BMAX ? LE : SBA_BMAX;
-  static constexpr int pick(int t0) {
-    int r = ELL - t0;
-    int bmax = r < BM ? r : BM;
-    if (SBA_CHUNK_GREEDY) return bmax;
-    for (int b = bmax; b >= 1; --b) {
-      int logd = ELL - t0 - b + LOGTC;
-      if (logd >= 5 || logd + b <= 5) return b;
-    }
-    return bmax;
-  }
-};
-
-template <int ELL, int LOGTC, int T0>
-__device__ __forceinline__ void tile_fwd(uint32_t *lds, const uint32_t *tw, uint32_t H,
-                                         uint32_t p, uint32_t pinv) {
-  if constexpr (T0 < ELL) {
-    constexpr int B = ChunkPlan<ELL, LOGTC>::pick(T0);
-    tile_chunk<ELL, LOGTC, T0, B, false>(lds, tw, H, p, pinv);
-    tile_fwd<ELL, LOGTC, T0 + B>(lds, tw, H, p, pinv);
-  }
-}
-
-template <int ELL, int LOGTC, int T0>
-__device__ __forceinline__ void tile_inv(uint32_t *lds, const uint32_t *tw, uint32_t H,
-                                         uint32_t p, uint32_t pinv) {
-  if constexpr (T0 < ELL) {
-    constexpr int B = ChunkPlan<ELL, LOGTC>::pick(T0);
-    tile_inv<ELL, LOGTC, T0 + B>(lds, tw, H, p, pinv);
-    tile_chunk<ELL, LOGTC, T0, B, true>(lds, tw, H, p, pinv);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// job descriptors: every kernel serves one multiply (JobRef.one) or a whole
-// size class of a batch (JobRef.arr[blockIdx.z]); all jobs of one launch
-// share the transform length N = 2^L.
-// ---------------------------------------------------------------------------
-struct Job {
-  const uint64_t *a, *b;          // operand limbs (device)
-  uint64_t *r;                    // product limbs (device), an+bn of them
-  uint32_t *aspec, *bspec;        // K x N residues each
-  uint8_t *lmaps;                 // one carry map per limb, padded to blocks
-  uint32_t *bmaps, *bcin;         // per carry block
-  uint32_t an, bn, zlen, Z, nb;   // limbs, coefficients, product limbs, carry blocks
-};
-// global-address-space views of descriptor pointers: loads through a pointer
-// read from memory are otherwise generic (flat) and serialised against LDS
-typedef __attribute__((address_space(1))) uint32_t g_u32;
-typedef __attribute__((address_space(1))) const uint32_t gc_u32;
-typedef __attribute__((address_space(1))) uint64_t g_u64;
-typedef __attribute__((address_space(1))) const uint64_t gc_u64;
-typedef __attribute__((address_space(1))) uint8_t g_u8;
-
-struct JobRef {
-  const Job *arr;
-  Job one;
-};
-// The descriptor is uniform across the block; readfirstlane moves every field
-// into scalar registers so tile addresses stay scalar-base + lane-offset.
-template <typename T> __device__ __forceinline__ T *uni_ptr(T *p) {
-  const uint64_t v = (uint64_t)p;
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-  return (T *)(((uint64_t)hi << 32) | lo);
-}
-__device__ __forceinline__ uint32_t uni_u32(uint32_t x) {
-  return __builtin_amdgcn_readfirstlane(x);
-}
-__device__ __forceinline__ Job get_job(const JobRef &jr) {
-  Job j;
-  if (jr.arr) {
-    j = jr.arr[blockIdx.z];
-  } else {
-    j = jr.one;
-  }
-  j.a = uni_ptr(j.a); j.b = uni_ptr(j.b); j.r = uni_ptr(j.r);
-  j.aspec = uni_ptr(j.aspec); j.bspec = uni_ptr(j.bspec); j.lmaps = uni_ptr(j.lmaps);
-  j.bmaps = uni_ptr(j.bmaps); j.bcin = uni_ptr(j.bcin);
-  j.an = uni_u32(j.an); j.bn = uni_u32(j.bn); j.zlen = uni_u32(j.zlen); j.Z = uni_u32(j.Z);
-  j.nb = uni_u32(j.nb);
-  return j;
-}
-
-// ---------------------------------------------------------------------------
-// pass kernels
-// ---------------------------------------------------------------------------
-// First forward pass (global levels [0, ELL)): limbs -> residues of all K
-// primes -> ELL levels -> spec[q][.]. Tiles = column tiles (H = 0).
-// which = 0: operand a into aspec; 1: operand b into bspec.
-template <int ELL, int LOGTC>
-__global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd(PrimeArgs pa, JobRef jr, int which,
-                                                             int L) {
-  constexpr int TC = 1 << LOGTC;
-  constexpr int NE = 1 << (ELL + LOGTC);
-  constexpr int E = NE / NT;
-  __shared__ uint32_t lds[Pad<LOGTC>::words(NE)];
-  const Job jb = get_job(jr);
-  gc_u64 *limbs = (gc_u64 *)(which ? jb.b : jb.a);
-  g_u32 *spec = (g_u32 *)(which ? jb.bspec : jb.aspec);
-  const uint32_t nlimbs = which ? jb.bn : jb.an;
-  const uint32_t N = 1u << L;             // N <= 2^25: 32-bit offsets throughout
-  const uint32_t S = 1u << (L - ELL);
-  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-  const uint32_t col0 = tile << LOGTC;
-  const uint32_t nl = nlimbs < N ? nlimbs : N;
-  uint32_t *lt = lds + lpadt<LOGTC>(threadIdx.x);
-  // element e = tid + i*NT sits at col0 + toff + uoff(i): a per-thread part
-  // and a uniform part (scalar base + 32-bit lane offset addressing)
-  const uint32_t toff = (threadIdx.x >> LOGTC) * S + (threadIdx.x & (TC - 1));
-  // SBA_FIRST_SPLIT: one prime per block (grid y), else all primes per block
-  const int q0 = SBA_FIRST_SPLIT ? (int)blockIdx.y : 0;
-  const int q1 = SBA_FIRST_SPLIT ? q0 + 1 : SBA_K;
-#pragma unroll 1
-  for (int q = q0; q < q1; ++q) {
-    const uint32_t p = pa.p[q], pinv = pa.pinv[q], r2 = pa.r2[q];
-    __syncthreads();
-    // the tile's limbs are re-read per prime (cache-resident after the first)
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
-      gc_u64 *lp = limbs + uoff;
-      lt[i * Pad<LOGTC>::words(NT)] = limb_residue(uoff + toff < nl ? lp[toff] : 0, p, pinv, r2);
-    }
-    tile_fwd<ELL, LOGTC, 0>(lds, pa.wn[q], 0, p, pinv);
-    __syncthreads();
-    g_u32 *out = spec + q * N;
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
-      (out + uoff)[toff] = lt[i * Pad<LOGTC>::words(NT)];
-    }
-  }
-}
-
-// In-place pass over global levels [s0, s0 + ELL) of one prime
-// (blockIdx.y), forward or inverse, on aspec (which = 0) or bspec (1).
-template <int ELL, int LOGTC, bool INV>
-__global__ __launch_bounds__(NT, SBA_WAVES) void k_pass(PrimeArgs pa, JobRef jr, int which, int L,
-                                                        int s0) {
-  constexpr int TC = 1 << LOGTC;
-  constexpr int NE = 1 << (ELL + LOGTC);
-  constexpr int E = NE / NT;
-  __shared__ uint32_t lds[Pad<LOGTC>::words(NE)];
-  const Job jb = get_job(jr);
-  g_u32 *spec = (g_u32 *)(which ? jb.bspec : jb.aspec);
-  const int q = blockIdx.y;
-  const uint32_t N = 1u << L;
-  const uint32_t S = 1u << (L - s0 - ELL);
-  const uint32_t ctiles = S >> LOGTC;
-  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-  const uint32_t H = tile / ctiles;
-  const uint32_t col0 = (tile % ctiles) << LOGTC;
-  const uint32_t base = (H << (L - s0)) + col0;
-  g_u32 *buf = spec + q * N + base;
-  const uint32_t p = pa.p[q], pinv = pa.pinv[q];
-  uint32_t *lt = lds + lpadt<LOGTC>(threadIdx.x);
-  const uint32_t toff = (threadIdx.x >> LOGTC) * S + (threadIdx.x & (TC - 1));
-#pragma unroll
-  for (int i = 0; i < E; ++i) {
-    const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
-    lt[i * Pad<LOGTC>::words(NT)] = (buf + uoff)[toff];
-  }
-  if (!INV) tile_fwd<ELL, LOGTC, 0>(lds, pa.wn[q], H, p, pinv);
-  else tile_inv<ELL, LOGTC, 0>(lds, pa.wni[q], H, p, pinv);
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < E; ++i) {
-    const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
-    (buf + uoff)[toff] = lt[i * Pad<LOGTC>::words(NT)];
-  }
-}
-
-// Last forward levels of b + pointwise product with a's spectrum (or b's own
-// square) + first inverse levels, on contiguous tiles (S = 1, TC = 1).
-// LIMB_IN: the pass is the whole transform (s0 = 0) and reads b's limbs.
-template <int ELL, bool LIMB_IN, bool SQR>
-__global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr, int L) {
-  constexpr int NE = 1 << ELL;
-  constexpr int E = NE / NT;
-  __shared__ uint32_t lds[Pad<0>::words(NE)];
-  const Job jb = get_job(jr);
-  const int q = blockIdx.y;
-  const uint32_t N = 1u << L;
-  const uint32_t H = blockIdx.x;
-  const uint32_t base = H << ELL;
-  const uint32_t nb = jb.bn < N ? jb.bn : N;
-  const uint32_t p = pa.p[q], pinv = pa.pinv[q];
-  g_u32 *buf = (g_u32 *)jb.bspec + q * N + base;
-  gc_u32 *as = (gc_u32 *)jb.aspec + q * N + base;
-  gc_u64 *bl = (gc_u64 *)jb.b + base;
-  uint32_t *lt = lds + lpadt<0>(threadIdx.x);
-  const uint32_t tid = threadIdx.x;
-#pragma unroll
-  for (int i = 0; i < E; ++i) {
-    uint32_t v;
-    if (LIMB_IN) {
-      v = limb_residue(base + i * NT + tid < nb ? (bl + i * NT)[tid] : 0, p, pinv, pa.r2[q]);
-    } else {
-      v = (buf + i * NT)[tid];
-    }
-    lt[i * Pad<0>::words(NT)] = v;
-  }
-  tile_fwd<ELL, 0, 0>(lds, pa.wn[q], H, p, pinv);
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < E; ++i) {
-    const uint32_t v = lt[i * Pad<0>::words(NT)];
-    const uint32_t o = SQR ? red1(v, p) : red1((as + i * NT)[tid], p);
-    lt[i * Pad<0>::words(NT)] = mont_mul(v, o, p, pinv);
-  }
-  tile_inv<ELL, 0, 0>(lds, pa.wni[q], H, p, pinv);
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < E; ++i) (buf + i * NT)[tid] = lt[i * Pad<0>::words(NT)];
-}
 
 // ---------------------------------------------------------------------------
 // Garner: the five residues of coefficient i -> 3 u64 words
@@ -750,12 +384,6 @@ static int lg_for(uint64_t zlen) {
   return L;
 }
 
-extern "C" size_t sba_gpu_workspace_bytes(int64_t an, int64_t bn) {
-  if (an <= 0 || bn <= 0) return 0;
-  const int L = lg_for((uint64_t)an + (uint64_t)bn - 1);
-  if (L > SBA_MAX_LG) return 0;
-  return ws_need((size_t)1 << L);
-}
 
 // CRT cap: largest min(an, bn) with min * (2^64-1)^2 < prod(p)
 extern "C" uint64_t sba_gpu_min_operand_cap(void) {
@@ -790,62 +418,73 @@ extern "C" int sba_gpu_max_lg(void) { return SBA_MAX_LG; }
 // ---------------------------------------------------------------------------
 // launch helpers (template dispatch on the pass shape)
 // ---------------------------------------------------------------------------
-typedef void (*first_fn)(PrimeArgs, JobRef, int, int);
-typedef void (*pass_fn)(PrimeArgs, JobRef, int, int, int);
-typedef void (*fused_fn)(PrimeArgs, JobRef, int);
 
-// first pass of the two-pass plan: ELL = L - 13 levels, TC = 2^(13 - ELL)
-static first_fn first_two_pass(int ell) {
-  switch (ell) {
-    case 1: return k_first_fwd<1, 12>;
-    case 2: return k_first_fwd<2, 11>;
-    case 3: return k_first_fwd<3, 10>;
-    case 4: return k_first_fwd<4, 9>;
-    case 5: return k_first_fwd<5, 8>;
-    case 6: return k_first_fwd<6, 7>;
-    case 7: return k_first_fwd<7, 6>;
-    case 8: return k_first_fwd<8, 5>;
-    case 9: return k_first_fwd<9, 4>;
-    case 10: return k_first_fwd<10, 3>;
-    case 11: return k_first_fwd<11, 2>;
-    case 12: return k_first_fwd<12, 1>;
-  }
-  return nullptr;
+// ---------------------------------------------------------------------------
+// plans: how a transform of length N = 2^L is cut into passes
+//   last   levels of the last pass, on contiguous tiles of 2^last residues.
+//          last == L: the whole transform is one tile (L <= 13). Otherwise a
+//          first pass over column tiles of 2^13 residues (2^(L-last) rows x
+//          2^(13-L+last) columns) takes the other L - last levels.
+//   bm     widest register chunk inside a tile: radix-2^bm stages (3..5)
+//   split  first forward pass with one prime per block (1), or all K primes
+//          per block reusing the tile's limb loads (0)
+// ---------------------------------------------------------------------------
+static bool plan_ok(int L, const Plan &p) {
+  if (p.bm < 3 || p.bm > 5 || (p.split != 0 && p.split != 1)) return false;
+  if (p.last == L) return L >= 10 && L <= 13;
+  if (p.last < 11 || p.last > 13 || L < 13) return false;
+  const int ell = L - p.last;
+  return ell >= 1 && ell <= 12;
 }
-static pass_fn inv_first_two_pass(int ell) {
-  switch (ell) {
-    case 1: return k_pass<1, 12, true>;
-    case 2: return k_pass<2, 11, true>;
-    case 3: return k_pass<3, 10, true>;
-    case 4: return k_pass<4, 9, true>;
-    case 5: return k_pass<5, 8, true>;
-    case 6: return k_pass<6, 7, true>;
-    case 7: return k_pass<7, 6, true>;
-    case 8: return k_pass<8, 5, true>;
-    case 9: return k_pass<9, 4, true>;
-    case 10: return k_pass<10, 3, true>;
-    case 11: return k_pass<11, 2, true>;
-    case 12: return k_pass<12, 1, true>;
-  }
-  return nullptr;
+// the hand-written rule: widest last pass, radix 32, one prime per block
+static Plan heuristic_plan(int L) {
+  Plan p;
+  p.last = L <= 13 ? L : 13;
+  p.bm = SBA_BMAX;
+  p.split = SBA_FIRST_SPLIT;
+  return p;
 }
-static first_fn first_single(int L) {
-  switch (L) {
-    case 10: return k_first_fwd<10, 0>;
-    case 11: return k_first_fwd<11, 0>;
-    case 12: return k_first_fwd<12, 0>;
-    case 13: return k_first_fwd<13, 0>;
+
+static Launchers launchers(int L, int sqr, const Plan &p) {
+  switch (p.bm) {
+    case 3: return sba_launchers_bm3(L, sqr, p.last);
+    case 4: return sba_launchers_bm4(L, sqr, p.last);
+    default: return sba_launchers_bm5(L, sqr, p.last);
   }
-  return nullptr;
 }
-static fused_fn fused_single(int L, int sqr) {
-  switch (L) {
-    case 10: return sqr ? k_fused<10, true, true> : k_fused<10, true, false>;
-    case 11: return sqr ? k_fused<11, true, true> : k_fused<11, true, false>;
-    case 12: return sqr ? k_fused<12, true, true> : k_fused<12, true, false>;
-    case 13: return sqr ? k_fused<13, true, true> : k_fused<13, true, false>;
+
+// plan table: (sqr, L, bucket of the class's job count) -> plan. Starts as
+// the shipped table (plan_table.h: the fastest plan per key measured on
+// MI355X; the rule where a key was not measured) until the autotuner or a
+// caller stores another choice.
+static Plan g_plan[2][SBA_MAX_LG + 1][SBA_PLAN_BUCKETS];
+static int g_plan_init = 0;
+static Plan shipped_plan(int sq, int L, int k) {
+  if (L >= 0 && L < SBA_PT_NL && k < SBA_PLAN_BUCKETS) {
+    const signed char *e = kPlanTable[sq][L][k];
+    const Plan p = {e[0], e[1], e[2]};
+    if (e[0] > 0 && plan_ok(L, p)) return p;
   }
-  return nullptr;
+  return heuristic_plan(L);
+}
+static void plan_defaults(void) {
+  for (int sq = 0; sq < 2; ++sq)
+    for (int L = 0; L <= SBA_MAX_LG; ++L)
+      for (int k = 0; k < SBA_PLAN_BUCKETS; ++k) g_plan[sq][L][k] = shipped_plan(sq, L, k);
+  g_plan_init = 1;
+}
+static int plan_bucket(uint32_t njobs) {
+  int b = 0;
+  while (njobs >= 4 && b < SBA_PLAN_BUCKETS - 1) {
+    njobs >>= 2;
+    ++b;
+  }
+  return b;
+}
+static Plan plan_for(int L, int sqr, uint32_t njobs) {
+  if (!g_plan_init) plan_defaults();
+  if (L < 0 || L > SBA_MAX_LG) return heuristic_plan(L);
+  return g_plan[sqr ? 1 : 0][L][plan_bucket(njobs)];
 }
 
 // ---------------------------------------------------------------------------
@@ -933,75 +572,79 @@ extern "C" int sba_gpu_profile_get(int idx, const char **name, double *ms, doubl
   return 1;
 }
 
-#define LAST_ELL 13
+static int enqueue_garner(DevCtx &c, uint32_t Nt, uint32_t stride, const JobRef &jr,
+                          uint32_t njobs, uint32_t nbmax, double zsum, hipStream_t st);
+static const AccArgs kNoAcc = {nullptr, 0, 0, 0, 0};
 
 // carry blocks of a product of Z limbs
 static uint32_t carry_blocks(uint64_t Z) { return (uint32_t)((Z + CARRY_LPB - 1) / CARRY_LPB); }
 
 // Enqueue the whole multiply for `njobs` jobs sharing N = 2^L (jr.arr ==
-// nullptr: the single job jr.one). nbmax = the largest carry-block count;
-// zsum / asum / bsum feed the profiler's algorithmic byte counts.
-static int enqueue_class(DevCtx &c, int L, int sqr, const JobRef &jr, uint32_t njobs,
-                         uint32_t nbmax, double asum, double bsum, double zsum, hipStream_t st) {
+// nullptr: the single job jr.one) under plan pl. nbmax = the largest
+// carry-block count; zsum / asum / bsum feed the profiler's byte counts.
+static int enqueue_class(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef &jr,
+                         uint32_t njobs, uint32_t nbmax, double asum, double bsum, double zsum,
+                         hipStream_t st) {
+  if (!plan_ok(L, pl)) return SBA_BAND;
+  const Launchers k = launchers(L, sqr, pl);
   const uint32_t N = 1u << L;
   const PrimeArgs &pa = c.pa;
   const double KN4 = 4.0 * SBA_K * (double)N * njobs;
-  if (L <= LAST_ELL) {
+  const uint32_t gy1 = pl.split ? SBA_K : 1;
+  if (pl.last == L) {
     if (!sqr) {
       PROF("first_fwd", 8.0 * asum + KN4);
-      hipLaunchKernelGGL(first_single(L), dim3(1, SBA_FIRST_SPLIT ? SBA_K : 1, njobs), dim3(NT),
-                         0, st, pa, jr, 0, L);
+      hipLaunchKernelGGL(k.first, dim3(1, gy1, njobs), dim3(NT), 0, st, pa, jr, 0, L);
     }
     {
       PROF("fused_last", 8.0 * bsum + (sqr ? 1 : 2) * KN4);
-      hipLaunchKernelGGL(fused_single(L, sqr), dim3(1, SBA_K, njobs), dim3(NT), 0, st, pa, jr,
-                         L);
+      hipLaunchKernelGGL(k.fused, dim3(1, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L);
     }
   } else {
-    const int ell1 = L - LAST_ELL;
+    const int ell1 = L - pl.last;
     const uint32_t ntile1 = N >> 13;
-    const uint32_t ntile2 = N >> LAST_ELL;
-    first_fn f1 = first_two_pass(ell1);
+    const uint32_t ntile2 = N >> pl.last;
     if (!sqr) {
       PROF("first_fwd", 8.0 * asum + KN4);
-      hipLaunchKernelGGL(f1, dim3(ntile1, SBA_FIRST_SPLIT ? SBA_K : 1, njobs), dim3(NT), 0, st,
-                         pa, jr, 0, L);
+      hipLaunchKernelGGL(k.first, dim3(ntile1, gy1, njobs), dim3(NT), 0, st, pa, jr, 0, L);
     }
     {
       PROF("first_fwd", 8.0 * bsum + KN4);
-      hipLaunchKernelGGL(f1, dim3(ntile1, SBA_FIRST_SPLIT ? SBA_K : 1, njobs), dim3(NT), 0, st,
-                         pa, jr, 1, L);
+      hipLaunchKernelGGL(k.first, dim3(ntile1, gy1, njobs), dim3(NT), 0, st, pa, jr, 1, L);
     }
     if (!sqr) {
       PROF("fwd_last", 2 * KN4);
-      hipLaunchKernelGGL((k_pass<LAST_ELL, 0, false>), dim3(ntile2, SBA_K, njobs), dim3(NT), 0,
-                         st, pa, jr, 0, L, L - LAST_ELL);
+      hipLaunchKernelGGL(k.fwd_last, dim3(ntile2, SBA_K, njobs), dim3(NT), 0, st, pa, jr, 0, L,
+                         ell1, kNoAcc);
     }
     {
       PROF("fused_last", (sqr ? 2 : 3) * KN4);
-      if (sqr)
-        hipLaunchKernelGGL((k_fused<LAST_ELL, false, true>), dim3(ntile2, SBA_K, njobs),
-                           dim3(NT), 0, st, pa, jr, L);
-      else
-        hipLaunchKernelGGL((k_fused<LAST_ELL, false, false>), dim3(ntile2, SBA_K, njobs),
-                           dim3(NT), 0, st, pa, jr, L);
+      hipLaunchKernelGGL(k.fused, dim3(ntile2, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L);
     }
     {
       PROF("inv_first", 2 * KN4);
-      hipLaunchKernelGGL(inv_first_two_pass(ell1), dim3(ntile1, SBA_K, njobs), dim3(NT), 0, st,
-                         pa, jr, 1, L, 0);
+      hipLaunchKernelGGL(k.inv_first, dim3(ntile1, SBA_K, njobs), dim3(NT), 0, st, pa, jr, 1, L,
+                         0, kNoAcc);
     }
   }
+  return enqueue_garner(c, N, N, jr, njobs, nbmax, zsum, st);
+}
+
+// Garner + carry scan for njobs jobs whose product residues sit in bspec
+// with prime stride `stride`, left scaled by the transform length Nt by the
+// unnormalised inverse
+static int enqueue_garner(DevCtx &c, uint32_t Nt, uint32_t stride, const JobRef &jr,
+                          uint32_t njobs, uint32_t nbmax, double zsum, hipStream_t st) {
   GarnerArgs ga = c.ga_base;
   for (int q = 0; q < SBA_K; ++q) {
     const uint64_t p = kPrimes[q];
-    const uint64_t ninv = h_invm(N % p, p);
-    // scale = N^-1 * R^2 mod p
+    const uint64_t ninv = h_invm(Nt % p, p);
+    // scale = Nt^-1 * R^2 mod p
     ga.scale[q] = (uint32_t)((unsigned __int128)ninv * (((unsigned __int128)1 << 64) % p) % p);
   }
   {
     PROF("garner_maps", 4.0 * SBA_K * zsum + 9.0 * zsum);
-    hipLaunchKernelGGL(k_garner_maps, dim3(nbmax, 1, njobs), dim3(NT), 0, st, ga, jr, N);
+    hipLaunchKernelGGL(k_garner_maps, dim3(nbmax, 1, njobs), dim3(NT), 0, st, ga, jr, stride);
   }
   {
     PROF("carry_blocks", 8.0 * zsum / CARRY_LPB);
@@ -1035,11 +678,166 @@ static int get_ctx(DevCtx **out) {
   return 0;
 }
 
-static int in_band(int64_t an, int64_t bn) {
+// ---------------------------------------------------------------------------
+// chunked products: the shorter operand's spectrum (length Nc = 2^Lc) is
+// computed once; the longer one is cut into chunks of C = Nc - bn + 1
+// limbs, each transformed, multiplied by that spectrum and inverted straight
+// into an accumulator of residues (overlapping tails added mod p); one
+// Garner + carry pass over the accumulator finishes. Used when it moves
+// clearly fewer transform elements than one long transform, and for
+// products longer than 2^25 coefficients.
+// ---------------------------------------------------------------------------
+#define SBA_MAX_Z ((uint64_t)1 << 29)   // product limbs: residue offsets stay 32-bit
+#define CHUNK_MIN 8192                  // smallest chunk (limbs)
+#define CHUNK_LAUNCH_EQ 150000.0        // one launch costs about this many element-transforms
+
+struct ChunkChoice {
+  int Lc;
+  uint64_t C, n;                        // chunk limbs, chunks (n == 0: no chunking)
+};
+
+// an >= bn
+static ChunkChoice chunk_choice(uint64_t an, uint64_t bn, int sqr) {
+  ChunkChoice best = {0, 0, 0};
+  const int L = lg_for(an + bn - 1);
+  if (L <= SBA_MAX_LG && an < 4 * bn) return best;       // near-balanced: one transform
+  // direct: forward a, forward b (not for a square), inverse
+  double cbest = L <= SBA_MAX_LG ? 0.9 * (sqr ? 2.0 : 3.0) * (double)((uint64_t)1 << L) : 1e300;
+  for (int Lc = 14; Lc <= SBA_MAX_LG && Lc < L; ++Lc) {
+    const uint64_t Nc = (uint64_t)1 << Lc;
+    if (bn + CHUNK_MIN > Nc) continue;
+    const uint64_t C = Nc - bn + 1;
+    const uint64_t n = (an + C - 1) / C;
+    if (n < 2) continue;
+    const double cost = (double)Nc * (1.0 + 2.0 * (double)n) + CHUNK_LAUNCH_EQ * 3.0 * (double)n;
+    if (cost < cbest) {
+      cbest = cost;
+      best.Lc = Lc;
+      best.C = C;
+      best.n = n;
+    }
+  }
+  return best;
+}
+
+// short spectrum, chunk spectrum, accumulator, carry maps of Z limbs
+static size_t ws_chunked(int Lc, uint64_t Z) {
+  const size_t Nc = (size_t)1 << Lc, Zs = (Z + 255) & ~(size_t)255, nb = Z / CARRY_LPB + 2;
+  return 2 * SBA_K * Nc * 4 + SBA_K * Zs * 4 + nb * CARRY_LPB + 2 * nb * 4 + 4096;
+}
+
+// an >= bn; ws holds ws_chunked(cp.Lc, an + bn) bytes
+static int enqueue_chunked(DevCtx &c, const ChunkChoice &cp, uint64_t *rp, const uint64_t *ap,
+                           uint64_t an, const uint64_t *bp, uint64_t bn, void *ws,
+                           hipStream_t st) {
+  const int Lc = cp.Lc;
+  const uint32_t Nc = 1u << Lc;
+  const Plan pl = plan_for(Lc, 0, 1);
+  if (!plan_ok(Lc, pl) || pl.last == Lc) return SBA_BAND;
+  const Launchers k = launchers(Lc, 0, pl);
+  const PrimeArgs &pa = c.pa;
+  const uint64_t Z = an + bn;
+  const uint32_t Zs = (uint32_t)((Z + 255) & ~(uint64_t)255);
+  const int ell1 = Lc - pl.last;
+  const uint32_t ntile1 = Nc >> 13, ntile2 = Nc >> pl.last;
+  const uint32_t gy1 = pl.split ? SBA_K : 1;
+  const double KN4 = 4.0 * SBA_K * Nc;
+  JobRef jr;
+  jr.arr = nullptr;
+  Job &j = jr.one;
+  memset(&j, 0, sizeof j);
+  j.aspec = (uint32_t *)ws;
+  j.bspec = j.aspec + (size_t)SBA_K * Nc;
+  uint32_t *acc = j.bspec + (size_t)SBA_K * Nc;
+  // the short operand's whole spectrum (the "a" side of the passes)
+  j.a = bp;
+  j.an = (uint32_t)bn;
+  {
+    PROF("first_fwd", 8.0 * bn + KN4);
+    hipLaunchKernelGGL(k.first, dim3(ntile1, gy1, 1), dim3(NT), 0, st, pa, jr, 0, Lc);
+  }
+  {
+    PROF("fwd_last", 2 * KN4);
+    hipLaunchKernelGGL(k.fwd_last, dim3(ntile2, SBA_K, 1), dim3(NT), 0, st, pa, jr, 0, Lc, ell1,
+                       kNoAcc);
+  }
+  for (uint64_t ci = 0; ci < cp.n; ++ci) {
+    const uint64_t off = ci * cp.C;
+    const uint64_t len = an - off < cp.C ? an - off : cp.C;
+    j.b = ap + off;
+    j.bn = (uint32_t)len;
+    {
+      PROF("first_fwd", 8.0 * len + KN4);
+      hipLaunchKernelGGL(k.first, dim3(ntile1, gy1, 1), dim3(NT), 0, st, pa, jr, 1, Lc);
+    }
+    {
+      PROF("fused_last", 3 * KN4);
+      hipLaunchKernelGGL(k.fused, dim3(ntile2, SBA_K, 1), dim3(NT), 0, st, pa, jr, Lc);
+    }
+    {
+      AccArgs aa;
+      aa.dst = acc;
+      aa.stride = Zs;
+      aa.off = (uint32_t)off;
+      aa.ovl = ci ? (uint32_t)(bn - 1) : 0u;     // the previous chunk's tail
+      aa.lim = (uint32_t)(len + bn - 1);          // this chunk's product
+      PROF("inv_first", KN4 + 8.0 * SBA_K * (len + bn));
+      hipLaunchKernelGGL(k.inv_first, dim3(ntile1, SBA_K, 1), dim3(NT), 0, st, pa, jr, 1, Lc, 0,
+                         aa);
+    }
+  }
+  JobRef jg;
+  jg.arr = nullptr;
+  Job &g = jg.one;
+  memset(&g, 0, sizeof g);
+  g.bspec = acc;
+  g.r = rp;
+  g.zlen = (uint32_t)(Z - 1);
+  g.Z = (uint32_t)Z;
+  g.nb = carry_blocks(Z);
+  const size_t nbcap = Z / CARRY_LPB + 2;
+  g.lmaps = (uint8_t *)(acc + (size_t)SBA_K * Zs);
+  g.bmaps = (uint32_t *)(g.lmaps + nbcap * CARRY_LPB);
+  g.bcin = g.bmaps + nbcap;
+  return enqueue_garner(c, Nc, Zs, jg, 1, g.nb, (double)Z, st);
+}
+
+// one transform of length <= 2^25 (batches use only this)
+static int in_band_direct(int64_t an, int64_t bn) {
   if (an <= 0 || bn <= 0) return 0;
   const uint64_t mn = (uint64_t)(an < bn ? an : bn);
   if (mn > sba_gpu_min_operand_cap()) return 0;
   return lg_for((uint64_t)an + (uint64_t)bn - 1) <= SBA_MAX_LG;
+}
+
+// single products: direct, or chunked along the longer operand
+static int in_band(int64_t an, int64_t bn) {
+  if (in_band_direct(an, bn)) return 1;
+  if (an <= 0 || bn <= 0) return 0;
+  const uint64_t mx = (uint64_t)(an > bn ? an : bn), mn = (uint64_t)(an < bn ? an : bn);
+  if (mn > sba_gpu_min_operand_cap() || mx + mn > SBA_MAX_Z) return 0;
+  return chunk_choice(mx, mn, 0).n > 0;
+}
+
+extern "C" int sba_gpu_in_band(int64_t an, int64_t bn) { return in_band(an, bn); }
+
+extern "C" int sba_gpu_chunking(int64_t an, int64_t bn, int sqr, int *Lc, uint64_t *chunk,
+                                uint64_t *nchunks) {
+  if (!in_band(an, bn)) return SBA_BAND;
+  const uint64_t mx = (uint64_t)(an > bn ? an : bn), mn = (uint64_t)(an < bn ? an : bn);
+  const ChunkChoice cp = chunk_choice(mx, mn, sqr);
+  *Lc = cp.Lc;
+  *chunk = cp.C;
+  *nchunks = cp.n;
+  return SBA_OK;
+}
+
+extern "C" size_t sba_gpu_workspace_bytes(int64_t an, int64_t bn) {
+  if (!in_band(an, bn)) return 0;
+  const uint64_t mx = (uint64_t)(an > bn ? an : bn), mn = (uint64_t)(an < bn ? an : bn);
+  const ChunkChoice cp = chunk_choice(mx, mn, 0);
+  if (cp.n) return ws_chunked(cp.Lc, mx + mn);
+  return ws_need((size_t)1 << lg_for(mx + mn - 1));
 }
 
 static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64_t *bp,
@@ -1049,7 +847,8 @@ static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64
     const uint64_t *t = ap; ap = bp; bp = t;
     int64_t tn = an; an = bn; bn = tn;
   }
-  const int L = lg_for((uint64_t)an + (uint64_t)bn - 1);
+  const ChunkChoice cp = chunk_choice((uint64_t)an, (uint64_t)bn, sqr);
+  const int L = cp.n ? cp.Lc : lg_for((uint64_t)an + (uint64_t)bn - 1);
   hipStream_t st = (hipStream_t)stream;
   std::lock_guard<std::mutex> lk(g_mu);
   DevCtx *c;
@@ -1057,7 +856,8 @@ static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64
   if (rc) return rc;
   rc = ctx_twiddles(*c, L, st);
   if (rc) return rc;
-  const size_t need = ws_need((size_t)1 << L);
+  const size_t need = cp.n ? ws_chunked(cp.Lc, (uint64_t)an + (uint64_t)bn)
+                           : ws_need((size_t)1 << L);
   if (!ws) {
     if (c->ws_bytes < need) {
       if (c->ws) HIPCHK(hipFree(c->ws));
@@ -1070,6 +870,8 @@ static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64
   } else if (ws_bytes < need) {
     return SBA_ERR_WORKSPACE;
   }
+  if (cp.n)
+    return enqueue_chunked(*c, cp, rp, ap, (uint64_t)an, sqr ? ap : bp, (uint64_t)bn, ws, st);
   JobRef jr;
   jr.arr = nullptr;
   Job &j = jr.one;
@@ -1082,7 +884,8 @@ static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64
   j.Z = (uint32_t)(an + bn);
   j.nb = carry_blocks(j.Z);
   job_layout(j, ws, 1u << L);
-  return enqueue_class(*c, L, sqr, jr, 1, j.nb, (double)an, (double)bn, (double)j.Z, st);
+  return enqueue_class(*c, L, sqr, plan_for(L, sqr, 1), jr, 1, j.nb, (double)an, (double)bn,
+                       (double)j.Z, st);
 }
 
 extern "C" int sba_gpu_mul_dev(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64_t *bp,
@@ -1110,103 +913,37 @@ extern "C" int sba_gpu_sqr_dev_ws(uint64_t *rp, const uint64_t *ap, int64_t an, 
 // ---------------------------------------------------------------------------
 // batches: many independent multiplies of uneven sizes. Jobs are grouped in
 // size classes (same N = 2^L, same mul/sqr kind); one launch sequence per
-// class covers all its jobs (grid z = job). The whole batch is captured once
-// into a hipGraph and replayed by sba_batch_run.
+// class covers all its jobs (grid z = job), under the class's plan.
 // ---------------------------------------------------------------------------
-struct BatchClass {
-  int L, sqr;
-  uint32_t njobs, nbmax;
-  Job *djobs;                      // device array
-  double asum, bsum, zsum;
+struct ClassDesc {
+  int L, sqr, start;
+  uint32_t n, nbmax;
+  double as, bs, zs;
+  Plan plan;
 };
 
-struct sba_batch {
-  int dev;
-  void *ws;                        // all jobs' workspaces
-  Job *djobs_all;
-  int nclass;
-  BatchClass cls[2 * (SBA_MAX_LG + 1)];
-  hipStream_t cap_stream;          // private stream used for capture
-  hipGraph_t graph;
-  hipGraphExec_t exec;
-  int use_graph;
-};
-
-static int batch_enqueue(sba_batch *b, hipStream_t st) {
-  DevCtx *c;
-  int rc = get_ctx(&c);
-  if (rc) return rc;
-  for (int k = 0; k < b->nclass; ++k) {
-    BatchClass &bc = b->cls[k];
-    JobRef jr;
-    jr.arr = bc.djobs;
-    memset(&jr.one, 0, sizeof jr.one);
-    rc = enqueue_class(*c, bc.L, bc.sqr, jr, bc.njobs, bc.nbmax, bc.asum, bc.bsum, bc.zsum, st);
-    if (rc != SBA_OK) return rc;
-  }
-  return SBA_OK;
+static size_t job_ws(const sba_job &j) {
+  const uint64_t an = (uint64_t)j.an, bn = j.bp ? (uint64_t)j.bn : an;
+  return (ws_need((size_t)1 << lg_for(an + bn - 1)) + 255) & ~(size_t)255;
 }
+static size_t table_bytes(int njobs) { return ((size_t)njobs * sizeof(Job) + 255) & ~(size_t)255; }
 
-extern "C" void sba_batch_destroy(sba_batch *b) {
-  if (!b) return;
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
-  if (b->ws) (void)hipFree(b->ws);
-  if (b->djobs_all) (void)hipFree(b->djobs_all);
-  free(b);
-}
-
-extern "C" sba_batch *sba_batch_create(const sba_job *jobs, int njobs, int use_graph,
-                                       int *status) {
-  int st_dummy;
-  if (!status) status = &st_dummy;
-  *status = SBA_OK;
-  if (njobs <= 0) { *status = SBA_BAND; return nullptr; }
-  for (int i = 0; i < njobs; ++i)
-    if (!in_band(jobs[i].an, jobs[i].bp ? jobs[i].bn : jobs[i].an)) {
-      *status = SBA_BAND;
-      return nullptr;
-    }
-  std::lock_guard<std::mutex> lk(g_mu);
-  DevCtx *c;
-  if ((*status = get_ctx(&c)) != 0) return nullptr;
-  sba_batch *b = (sba_batch *)calloc(1, sizeof(sba_batch));
-  if (!b) { *status = SBA_ERR_HIP; return nullptr; }
-  b->use_graph = use_graph;
-  (void)hipGetDevice(&b->dev);
-  // classify: key (L, sqr); jobs ordered by class
-  int Lmax = 0;
-  size_t total = 0;
+// Orders the jobs by class into hj[], carving each job's buffers from ws in
+// that order (job_ws bytes each), and fills cls[] (at most 2*(SBA_MAX_LG+1)
+// entries). Every job must be in band. Returns the number of classes.
+static int classify_jobs(const sba_job *jobs, int njobs, char *ws, Job *hj, ClassDesc *cls) {
+  int ncls = 0, pos = 0;
+  size_t off = 0;
   int count[2][SBA_MAX_LG + 1] = {};
   for (int i = 0; i < njobs; ++i) {
     const int sq = jobs[i].bp == nullptr;
-    const int L = lg_for((uint64_t)jobs[i].an + (uint64_t)(sq ? jobs[i].an : jobs[i].bn) - 1);
-    count[sq][L]++;
-    if (L > Lmax) Lmax = L;
-    total += (ws_need((size_t)1 << L) + 255) & ~(size_t)255;
+    const uint64_t an = (uint64_t)jobs[i].an, bn = sq ? an : (uint64_t)jobs[i].bn;
+    count[sq][lg_for(an + bn - 1)]++;
   }
-  int rc = ctx_twiddles(*c, Lmax, 0);
-  if (rc) { free(b); *status = rc; return nullptr; }
-  if (hipMalloc(&b->ws, total) != hipSuccess ||
-      hipMalloc(&b->djobs_all, sizeof(Job) * (size_t)njobs) != hipSuccess) {
-    sba_batch_destroy(b);
-    *status = SBA_ERR_HIP;
-    return nullptr;
-  }
-  Job *hj = (Job *)calloc((size_t)njobs, sizeof(Job));
-  int pos = 0;
-  size_t off = 0;
   for (int sq = 0; sq < 2; ++sq)
     for (int L = 0; L <= SBA_MAX_LG; ++L) {
       if (!count[sq][L]) continue;
-      BatchClass &bc = b->cls[b->nclass++];
-      bc.L = L;
-      bc.sqr = sq;
-      bc.njobs = 0;
-      bc.nbmax = 0;
-      bc.djobs = b->djobs_all + pos;
-      bc.asum = bc.bsum = bc.zsum = 0;
+      ClassDesc k = {L, sq, pos, 0, 0, 0, 0, 0, heuristic_plan(L)};
       for (int i = 0; i < njobs; ++i) {
         const int isq = jobs[i].bp == nullptr;
         const uint64_t an = (uint64_t)jobs[i].an, bn = isq ? an : (uint64_t)jobs[i].bn;
@@ -1222,15 +959,104 @@ extern "C" sba_batch *sba_batch_create(const sba_job *jobs, int njobs, int use_g
         j.zlen = (uint32_t)(an + bn - 1);
         j.Z = (uint32_t)(an + bn);
         j.nb = carry_blocks(j.Z);
-        job_layout(j, (char *)b->ws + off, 1u << L);
-        off += (ws_need((size_t)1 << L) + 255) & ~(size_t)255;
-        bc.njobs++;
-        if (j.nb > bc.nbmax) bc.nbmax = j.nb;
-        bc.asum += j.an;
-        bc.bsum += j.bn;
-        bc.zsum += j.Z;
+        job_layout(j, ws + off, 1u << L);
+        off += job_ws(jobs[i]);
+        k.n++;
+        if (j.nb > k.nbmax) k.nbmax = j.nb;
+        k.as += j.an;
+        k.bs += j.bn;
+        k.zs += j.Z;
       }
+      k.plan = plan_for(L, sq, k.n);
+      cls[ncls++] = k;
     }
+  return ncls;
+}
+
+static int jobs_in_band(const sba_job *jobs, int njobs) {
+  if (njobs <= 0 || !jobs) return 0;
+  for (int i = 0; i < njobs; ++i)
+    if (!in_band_direct(jobs[i].an, jobs[i].bp ? jobs[i].bn : jobs[i].an)) return 0;
+  return 1;
+}
+
+static int jobs_lmax(const sba_job *jobs, int njobs) {
+  int Lmax = 0;
+  for (int i = 0; i < njobs; ++i) {
+    const uint64_t an = (uint64_t)jobs[i].an, bn = jobs[i].bp ? (uint64_t)jobs[i].bn : an;
+    const int L = lg_for(an + bn - 1);
+    if (L > Lmax) Lmax = L;
+  }
+  return Lmax;
+}
+
+struct sba_batch {
+  int dev;
+  void *ws;                        // all jobs' workspaces
+  Job *djobs_all;
+  int nclass;
+  ClassDesc cls[2 * (SBA_MAX_LG + 1)];
+  hipStream_t cap_stream;          // private stream used for capture
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  int use_graph;
+};
+
+static int enqueue_classes(DevCtx &c, Job *djobs, const ClassDesc *cls, int ncls,
+                           hipStream_t st) {
+  for (int k = 0; k < ncls; ++k) {
+    JobRef jr;
+    jr.arr = djobs + cls[k].start;
+    memset(&jr.one, 0, sizeof jr.one);
+    const int rc = enqueue_class(c, cls[k].L, cls[k].sqr, cls[k].plan, jr, cls[k].n,
+                                 cls[k].nbmax, cls[k].as, cls[k].bs, cls[k].zs, st);
+    if (rc != SBA_OK) return rc;
+  }
+  return SBA_OK;
+}
+
+extern "C" void sba_batch_destroy(sba_batch *b) {
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+  if (b->ws) (void)hipFree(b->ws);
+  if (b->djobs_all) (void)hipFree(b->djobs_all);
+  free(b);
+}
+
+// Each class takes the plan stored for (L, kind, bucket of its job count)
+// when the batch is created: run the autotuner on the jobs first to have
+// measured plans captured.
+extern "C" sba_batch *sba_batch_create(const sba_job *jobs, int njobs, int use_graph,
+                                       int *status) {
+  int st_dummy;
+  if (!status) status = &st_dummy;
+  *status = SBA_OK;
+  if (!jobs_in_band(jobs, njobs)) {
+    *status = SBA_BAND;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  DevCtx *c;
+  if ((*status = get_ctx(&c)) != 0) return nullptr;
+  sba_batch *b = (sba_batch *)calloc(1, sizeof(sba_batch));
+  if (!b) { *status = SBA_ERR_HIP; return nullptr; }
+  b->use_graph = use_graph;
+  (void)hipGetDevice(&b->dev);
+  size_t total = 0;
+  for (int i = 0; i < njobs; ++i) total += job_ws(jobs[i]);
+  int rc = ctx_twiddles(*c, jobs_lmax(jobs, njobs), 0);
+  if (rc) { free(b); *status = rc; return nullptr; }
+  if (hipMalloc(&b->ws, total) != hipSuccess ||
+      hipMalloc(&b->djobs_all, sizeof(Job) * (size_t)njobs) != hipSuccess) {
+    sba_batch_destroy(b);
+    *status = SBA_ERR_HIP;
+    return nullptr;
+  }
+  Job *hj = (Job *)calloc((size_t)njobs, sizeof(Job));
+  if (!hj) { sba_batch_destroy(b); *status = SBA_ERR_HIP; return nullptr; }
+  b->nclass = classify_jobs(jobs, njobs, (char *)b->ws, hj, b->cls);
   hipError_t e = hipMemcpy(b->djobs_all, hj, sizeof(Job) * (size_t)njobs, hipMemcpyHostToDevice);
   free(hj);
   if (e != hipSuccess) { sba_batch_destroy(b); *status = SBA_ERR_HIP; return nullptr; }
@@ -1243,7 +1069,7 @@ extern "C" sba_batch *sba_batch_create(const sba_job *jobs, int njobs, int use_g
     }
     const int saved = g_prof_on;
     g_prof_on = 0;                               // no event records inside the graph
-    rc = batch_enqueue(b, b->cap_stream);
+    rc = enqueue_classes(*c, b->djobs_all, b->cls, b->nclass, b->cap_stream);
     g_prof_on = saved;
     hipError_t e2 = hipStreamEndCapture(b->cap_stream, &b->graph);
     if (rc != SBA_OK || e2 != hipSuccess ||
@@ -1264,7 +1090,10 @@ extern "C" int sba_batch_run(sba_batch *b, void *stream) {
     return SBA_OK;
   }
   std::lock_guard<std::mutex> lk(g_mu);
-  return batch_enqueue(b, st);
+  DevCtx *c;
+  int rc = get_ctx(&c);
+  if (rc) return rc;
+  return enqueue_classes(*c, b->djobs_all, b->cls, b->nclass, st);
 }
 
 extern "C" int sba_batch_classes(const sba_batch *b) { return b ? b->nclass : 0; }
@@ -1273,23 +1102,29 @@ extern "C" int sba_batch_classes(const sba_batch *b) { return b ? b->nclass : 0;
 // one-shot batch launch into a caller workspace (serving: no allocation, no
 // graph; the job table is staged through the workspace's tail)
 // ---------------------------------------------------------------------------
-static size_t job_ws(const sba_job &j) {
-  const uint64_t an = (uint64_t)j.an, bn = j.bp ? (uint64_t)j.bn : an;
-  return (ws_need((size_t)1 << lg_for(an + bn - 1)) + 255) & ~(size_t)255;
+extern "C" size_t sba_batch_ws_bytes(const sba_job *jobs, int njobs) {
+  if (!jobs_in_band(jobs, njobs)) return 0;
+  size_t t = 0;
+  for (int i = 0; i < njobs; ++i) t += job_ws(jobs[i]);
+  return t + table_bytes(njobs);
 }
 
-extern "C" size_t sba_batch_ws_bytes(const sba_job *jobs, int njobs) {
-  size_t t = 0;
-  for (int i = 0; i < njobs; ++i) {
-    if (!in_band(jobs[i].an, jobs[i].bp ? jobs[i].bn : jobs[i].an)) return 0;
-    t += job_ws(jobs[i]);
-  }
-  return t + (((size_t)njobs * sizeof(Job) + 255) & ~(size_t)255);
+// stage the job table of `jobs` (classified) at the workspace tail; on
+// success *dj is the device table and cls/ncls describe the classes
+static int stage_jobs(const sba_job *jobs, int njobs, void *ws, size_t need, hipStream_t st,
+                      Job **dj, ClassDesc *cls, int *ncls) {
+  *dj = (Job *)((char *)ws + need - table_bytes(njobs));
+  Job *hj = (Job *)calloc((size_t)njobs, sizeof(Job));
+  if (!hj) return SBA_ERR_HIP;
+  *ncls = classify_jobs(jobs, njobs, (char *)ws, hj, cls);
+  hipError_t e = hipMemcpyAsync(*dj, hj, sizeof(Job) * (size_t)njobs, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);   // hj is pageable and freed below
+  free(hj);
+  return e == hipSuccess ? SBA_OK : SBA_ERR_HIP;
 }
 
 extern "C" int sba_batch_launch_ws(const sba_job *jobs, int njobs, void *ws, size_t ws_bytes,
                                    void *stream) {
-  if (njobs <= 0) return SBA_BAND;
   const size_t need = sba_batch_ws_bytes(jobs, njobs);
   if (!need) return SBA_BAND;
   if (!ws || ws_bytes < need) return SBA_ERR_WORKSPACE;
@@ -1298,60 +1133,114 @@ extern "C" int sba_batch_launch_ws(const sba_job *jobs, int njobs, void *ws, siz
   DevCtx *c;
   int rc = get_ctx(&c);
   if (rc) return rc;
-  int Lmax = 0;
-  for (int i = 0; i < njobs; ++i) {
-    const uint64_t an = (uint64_t)jobs[i].an, bn = jobs[i].bp ? (uint64_t)jobs[i].bn : an;
-    const int L = lg_for(an + bn - 1);
-    if (L > Lmax) Lmax = L;
-  }
-  if ((rc = ctx_twiddles(*c, Lmax, st)) != 0) return rc;
-  // job table at the tail of the workspace
-  Job *dj = (Job *)((char *)ws + need - (((size_t)njobs * sizeof(Job) + 255) & ~(size_t)255));
-  Job *hj = (Job *)calloc((size_t)njobs, sizeof(Job));
-  if (!hj) return SBA_ERR_HIP;
-  struct Cls { int L, sqr, start, n; uint32_t nbmax; double as, bs, zs; } cls[2 * (SBA_MAX_LG + 1)];
-  int ncls = 0, pos = 0;
-  size_t off = 0;
-  for (int sq = 0; sq < 2; ++sq)
-    for (int L = 0; L <= SBA_MAX_LG; ++L) {
-      Cls k = {L, sq, pos, 0, 0, 0, 0, 0};
-      for (int i = 0; i < njobs; ++i) {
-        const int isq = jobs[i].bp == nullptr;
-        const uint64_t an = (uint64_t)jobs[i].an, bn = isq ? an : (uint64_t)jobs[i].bn;
-        if (isq != sq || lg_for(an + bn - 1) != L) continue;
-        Job &j = hj[pos++];
-        const bool swap = !isq && an < bn;
-        j.a = swap ? jobs[i].bp : jobs[i].ap;
-        j.b = isq ? jobs[i].ap : (swap ? jobs[i].ap : jobs[i].bp);
-        j.an = (uint32_t)(swap ? bn : an);
-        j.bn = (uint32_t)(swap ? an : bn);
-        j.r = jobs[i].rp;
-        j.zlen = (uint32_t)(an + bn - 1);
-        j.Z = (uint32_t)(an + bn);
-        j.nb = carry_blocks(j.Z);
-        job_layout(j, (char *)ws + off, 1u << L);
-        off += job_ws(jobs[i]);
-        k.n++;
-        if (j.nb > k.nbmax) k.nbmax = j.nb;
-        k.as += j.an;
-        k.bs += j.bn;
-        k.zs += j.Z;
-      }
-      if (k.n) cls[ncls++] = k;
-    }
-  hipError_t e = hipMemcpyAsync(dj, hj, sizeof(Job) * (size_t)njobs, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);   // hj is pageable and freed below
-  free(hj);
-  if (e != hipSuccess) return SBA_ERR_HIP;
-  for (int k = 0; k < ncls; ++k) {
-    JobRef jr;
-    jr.arr = dj + cls[k].start;
-    memset(&jr.one, 0, sizeof jr.one);
-    rc = enqueue_class(*c, cls[k].L, cls[k].sqr, jr, (uint32_t)cls[k].n, cls[k].nbmax, cls[k].as,
-                       cls[k].bs, cls[k].zs, st);
-    if (rc != SBA_OK) return rc;
-  }
+  if ((rc = ctx_twiddles(*c, jobs_lmax(jobs, njobs), st)) != 0) return rc;
+  Job *dj;
+  ClassDesc cls[2 * (SBA_MAX_LG + 1)];
+  int ncls = 0;
+  if ((rc = stage_jobs(jobs, njobs, ws, need, st, &dj, cls, &ncls)) != SBA_OK) return rc;
+  return enqueue_classes(*c, dj, cls, ncls, st);
+}
+
+// ---------------------------------------------------------------------------
+// plan table access and plan timing (the autotuner's measurements)
+// ---------------------------------------------------------------------------
+extern "C" int sba_job_class(const sba_job *j, int *L, int *sqr) {
+  if (!j || !in_band_direct(j->an, j->bp ? j->bn : j->an)) return SBA_BAND;
+  const uint64_t an = (uint64_t)j->an, bn = j->bp ? (uint64_t)j->bn : an;
+  *L = lg_for(an + bn - 1);
+  *sqr = j->bp == nullptr;
   return SBA_OK;
+}
+
+extern "C" int sba_plan_bucket(int njobs) { return plan_bucket(njobs < 1 ? 1u : (uint32_t)njobs); }
+
+extern "C" int sba_plan_set(int L, int sqr, int bucket, int last, int bm, int split) {
+  Plan p = {last, bm, split};
+  if (L < 10 || L > SBA_MAX_LG || bucket < 0 || bucket >= SBA_PLAN_BUCKETS || !plan_ok(L, p))
+    return SBA_BAND;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_plan_init) plan_defaults();
+  g_plan[sqr ? 1 : 0][L][bucket] = p;
+  return SBA_OK;
+}
+
+extern "C" int sba_plan_get(int L, int sqr, int bucket, int *last, int *bm, int *split) {
+  if (L < 10 || L > SBA_MAX_LG || bucket < 0 || bucket >= SBA_PLAN_BUCKETS) return SBA_BAND;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_plan_init) plan_defaults();
+  const Plan &p = g_plan[sqr ? 1 : 0][L][bucket];
+  *last = p.last;
+  *bm = p.bm;
+  *split = p.split;
+  return SBA_OK;
+}
+
+extern "C" void sba_plan_reset(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  plan_defaults();
+}
+
+extern "C" int sba_plan_heuristic(int L, int *last, int *bm, int *split) {
+  if (L < 10 || L > SBA_MAX_LG) return SBA_BAND;
+  const Plan p = heuristic_plan(L);
+  *last = p.last;
+  *bm = p.bm;
+  *split = p.split;
+  return SBA_OK;
+}
+
+extern "C" int sba_plan_candidates(int L, int *plans, int max) {
+  int n = 0;
+  for (int last = 10; last <= 13; ++last)
+    for (int bm = 3; bm <= 5; ++bm)
+      for (int split = 0; split <= 1; ++split) {
+        const Plan p = {last, bm, split};
+        if (!plan_ok(L, p)) continue;
+        if (plans && n < max) {
+          plans[3 * n] = p.last;
+          plans[3 * n + 1] = p.bm;
+          plans[3 * n + 2] = p.split;
+        }
+        ++n;
+      }
+  return n;
+}
+
+extern "C" int sba_class_time(const sba_job *jobs, int njobs, int last, int bm, int split,
+                              int reps, void *ws, size_t ws_bytes, void *stream, float *ms) {
+  const size_t need = sba_batch_ws_bytes(jobs, njobs);
+  if (!need || reps < 1 || !ms) return SBA_BAND;
+  if (!ws || ws_bytes < need) return SBA_ERR_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lk(g_mu);
+  DevCtx *c;
+  int rc = get_ctx(&c);
+  if (rc) return rc;
+  if ((rc = ctx_twiddles(*c, jobs_lmax(jobs, njobs), st)) != 0) return rc;
+  Job *dj;
+  ClassDesc cls[2 * (SBA_MAX_LG + 1)];
+  int ncls = 0;
+  if ((rc = stage_jobs(jobs, njobs, ws, need, st, &dj, cls, &ncls)) != SBA_OK) return rc;
+  if (ncls != 1) return SBA_BAND;                 // one size class per measurement
+  cls[0].plan = Plan{last, bm, split};
+  if (!plan_ok(cls[0].L, cls[0].plan)) return SBA_BAND;
+  const int saved = g_prof_on;
+  g_prof_on = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  rc = enqueue_classes(*c, dj, cls, 1, st);       // warm-up
+  if (rc == SBA_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = SBA_ERR_HIP;
+  if (rc == SBA_OK && hipEventRecord(e0, st) != hipSuccess) rc = SBA_ERR_HIP;
+  for (int r = 0; r < reps && rc == SBA_OK; ++r) rc = enqueue_classes(*c, dj, cls, 1, st);
+  if (rc == SBA_OK && hipEventRecord(e1, st) != hipSuccess) rc = SBA_ERR_HIP;
+  if (rc == SBA_OK && hipEventSynchronize(e1) != hipSuccess) rc = SBA_ERR_HIP;
+  float t = 0.f;
+  if (rc == SBA_OK && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = SBA_ERR_HIP;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  g_prof_on = saved;
+  if (rc == SBA_OK) *ms = t / (float)reps;
+  return rc;
 }
 
 // the plain per-call path over the same jobs: one sba_gpu_mul_dev /

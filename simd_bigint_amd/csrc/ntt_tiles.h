@@ -1,0 +1,487 @@
+// ntt_tiles.h -- device side of the tiled NTT passes: argument blocks, job
+// descriptors, LDS tile machinery and the pass kernels (templates), plus the
+// table that maps a pass plan to kernel instantiations. Included by
+// ntt_engine.hip and by ntt_pass_bm.hip, which instantiates the kernels of
+// one register-chunk radix per object so the three compile in parallel.
+#pragma once
+#include "gpu_arith.h"
+#include "simd_bigint_amd.h"
+
+// ---------------------------------------------------------------------------
+// kernel argument block (by value)
+// ---------------------------------------------------------------------------
+struct PrimeArgs {
+  uint32_t p[SBA_K], pinv[SBA_K], r2[SBA_K];
+  const uint32_t *wn[SBA_K];
+  const uint32_t *wni[SBA_K];
+};
+
+struct GarnerArgs {
+  uint32_t p[SBA_K], pinv[SBA_K];
+  uint32_t scale[SBA_K];          // N^-1 * R^2 mod p (undoes N and the pointwise R^-1)
+  uint32_t cm[SBA_K][SBA_K];      // (prod_{j<k} p_j mod p_q) * R mod p_q
+  uint32_t im[SBA_K];             // (prod_{j<q} p_j)^-1 * R mod p_q
+};
+
+#define NT 256
+// occupancy hint for the transform kernels (waves per SIMD) and the widest
+// register chunk (radix 2^SBA_BMAX), fixed by measurement
+#ifndef SBA_WAVES
+#define SBA_WAVES 1
+#endif
+#ifndef SBA_BMAX
+#define SBA_BMAX 5
+#endif
+// 1: the first forward pass runs one prime per block (grid y = prime)
+#ifndef SBA_FIRST_SPLIT
+#define SBA_FIRST_SPLIT 1
+#endif
+// 1: widest chunks even where group addressing is not linear
+#ifndef SBA_CHUNK_GREEDY
+#define SBA_CHUNK_GREEDY 0
+#endif
+// LDS padding: PW words after every 32-word row. A column tile (TC columns
+// of 2^ELL rows) pads by TC words for TC <= 16 -- the last chunk then puts
+// the TC columns x 32/TC groups of a half-wave on 32 distinct banks -- and
+// by 1 word otherwise (the contiguous tiles).
+template <int LOGTC> struct Pad {
+  static constexpr int PW = LOGTC <= 4 ? (1 << LOGTC) : 1;
+  static constexpr int words(int ne) { return ne + (ne / 32) * PW; }
+};
+template <int LOGTC> __device__ __forceinline__ uint32_t lpadt(uint32_t i) {
+  return i + (i >> 5) * Pad<LOGTC>::PW;
+}
+
+// blockIdx -> tile so that consecutive tiles (which share cache lines and
+// twiddle rows) run on the same XCD: the dispatcher hands block b to XCD
+// b % 8.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
+  if (n & 7) return b;
+  return (b & 7) * (n >> 3) + (b >> 3);
+}
+
+template <int X> struct ILog2 { static constexpr int v = 1 + ILog2<X / 2>::v; };
+template <> struct ILog2<1> { static constexpr int v = 0; };
+
+// residue of a limb, lazy in [0, 2p): (hi * 2^32 mod p) + (lo mod p).
+// p > 2^30 so lo < 4p: two folds bring lo into [0, p).
+__device__ __forceinline__ uint32_t limb_residue(uint64_t x, uint32_t p, uint32_t pinv,
+                                                 uint32_t r2) {
+  uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+  uint32_t h = red1(mont_mul(hi, r2, p, pinv), p);
+  uint32_t l2 = lo - 2u * p;
+  uint32_t l = l2 < lo ? l2 : lo;
+  return h + red1(l, p);
+}
+
+// ---------------------------------------------------------------------------
+// one chunk of B levels (local levels T0 .. T0+B-1 of an ELL-level pass)
+//
+// LDS index of tile element (k, c) is (k << LOGTC) | c, padded by one word
+// per 32 (lpad) against bank conflicts. A chunk's group of 2^B elements has
+// stride D = 2^(LB + LOGTC) words; when D >= 32, or when the whole group
+// sits inside one 32-word row (D * 2^B <= 32), the padded addresses are
+// base + m * const, so one address register serves the group (ds offsets).
+// ---------------------------------------------------------------------------
+template <int ELL, int LOGTC, int T0, int B> struct ChunkGeom {
+  static constexpr int LB = ELL - T0 - B;
+  static constexpr int LOGD = LB + LOGTC;
+  static constexpr bool LINEAR = LOGD >= 5 || LOGD + B <= 5;
+  static constexpr int DP =
+      LOGD >= 5 ? (1 << LOGD) + (1 << (LOGD - 5)) * Pad<LOGTC>::PW : (1 << LOGD);
+};
+
+template <int ELL, int LOGTC, int T0, int B, bool INV>
+__device__ __forceinline__ void tile_chunk(uint32_t *lds, const uint32_t *__restrict__ tw,
+                                           uint32_t H, uint32_t p, uint32_t pinv) {
+  constexpr int TC = 1 << LOGTC;
+  constexpr int NE = 1 << (ELL + LOGTC);
+  constexpr int GPT = NE / ((1 << B) * NT);
+  using G = ChunkGeom<ELL, LOGTC, T0, B>;
+  constexpr int LB = G::LB;
+  static_assert(GPT >= 1, "chunk too wide for the tile");
+  __syncthreads();
+#pragma unroll
+  for (int gi = 0; gi < GPT; ++gi) {
+    const uint32_t g = threadIdx.x + gi * NT;
+    const uint32_t c = g & (TC - 1);
+    const uint32_t rest = g >> LOGTC;
+    const uint32_t lo = rest & ((1u << LB) - 1);
+    const uint32_t hb = rest >> LB;
+    const uint32_t kbase = (hb << (ELL - T0)) | lo;
+    const uint32_t hj = (H << T0) | hb;
+    const uint32_t i0 = (kbase << LOGTC) | c;
+    uint32_t *l0 = lds + lpadt<LOGTC>(i0);
+    uint32_t v[1 << B];
+#pragma unroll
+    for (int m = 0; m < (1 << B); ++m)
+      v[m] = G::LINEAR ? l0[m * G::DP] : lds[lpadt<LOGTC>(i0 + ((uint32_t)m << G::LOGD))];
+    if (!INV) {
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        const int half = 1 << (B - u - 1);
+#pragma unroll
+        for (int m2 = 0; m2 < (1 << u); ++m2) {
+          const uint32_t w = tw[(hj << u) | (uint32_t)m2];
+#pragma unroll
+          for (int r = 0; r < half; ++r) {
+            const int m = (m2 << (B - u)) | r;
+            bfly_fwd(v[m], v[m + half], w, p, pinv);
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int u = B - 1; u >= 0; --u) {
+        const int half = 1 << (B - u - 1);
+#pragma unroll
+        for (int m2 = 0; m2 < (1 << u); ++m2) {
+          const uint32_t w = tw[(hj << u) | (uint32_t)m2];
+#pragma unroll
+          for (int r = 0; r < half; ++r) {
+            const int m = (m2 << (B - u)) | r;
+            bfly_inv(v[m], v[m + half], w, p, pinv);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < (1 << B); ++m) {
+      if (G::LINEAR) l0[m * G::DP] = v[m];
+      else lds[lpadt<LOGTC>(i0 + ((uint32_t)m << G::LOGD))] = v[m];
+    }
+  }
+}
+
+// chunk sizes: the largest B <= BM whose group addressing is linear,
+// falling back to BM when no size is. BMAX (radix 2^BMAX) is a plan choice.
+template <int ELL, int LOGTC, int BMAX> struct ChunkPlan {
+  static constexpr int E = (1 << (ELL + LOGTC)) / NT;
+  static constexpr int LE = ILog2<E>::v;
+  static constexpr int BM = LE < BMAX ? LE : BMAX;
+  static constexpr int pick(int t0) {
+    int r = ELL - t0;
+    int bmax = r < BM ? r : BM;
+    if (SBA_CHUNK_GREEDY) return bmax;
+    for (int b = bmax; b >= 1; --b) {
+      int logd = ELL - t0 - b + LOGTC;
+      if (logd >= 5 || logd + b <= 5) return b;
+    }
+    return bmax;
+  }
+};
+
+template <int ELL, int LOGTC, int BMAX, int T0>
+__device__ __forceinline__ void tile_fwd(uint32_t *lds, const uint32_t *tw, uint32_t H,
+                                         uint32_t p, uint32_t pinv) {
+  if constexpr (T0 < ELL) {
+    constexpr int B = ChunkPlan<ELL, LOGTC, BMAX>::pick(T0);
+    tile_chunk<ELL, LOGTC, T0, B, false>(lds, tw, H, p, pinv);
+    tile_fwd<ELL, LOGTC, BMAX, T0 + B>(lds, tw, H, p, pinv);
+  }
+}
+
+template <int ELL, int LOGTC, int BMAX, int T0>
+__device__ __forceinline__ void tile_inv(uint32_t *lds, const uint32_t *tw, uint32_t H,
+                                         uint32_t p, uint32_t pinv) {
+  if constexpr (T0 < ELL) {
+    constexpr int B = ChunkPlan<ELL, LOGTC, BMAX>::pick(T0);
+    tile_inv<ELL, LOGTC, BMAX, T0 + B>(lds, tw, H, p, pinv);
+    tile_chunk<ELL, LOGTC, T0, B, true>(lds, tw, H, p, pinv);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// job descriptors: every kernel serves one multiply (JobRef.one) or a whole
+// size class of a batch (JobRef.arr[blockIdx.z]); all jobs of one launch
+// share the transform length N = 2^L.
+// ---------------------------------------------------------------------------
+struct Job {
+  const uint64_t *a, *b;          // operand limbs (device)
+  uint64_t *r;                    // product limbs (device), an+bn of them
+  uint32_t *aspec, *bspec;        // K x N residues each
+  uint8_t *lmaps;                 // one carry map per limb, padded to blocks
+  uint32_t *bmaps, *bcin;         // per carry block
+  uint32_t an, bn, zlen, Z, nb;   // limbs, coefficients, product limbs, carry blocks
+};
+// global-address-space views of descriptor pointers: loads through a pointer
+// read from memory are otherwise generic (flat) and serialised against LDS
+typedef __attribute__((address_space(1))) uint32_t g_u32;
+typedef __attribute__((address_space(1))) const uint32_t gc_u32;
+typedef __attribute__((address_space(1))) uint64_t g_u64;
+typedef __attribute__((address_space(1))) const uint64_t gc_u64;
+typedef __attribute__((address_space(1))) uint8_t g_u8;
+
+// accumulator target of a chunked product's last inverse pass (dst ==
+// nullptr: ordinary in-place store)
+struct AccArgs {
+  uint32_t *dst;                  // K x stride residues
+  uint32_t stride, off, ovl, lim;
+};
+
+struct JobRef {
+  const Job *arr;
+  Job one;
+};
+// The descriptor is uniform across the block; readfirstlane moves every field
+// into scalar registers so tile addresses stay scalar-base + lane-offset.
+template <typename T> __device__ __forceinline__ T *uni_ptr(T *p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (T *)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ uint32_t uni_u32(uint32_t x) {
+  return __builtin_amdgcn_readfirstlane(x);
+}
+__device__ __forceinline__ Job get_job(const JobRef &jr) {
+  Job j;
+  if (jr.arr) {
+    j = jr.arr[blockIdx.z];
+  } else {
+    j = jr.one;
+  }
+  j.a = uni_ptr(j.a); j.b = uni_ptr(j.b); j.r = uni_ptr(j.r);
+  j.aspec = uni_ptr(j.aspec); j.bspec = uni_ptr(j.bspec); j.lmaps = uni_ptr(j.lmaps);
+  j.bmaps = uni_ptr(j.bmaps); j.bcin = uni_ptr(j.bcin);
+  j.an = uni_u32(j.an); j.bn = uni_u32(j.bn); j.zlen = uni_u32(j.zlen); j.Z = uni_u32(j.Z);
+  j.nb = uni_u32(j.nb);
+  return j;
+}
+
+// ---------------------------------------------------------------------------
+// pass kernels
+// ---------------------------------------------------------------------------
+// First forward pass (global levels [0, ELL)): limbs -> residues of all K
+// primes -> ELL levels -> spec[q][.]. Tiles = column tiles (H = 0).
+// which = 0: operand a into aspec; 1: operand b into bspec.
+template <int ELL, int LOGTC, int BMAX>
+__global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd(PrimeArgs pa, JobRef jr, int which,
+                                                             int L) {
+  constexpr int TC = 1 << LOGTC;
+  constexpr int NE = 1 << (ELL + LOGTC);
+  constexpr int E = NE / NT;
+  __shared__ uint32_t lds[Pad<LOGTC>::words(NE)];
+  const Job jb = get_job(jr);
+  gc_u64 *limbs = (gc_u64 *)(which ? jb.b : jb.a);
+  g_u32 *spec = (g_u32 *)(which ? jb.bspec : jb.aspec);
+  const uint32_t nlimbs = which ? jb.bn : jb.an;
+  const uint32_t N = 1u << L;             // N <= 2^25: 32-bit offsets throughout
+  const uint32_t S = 1u << (L - ELL);
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t col0 = tile << LOGTC;
+  const uint32_t nl = nlimbs < N ? nlimbs : N;
+  uint32_t *lt = lds + lpadt<LOGTC>(threadIdx.x);
+  // element e = tid + i*NT sits at col0 + toff + uoff(i): a per-thread part
+  // and a uniform part (scalar base + 32-bit lane offset addressing)
+  const uint32_t toff = (threadIdx.x >> LOGTC) * S + (threadIdx.x & (TC - 1));
+  // grid y = K: one prime per block; grid y = 1: all primes per block
+  const bool split = gridDim.y > 1;
+  const int q0 = split ? (int)blockIdx.y : 0;
+  const int q1 = split ? q0 + 1 : SBA_K;
+#pragma unroll 1
+  for (int q = q0; q < q1; ++q) {
+    const uint32_t p = pa.p[q], pinv = pa.pinv[q], r2 = pa.r2[q];
+    __syncthreads();
+    // the tile's limbs are re-read per prime (cache-resident after the first)
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+      gc_u64 *lp = limbs + uoff;
+      lt[i * Pad<LOGTC>::words(NT)] = limb_residue(uoff + toff < nl ? lp[toff] : 0, p, pinv, r2);
+    }
+    tile_fwd<ELL, LOGTC, BMAX, 0>(lds, pa.wn[q], 0, p, pinv);
+    __syncthreads();
+    g_u32 *out = spec + q * N;
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+      (out + uoff)[toff] = lt[i * Pad<LOGTC>::words(NT)];
+    }
+  }
+}
+
+// In-place pass over global levels [s0, s0 + ELL) of one prime
+// (blockIdx.y), forward or inverse, on aspec (which = 0) or bspec (1).
+template <int ELL, int LOGTC, bool INV, int BMAX>
+__global__ __launch_bounds__(NT, SBA_WAVES) void k_pass(PrimeArgs pa, JobRef jr, int which, int L,
+                                                        int s0, AccArgs acc) {
+  constexpr int TC = 1 << LOGTC;
+  constexpr int NE = 1 << (ELL + LOGTC);
+  constexpr int E = NE / NT;
+  __shared__ uint32_t lds[Pad<LOGTC>::words(NE)];
+  const Job jb = get_job(jr);
+  g_u32 *spec = (g_u32 *)(which ? jb.bspec : jb.aspec);
+  const int q = blockIdx.y;
+  const uint32_t N = 1u << L;
+  const uint32_t S = 1u << (L - s0 - ELL);
+  const uint32_t ctiles = S >> LOGTC;
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t H = tile / ctiles;
+  const uint32_t col0 = (tile % ctiles) << LOGTC;
+  const uint32_t base = (H << (L - s0)) + col0;
+  g_u32 *buf = spec + q * N + base;
+  const uint32_t p = pa.p[q], pinv = pa.pinv[q];
+  uint32_t *lt = lds + lpadt<LOGTC>(threadIdx.x);
+  const uint32_t toff = (threadIdx.x >> LOGTC) * S + (threadIdx.x & (TC - 1));
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+    lt[i * Pad<LOGTC>::words(NT)] = (buf + uoff)[toff];
+  }
+  if (!INV) tile_fwd<ELL, LOGTC, BMAX, 0>(lds, pa.wn[q], H, p, pinv);
+  else tile_inv<ELL, LOGTC, BMAX, 0>(lds, pa.wni[q], H, p, pinv);
+  __syncthreads();
+  if (INV && acc.dst) {
+    // one chunk of a chunked product: coefficient x of the chunk lands at
+    // acc.off + x of the accumulator; x < acc.ovl adds to the previous
+    // chunk's tail, x >= acc.lim (beyond the chunk's product) is skipped
+    g_u32 *d = (g_u32 *)acc.dst + q * acc.stride + acc.off + base;
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+      const uint32_t x = base + uoff + toff;
+      if (x < acc.lim) {
+        uint32_t v = lt[i * Pad<LOGTC>::words(NT)];
+        if (x < acc.ovl) v = red1(v, p) + red1((d + uoff)[toff], p);
+        (d + uoff)[toff] = v;
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+    (buf + uoff)[toff] = lt[i * Pad<LOGTC>::words(NT)];
+  }
+}
+
+// Last forward levels of b + pointwise product with a's spectrum (or b's own
+// square) + first inverse levels, on contiguous tiles (S = 1, TC = 1).
+// LIMB_IN: the pass is the whole transform (s0 = 0) and reads b's limbs.
+template <int ELL, bool LIMB_IN, bool SQR, int BMAX>
+__global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr, int L) {
+  constexpr int NE = 1 << ELL;
+  constexpr int E = NE / NT;
+  __shared__ uint32_t lds[Pad<0>::words(NE)];
+  const Job jb = get_job(jr);
+  const int q = blockIdx.y;
+  const uint32_t N = 1u << L;
+  const uint32_t H = blockIdx.x;
+  const uint32_t base = H << ELL;
+  const uint32_t nb = jb.bn < N ? jb.bn : N;
+  const uint32_t p = pa.p[q], pinv = pa.pinv[q];
+  g_u32 *buf = (g_u32 *)jb.bspec + q * N + base;
+  gc_u32 *as = (gc_u32 *)jb.aspec + q * N + base;
+  gc_u64 *bl = (gc_u64 *)jb.b + base;
+  uint32_t *lt = lds + lpadt<0>(threadIdx.x);
+  const uint32_t tid = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    uint32_t v;
+    if (LIMB_IN) {
+      v = limb_residue(base + i * NT + tid < nb ? (bl + i * NT)[tid] : 0, p, pinv, pa.r2[q]);
+    } else {
+      v = (buf + i * NT)[tid];
+    }
+    lt[i * Pad<0>::words(NT)] = v;
+  }
+  tile_fwd<ELL, 0, BMAX, 0>(lds, pa.wn[q], H, p, pinv);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const uint32_t v = lt[i * Pad<0>::words(NT)];
+    const uint32_t o = SQR ? red1(v, p) : red1((as + i * NT)[tid], p);
+    lt[i * Pad<0>::words(NT)] = mont_mul(v, o, p, pinv);
+  }
+  tile_inv<ELL, 0, BMAX, 0>(lds, pa.wni[q], H, p, pinv);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < E; ++i) (buf + i * NT)[tid] = lt[i * Pad<0>::words(NT)];
+}
+
+// ---------------------------------------------------------------------------
+// launch tables
+// ---------------------------------------------------------------------------
+typedef void (*first_fn)(PrimeArgs, JobRef, int, int);
+typedef void (*pass_fn)(PrimeArgs, JobRef, int, int, int, AccArgs);
+typedef void (*fused_fn)(PrimeArgs, JobRef, int);
+struct Plan { int last, bm, split; };
+
+#define SBA_ELL_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
+template <int BM> struct KTab {
+  // first pass: ELL levels over column tiles of 2^(13 - ELL) columns
+  static first_fn first_two(int ell) {
+    switch (ell) {
+#define SBA_C(E) case E: return k_first_fwd<E, 13 - E, BM>;
+      SBA_ELL_CASES(SBA_C)
+#undef SBA_C
+    }
+    return nullptr;
+  }
+  static pass_fn inv_two(int ell) {
+    switch (ell) {
+#define SBA_C(E) case E: return k_pass<E, 13 - E, true, BM>;
+      SBA_ELL_CASES(SBA_C)
+#undef SBA_C
+    }
+    return nullptr;
+  }
+  static first_fn first_single(int L) {
+    switch (L) {
+      case 10: return k_first_fwd<10, 0, BM>;
+      case 11: return k_first_fwd<11, 0, BM>;
+      case 12: return k_first_fwd<12, 0, BM>;
+      case 13: return k_first_fwd<13, 0, BM>;
+    }
+    return nullptr;
+  }
+  static fused_fn fused_single(int L, int sqr) {
+    switch (L) {
+      case 10: return sqr ? k_fused<10, true, true, BM> : k_fused<10, true, false, BM>;
+      case 11: return sqr ? k_fused<11, true, true, BM> : k_fused<11, true, false, BM>;
+      case 12: return sqr ? k_fused<12, true, true, BM> : k_fused<12, true, false, BM>;
+      case 13: return sqr ? k_fused<13, true, true, BM> : k_fused<13, true, false, BM>;
+    }
+    return nullptr;
+  }
+  static pass_fn fwd_last(int last) {
+    switch (last) {
+      case 11: return k_pass<11, 0, false, BM>;
+      case 12: return k_pass<12, 0, false, BM>;
+      case 13: return k_pass<13, 0, false, BM>;
+    }
+    return nullptr;
+  }
+  static fused_fn fused_last(int last, int sqr) {
+    switch (last) {
+      case 11: return sqr ? k_fused<11, false, true, BM> : k_fused<11, false, false, BM>;
+      case 12: return sqr ? k_fused<12, false, true, BM> : k_fused<12, false, false, BM>;
+      case 13: return sqr ? k_fused<13, false, true, BM> : k_fused<13, false, false, BM>;
+    }
+    return nullptr;
+  }
+};
+
+struct Launchers {
+  first_fn first;
+  pass_fn fwd_last, inv_first;
+  fused_fn fused;
+};
+template <int BM> static Launchers launchers_bm(int L, int sqr, int last) {
+  Launchers k = {};
+  if (last == L) {
+    k.first = KTab<BM>::first_single(L);
+    k.fused = KTab<BM>::fused_single(L, sqr);
+  } else {
+    k.first = KTab<BM>::first_two(L - last);
+    k.fwd_last = KTab<BM>::fwd_last(last);
+    k.fused = KTab<BM>::fused_last(last, sqr);
+    k.inv_first = KTab<BM>::inv_two(L - last);
+  }
+  return k;
+}
+// defined in ntt_pass_bm.hip, one object per radix
+Launchers sba_launchers_bm3(int L, int sqr, int last);
+Launchers sba_launchers_bm4(int L, int sqr, int last);
+Launchers sba_launchers_bm5(int L, int sqr, int last);

@@ -121,8 +121,11 @@ def test_band_edges():
     L = sba.lib()
     a = np.ones(3, np.uint64)
     r = np.zeros(4, np.uint64)
-    # beyond 2^25 coefficients: rejected before any buffer is read
-    assert L.nt_mul(sba._arr(r), sba._arr(a), (1 << 24) + 2, sba._arr(a), 1 << 24) == 0
+    # beyond the reference's band (nt_mul's 0): rejected before any buffer
+    # is read; 2^25 trunks of 84 bits (balanced) / 88 bits (short operand)
+    assert L.nt_mul(sba._arr(r), sba._arr(a), 22020097, sba._arr(a), 22020097) == 0
+    assert L.nt_mul(sba._arr(r), sba._arr(a), 46137345, sba._arr(a), 1) == 0
+    assert L.nt_sqr(sba._arr(r), sba._arr(a), 22020097) == 0
     # exactly 2^25 coefficients is in band (largest transform)
     st = [3]
     x = xorshift_fill(1 << 24, st)

@@ -108,3 +108,25 @@ def test_divrem_short_dividend_mirrors_reference_lengths():
     L.divrem_u64(sba._arr(q), sba._arr(r), sba._arr(n), 3, sba._arr(d), 8,
                  ctypes.byref(qn), ctypes.byref(rn))
     assert qn.value == 1 and rn.value == 8 and q[0] == 0 and r[0] == 5 and not r[1:].any()
+
+
+def test_nt_band_matches_reference_trunk_counting():
+    """nt_mul's band is the reference's p50 band: at most 2^25 trunks in
+    the product. A short operand takes 88-bit trunks (2^25 * 88/64 =
+    46137344 limbs); a balanced pair past the 88-bit CRT capacity takes
+    84-bit trunks (2^25 * 84/64 / 2 = 22020096 limbs each). Outside, nt_*
+    return 0 before reading the operands (no device needed)."""
+    import simd_bigint_amd as sba
+    L = sba.lib()
+    assert L.sba_ref_nt_band(46137344, 1) == 1
+    assert L.sba_ref_nt_band(46137345, 1) == 0
+    assert L.sba_ref_nt_band(22020096, 22020096) == 1
+    assert L.sba_ref_nt_band(22020097, 22020097) == 0
+    assert L.sba_ref_nt_band(0, 5) == 0
+    a = np.ones(3, np.uint64)
+    r = np.zeros(4, np.uint64)
+    assert L.nt_mul(sba._arr(r), sba._arr(a), 46137345, sba._arr(a), 1) == 0
+    assert L.nt_sqr(sba._arr(r), sba._arr(a), 22020097) == 0
+    # the device band contains it
+    assert L.sba_gpu_in_band(46137344, 1) == 1
+    assert L.sba_gpu_in_band(22020096, 22020096) == 1
