@@ -295,6 +295,46 @@ def configs_leg(L, reps=10):
         out[name] = {"an": e["an"], "bn": e["bn"] or e["an"], "ms": round(ms, 4),
                      "ns_per_limb": round(ms * 1e6 / n, 5), "bit_exact": bool(exact)}
         log("bench:", name, out[name])
+    import simd_bigint_amd as sba
+    from simd_bigint_amd.workloads import xorshift_fill
+
+    def sha(x):
+        return hashlib.sha256(np.ascontiguousarray(x, dtype=np.uint64).tobytes()).hexdigest()
+
+    # divrem 2^21 / 2^20 limbs (divisor top bit set): Newton reciprocal on the
+    # GPU multiply; host buffers, so every multiply inside copies over PCIe
+    e = [x for x in gold["divrem"] if x["nn"] == 1 << 21 and x["fn"].endswith("_dc")][0]
+    st = [e["seed"]]
+    nn_ = xorshift_fill(e["nn"], st)
+    dd_ = xorshift_fill(e["dn"], st)
+    dd_[-1] |= np.uint64(1 << 63)
+    best = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        q, r = sba.divrem_u64(nn_, dd_, dc=True)
+        dt = (time.perf_counter() - t0) * 1e3
+        best = dt if best is None else min(best, dt)
+    out["cfg3_divrem_2^21/2^20"] = {
+        "nn": e["nn"], "dn": e["dn"], "ms": round(best, 3),
+        "bit_exact": bool(sha(q) == e["q_sha256"] and sha(r) == e["r_sha256"]),
+        "note": "host buffers; includes the PCIe copies of every multiply"}
+    log("bench: divrem", out["cfg3_divrem_2^21/2^20"])
+    # the metric multiply through the host entry nt_mul (H2D + D2H inside)
+    st = [42]
+    a = xorshift_fill(1 << 22, st)
+    b = xorshift_fill(1 << 22, st)
+    want = gl["metric_4Mi"]["sha256"]
+    best = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ok, r = sba.nt_mul(a, b)
+        dt = (time.perf_counter() - t0) * 1e3
+        best = dt if best is None else min(best, dt)
+    out["metric_via_host_nt_mul"] = {"an": 1 << 22, "bn": 1 << 22, "ms": round(best, 3),
+                                     "ns_per_limb": round(best * 1e6 / (1 << 22), 4),
+                                     "bit_exact": bool(ok == 1 and sha(r) == want),
+                                     "note": "PCIe-inclusive: 64 MiB in, 64 MiB out"}
+    log("bench: host nt_mul", out["metric_via_host_nt_mul"])
     return out
 
 

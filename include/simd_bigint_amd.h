@@ -118,6 +118,9 @@ int sba_gpu_in_band(int64_t an, int64_t bn);
  * one's spectrum of length 2^*Lc */
 int sba_gpu_chunking(int64_t an, int64_t bn, int sqr, int *Lc, uint64_t *chunk,
                      uint64_t *nchunks);
+/* L when a single product of (an, bn) runs at the radix-3 length 3 * 2^L
+ * (2^(L+1) < an+bn-1 <= 3 * 2^L, 13 <= L <= 22, not chunked), else 0 */
+int sba_gpu_radix3(int64_t an, int64_t bn, int sqr);
 uint64_t sba_gpu_min_operand_cap(void);
 int sba_gpu_max_lg(void);
 int sba_gpu_device_count(void);

@@ -153,6 +153,8 @@ def lib() -> ctypes.CDLL:
     L.sba_gpu_chunking.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ip,
                                    ctypes.POINTER(u64), ctypes.POINTER(u64)]
     L.sba_gpu_chunking.restype = ctypes.c_int
+    L.sba_gpu_radix3.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int]
+    L.sba_gpu_radix3.restype = ctypes.c_int
     L.sba_set_gpu_threshold.argtypes = [u64]
     L.sba_set_gpu_threshold.restype = None
     L.sba_get_gpu_threshold.argtypes = []

@@ -325,6 +325,16 @@ static int ctx_init(DevCtx &c) {
     c.ga_base.p[q] = (uint32_t)p;
     c.ga_base.pinv[q] = 0u - inv;
   }
+  for (int q = 0; q < SBA_K; ++q) {
+    // length-3 DFT constants: w = g^((p-1)/3) a cube root of unity
+    const uint64_t p = kPrimes[q];
+    const uint64_t w = h_powm(h_primroot(p), (p - 1) / 3, p);
+    const uint64_t inv2 = (p + 1) / 2;
+    const uint64_t h2 = (w + p - w * w % p) % p * inv2 % p;
+    c.pa.h1[q] = h_mont(p - inv2, p);
+    c.pa.h2[q] = h_mont(h2, p);
+    c.pa.h2i[q] = h_mont((p - h2) % p, p);
+  }
   for (int q = 1; q < SBA_K; ++q) {
     const uint64_t pq = kPrimes[q];
     uint64_t pp = 1;
@@ -598,7 +608,7 @@ static int enqueue_class(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef
     }
     {
       PROF("fused_last", 8.0 * bsum + (sqr ? 1 : 2) * KN4);
-      hipLaunchKernelGGL(k.fused, dim3(1, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L);
+      hipLaunchKernelGGL(k.fused, dim3(1, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L, 0);
     }
   } else {
     const int ell1 = L - pl.last;
@@ -612,14 +622,17 @@ static int enqueue_class(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef
       PROF("first_fwd", 8.0 * bsum + KN4);
       hipLaunchKernelGGL(k.first, dim3(ntile1, gy1, njobs), dim3(NT), 0, st, pa, jr, 1, L);
     }
-    if (!sqr) {
-      PROF("fwd_last", 2 * KN4);
-      hipLaunchKernelGGL(k.fwd_last, dim3(ntile2, SBA_K, njobs), dim3(NT), 0, st, pa, jr, 0, L,
-                         ell1, kNoAcc);
-    }
-    {
+    if (!sqr && SBA_MERGE_LAST) {
+      PROF("fused_last", 3 * KN4);
+      hipLaunchKernelGGL(k.fused2, dim3(ntile2, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L, 0);
+    } else {
+      if (!sqr) {
+        PROF("fwd_last", 2 * KN4);
+        hipLaunchKernelGGL(k.fwd_last, dim3(ntile2, SBA_K, njobs), dim3(NT), 0, st, pa, jr, 0, L,
+                           ell1, kNoAcc);
+      }
       PROF("fused_last", (sqr ? 2 : 3) * KN4);
-      hipLaunchKernelGGL(k.fused, dim3(ntile2, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L);
+      hipLaunchKernelGGL(k.fused, dim3(ntile2, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L, 0);
     }
     {
       PROF("inv_first", 2 * KN4);
@@ -772,7 +785,7 @@ static int enqueue_chunked(DevCtx &c, const ChunkChoice &cp, uint64_t *rp, const
     }
     {
       PROF("fused_last", 3 * KN4);
-      hipLaunchKernelGGL(k.fused, dim3(ntile2, SBA_K, 1), dim3(NT), 0, st, pa, jr, Lc);
+      hipLaunchKernelGGL(k.fused, dim3(ntile2, SBA_K, 1), dim3(NT), 0, st, pa, jr, Lc, 0);
     }
     {
       AccArgs aa;
@@ -800,6 +813,69 @@ static int enqueue_chunked(DevCtx &c, const ChunkChoice &cp, uint64_t *rp, const
   g.bmaps = (uint32_t *)(g.lmaps + nbcap * CARRY_LPB);
   g.bcin = g.bmaps + nbcap;
   return enqueue_garner(c, Nc, Zs, jg, 1, g.nb, (double)Z, st);
+}
+
+// ---------------------------------------------------------------------------
+// radix-3 lengths: a product with 2^(L+1) < zlen <= 3 * 2^L runs at length
+// 3 * 2^L instead of 2^(L+2) (a quarter fewer residues in every pass); the
+// column DFT-3 rides the first forward and the last inverse pass.
+// ---------------------------------------------------------------------------
+#define R3_MIN_L 13
+#define R3_MAX_L 22          // first-pass tiles keep >= 4 columns (2^11 per row)
+
+// L of the radix-3 class of zlen coefficients, 0 if none
+static int r3_lg(uint64_t zlen) {
+  if (!SBA_MERGE_LAST) return 0;
+  const int L2 = lg_for(zlen);
+  const int L = L2 - 2;
+  if (L < R3_MIN_L || L > R3_MAX_L) return 0;
+  return zlen <= (uint64_t)3 << L ? L : 0;
+}
+
+// the power-of-two plan of the same row length, with the first pass kept
+// within 1..9 levels (the radix-3 column tiles hold 2^11 residues per row)
+static Plan r3_plan(int L, int sqr, uint32_t njobs) {
+  Plan p = plan_for(L, sqr, njobs);
+  if (p.last >= L) p.last = L - 1;
+  if (p.last > 13) p.last = 13;
+  if (L - p.last > 9) p.last = L - 9;
+  return p;
+}
+
+static int enqueue_r3(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef &jr,
+                      uint32_t njobs, uint32_t nbmax, double asum, double bsum, double zsum,
+                      hipStream_t st) {
+  const int ell1 = L - pl.last;
+  if (ell1 < 1 || ell1 > 9 || pl.last < 11 || pl.last > 13) return SBA_BAND;
+  const Launchers k = launchers(L, sqr, pl);
+  const first_fn f1 = sba_r3_first(ell1);
+  const inv3_fn i1 = sba_r3_inv(ell1);
+  const uint32_t M = 1u << L;
+  const PrimeArgs &pa = c.pa;
+  const double KN4 = 4.0 * SBA_K * 3.0 * M * njobs;
+  const uint32_t ntile1 = M >> 11, ntile2 = M >> pl.last;
+  const uint32_t gy1 = pl.split ? SBA_K : 1;
+  if (!sqr) {
+    PROF("first_fwd", 8.0 * asum + KN4);
+    hipLaunchKernelGGL(f1, dim3(ntile1, gy1, njobs), dim3(NT), 0, st, pa, jr, 0, L);
+  }
+  {
+    PROF("first_fwd", 8.0 * bsum + KN4);
+    hipLaunchKernelGGL(f1, dim3(ntile1, gy1, njobs), dim3(NT), 0, st, pa, jr, 1, L);
+  }
+  {
+    PROF("fused_last", (sqr ? 2 : 3) * KN4);
+    if (sqr)
+      hipLaunchKernelGGL(k.fused, dim3(ntile2, SBA_K, 3 * njobs), dim3(NT), 0, st, pa, jr, L, 1);
+    else
+      hipLaunchKernelGGL(k.fused2, dim3(ntile2, SBA_K, 3 * njobs), dim3(NT), 0, st, pa, jr, L,
+                         1);
+  }
+  {
+    PROF("inv_first", 2 * KN4);
+    hipLaunchKernelGGL(i1, dim3(ntile1, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L);
+  }
+  return enqueue_garner(c, 3 * M, 3 * M, jr, njobs, nbmax, zsum, st);
 }
 
 // one transform of length <= 2^25 (batches use only this)
@@ -837,7 +913,16 @@ extern "C" size_t sba_gpu_workspace_bytes(int64_t an, int64_t bn) {
   const uint64_t mx = (uint64_t)(an > bn ? an : bn), mn = (uint64_t)(an < bn ? an : bn);
   const ChunkChoice cp = chunk_choice(mx, mn, 0);
   if (cp.n) return ws_chunked(cp.Lc, mx + mn);
+  const int L3 = r3_lg(mx + mn - 1);
+  if (L3) return ws_need((size_t)3 << L3);
   return ws_need((size_t)1 << lg_for(mx + mn - 1));
+}
+
+extern "C" int sba_gpu_radix3(int64_t an, int64_t bn, int sqr) {
+  if (!in_band(an, bn)) return 0;
+  const uint64_t mx = (uint64_t)(an > bn ? an : bn), mn = (uint64_t)(an < bn ? an : bn);
+  if (chunk_choice(mx, mn, sqr).n) return 0;
+  return r3_lg(mx + mn - 1);
 }
 
 static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64_t *bp,
@@ -848,7 +933,9 @@ static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64
     int64_t tn = an; an = bn; bn = tn;
   }
   const ChunkChoice cp = chunk_choice((uint64_t)an, (uint64_t)bn, sqr);
-  const int L = cp.n ? cp.Lc : lg_for((uint64_t)an + (uint64_t)bn - 1);
+  const int L3 = cp.n ? 0 : r3_lg((uint64_t)an + (uint64_t)bn - 1);
+  const int L = cp.n ? cp.Lc : (L3 ? L3 : lg_for((uint64_t)an + (uint64_t)bn - 1));
+  const size_t Ntot = L3 ? (size_t)3 << L3 : (size_t)1 << L;
   hipStream_t st = (hipStream_t)stream;
   std::lock_guard<std::mutex> lk(g_mu);
   DevCtx *c;
@@ -856,8 +943,7 @@ static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64
   if (rc) return rc;
   rc = ctx_twiddles(*c, L, st);
   if (rc) return rc;
-  const size_t need = cp.n ? ws_chunked(cp.Lc, (uint64_t)an + (uint64_t)bn)
-                           : ws_need((size_t)1 << L);
+  const size_t need = cp.n ? ws_chunked(cp.Lc, (uint64_t)an + (uint64_t)bn) : ws_need(Ntot);
   if (!ws) {
     if (c->ws_bytes < need) {
       if (c->ws) HIPCHK(hipFree(c->ws));
@@ -883,7 +969,10 @@ static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64
   j.zlen = (uint32_t)(an + bn - 1);
   j.Z = (uint32_t)(an + bn);
   j.nb = carry_blocks(j.Z);
-  job_layout(j, ws, 1u << L);
+  job_layout(j, ws, (uint32_t)Ntot);
+  if (L3)
+    return enqueue_r3(*c, L3, sqr, r3_plan(L3, sqr, 1), jr, 1, j.nb, (double)an, (double)bn,
+                      (double)j.Z, st);
   return enqueue_class(*c, L, sqr, plan_for(L, sqr, 1), jr, 1, j.nb, (double)an, (double)bn,
                        (double)j.Z, st);
 }

@@ -12,3 +12,23 @@
 Launchers SBA_CAT(sba_launchers_bm, SBA_TU_BM)(int L, int sqr, int last) {
   return launchers_bm<SBA_TU_BM>(L, sqr, last);
 }
+
+#if SBA_TU_BM == 5
+#define SBA_R3_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9)
+first_fn sba_r3_first(int ell) {
+  switch (ell) {
+#define SBA_C(E) case E: return k_first_fwd3<E, 11 - E>;
+    SBA_R3_CASES(SBA_C)
+#undef SBA_C
+  }
+  return nullptr;
+}
+inv3_fn sba_r3_inv(int ell) {
+  switch (ell) {
+#define SBA_C(E) case E: return k_inv_first3<E, 11 - E>;
+    SBA_R3_CASES(SBA_C)
+#undef SBA_C
+  }
+  return nullptr;
+}
+#endif

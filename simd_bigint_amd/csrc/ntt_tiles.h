@@ -12,6 +12,9 @@
 // ---------------------------------------------------------------------------
 struct PrimeArgs {
   uint32_t p[SBA_K], pinv[SBA_K], r2[SBA_K];
+  // length-3 DFT constants (Montgomery, canonical): h1 = -1/2,
+  // h2 = (w - w^2)/2 for the cube root w, h2i = -h2 (the inverse's)
+  uint32_t h1[SBA_K], h2[SBA_K], h2i[SBA_K];
   const uint32_t *wn[SBA_K];
   const uint32_t *wni[SBA_K];
 };
@@ -35,6 +38,11 @@ struct GarnerArgs {
 // 1: the first forward pass runs one prime per block (grid y = prime)
 #ifndef SBA_FIRST_SPLIT
 #define SBA_FIRST_SPLIT 1
+#endif
+// 1: a multiply's two last forward passes and the pointwise product run as
+// one kernel (k_fused2); 0: a's last pass is a separate kernel
+#ifndef SBA_MERGE_LAST
+#define SBA_MERGE_LAST 1
 #endif
 // 1: widest chunks even where group addressing is not linear
 #ifndef SBA_CHUNK_GREEDY
@@ -234,10 +242,10 @@ template <typename T> __device__ __forceinline__ T *uni_ptr(T *p) {
 __device__ __forceinline__ uint32_t uni_u32(uint32_t x) {
   return __builtin_amdgcn_readfirstlane(x);
 }
-__device__ __forceinline__ Job get_job(const JobRef &jr) {
+__device__ __forceinline__ Job get_job(const JobRef &jr, uint32_t idx) {
   Job j;
   if (jr.arr) {
-    j = jr.arr[blockIdx.z];
+    j = jr.arr[idx];
   } else {
     j = jr.one;
   }
@@ -248,6 +256,7 @@ __device__ __forceinline__ Job get_job(const JobRef &jr) {
   j.nb = uni_u32(j.nb);
   return j;
 }
+__device__ __forceinline__ Job get_job(const JobRef &jr) { return get_job(jr, blockIdx.z); }
 
 // ---------------------------------------------------------------------------
 // pass kernels
@@ -268,19 +277,19 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd(PrimeArgs pa, JobRe
   const uint32_t nlimbs = which ? jb.bn : jb.an;
   const uint32_t N = 1u << L;             // N <= 2^25: 32-bit offsets throughout
   const uint32_t S = 1u << (L - ELL);
+  // grid y = K: one prime per block; grid y = 1: all primes per block
+  const bool split = gridDim.y > 1;
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t q0 = split ? blockIdx.y : 0;
   const uint32_t col0 = tile << LOGTC;
   const uint32_t nl = nlimbs < N ? nlimbs : N;
   uint32_t *lt = lds + lpadt<LOGTC>(threadIdx.x);
   // element e = tid + i*NT sits at col0 + toff + uoff(i): a per-thread part
   // and a uniform part (scalar base + 32-bit lane offset addressing)
   const uint32_t toff = (threadIdx.x >> LOGTC) * S + (threadIdx.x & (TC - 1));
-  // grid y = K: one prime per block; grid y = 1: all primes per block
-  const bool split = gridDim.y > 1;
-  const int q0 = split ? (int)blockIdx.y : 0;
-  const int q1 = split ? q0 + 1 : SBA_K;
+  const uint32_t q1 = split ? q0 + 1 : SBA_K;
 #pragma unroll 1
-  for (int q = q0; q < q1; ++q) {
+  for (uint32_t q = q0; q < q1; ++q) {
     const uint32_t p = pa.p[q], pinv = pa.pinv[q], r2 = pa.r2[q];
     __syncthreads();
     // the tile's limbs are re-read per prime (cache-resident after the first)
@@ -359,20 +368,25 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_pass(PrimeArgs pa, JobRef jr,
 // Last forward levels of b + pointwise product with a's spectrum (or b's own
 // square) + first inverse levels, on contiguous tiles (S = 1, TC = 1).
 // LIMB_IN: the pass is the whole transform (s0 = 0) and reads b's limbs.
+// r3: the class has a radix-3 outer level (length 3 * 2^L): grid z runs over
+// (job, row) and the row's length-2^L transform sits at row * 2^L inside
+// each prime's 3 * 2^L residues (LIMB_IN is never launched with r3).
 template <int ELL, bool LIMB_IN, bool SQR, int BMAX>
-__global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr, int L) {
+__global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr, int L, int r3) {
   constexpr int NE = 1 << ELL;
   constexpr int E = NE / NT;
   __shared__ uint32_t lds[Pad<0>::words(NE)];
-  const Job jb = get_job(jr);
+  const uint32_t z = blockIdx.z;
+  const Job jb = get_job(jr, r3 ? z / 3 : z);
   const int q = blockIdx.y;
   const uint32_t N = 1u << L;
+  const uint32_t pstride = r3 ? 3 * N : N;
   const uint32_t H = blockIdx.x;
-  const uint32_t base = H << ELL;
+  const uint32_t base = (r3 ? (z % 3) * N : 0) + (H << ELL);
   const uint32_t nb = jb.bn < N ? jb.bn : N;
   const uint32_t p = pa.p[q], pinv = pa.pinv[q];
-  g_u32 *buf = (g_u32 *)jb.bspec + q * N + base;
-  gc_u32 *as = (gc_u32 *)jb.aspec + q * N + base;
+  g_u32 *buf = (g_u32 *)jb.bspec + q * pstride + base;
+  gc_u32 *as = (gc_u32 *)jb.aspec + q * pstride + base;
   gc_u64 *bl = (gc_u64 *)jb.b + base;
   uint32_t *lt = lds + lpadt<0>(threadIdx.x);
   const uint32_t tid = threadIdx.x;
@@ -400,12 +414,202 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
   for (int i = 0; i < E; ++i) (buf + i * NT)[tid] = lt[i * Pad<0>::words(NT)];
 }
 
+// Multiply, two-pass plans: the last forward levels of a AND of b on the same
+// tile (one set of twiddles), the pointwise product and the first inverse
+// levels. a's transformed tile is kept in registers (canonical, the
+// Montgomery operand) while b's is transformed in the same LDS.
+template <int ELL, int BMAX>
+__global__ __launch_bounds__(NT, SBA_WAVES) void k_fused2(PrimeArgs pa, JobRef jr, int L, int r3) {
+  constexpr int NE = 1 << ELL;
+  constexpr int E = NE / NT;
+  __shared__ uint32_t lds[Pad<0>::words(NE)];
+  const uint32_t z = blockIdx.z;
+  const Job jb = get_job(jr, r3 ? z / 3 : z);
+  const int q = blockIdx.y;
+  const uint32_t N = 1u << L;
+  const uint32_t pstride = r3 ? 3 * N : N;
+  const uint32_t H = blockIdx.x;
+  const uint32_t base = (r3 ? (z % 3) * N : 0) + (H << ELL);
+  const uint32_t p = pa.p[q], pinv = pa.pinv[q];
+  g_u32 *buf = (g_u32 *)jb.bspec + q * pstride + base;
+  gc_u32 *as = (gc_u32 *)jb.aspec + q * pstride + base;
+  uint32_t *lt = lds + lpadt<0>(threadIdx.x);
+  const uint32_t tid = threadIdx.x;
+  uint32_t av[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) lt[i * Pad<0>::words(NT)] = (as + i * NT)[tid];
+  tile_fwd<ELL, 0, BMAX, 0>(lds, pa.wn[q], H, p, pinv);
+  __syncthreads();
+  // each thread takes back only its own elements, then refills them with b
+#pragma unroll
+  for (int i = 0; i < E; ++i) av[i] = red1(lt[i * Pad<0>::words(NT)], p);
+#pragma unroll
+  for (int i = 0; i < E; ++i) lt[i * Pad<0>::words(NT)] = (buf + i * NT)[tid];
+  tile_fwd<ELL, 0, BMAX, 0>(lds, pa.wn[q], H, p, pinv);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < E; ++i)
+    lt[i * Pad<0>::words(NT)] = mont_mul(lt[i * Pad<0>::words(NT)], av[i], p, pinv);
+  tile_inv<ELL, 0, BMAX, 0>(lds, pa.wni[q], H, p, pinv);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < E; ++i) (buf + i * NT)[tid] = lt[i * Pad<0>::words(NT)];
+}
+
+// ---------------------------------------------------------------------------
+// radix-3 outer level: transform length 3M, M = 2^L, by the Good-Thomas
+// index map. Coefficient i sits in row i mod 3, column i mod M of a 3 x M
+// array (a ring isomorphism Z[x]/(x^3M - 1) = Z[u,v]/(u^3 - 1, v^M - 1)
+// since gcd(3, M) = 1), so a length-3 DFT down every column and three
+// independent length-M transforms along the rows need no twiddles between
+// the two factors. Column s holds limbs s, s + M, s + 2M; limb s + tM lands
+// in row (s + t*mu) mod 3, mu = M mod 3. Spectra keep the layout
+// [prime][row][M]; the inverse writes coefficients back in natural order,
+// and the factor 3 M is removed in the Garner stage.
+// ---------------------------------------------------------------------------
+// length-3 DFT of canonical (a, b, c) with h1 = -1/2 and h2 = (w - w^2)/2:
+//   z0 = a + b + c, z1 = a + w b + w^2 c, z2 = a + w^2 b + w c
+// (w + w^2 = -1), outputs lazy in [0, 2p)
+__device__ __forceinline__ void dft3(uint32_t a, uint32_t b, uint32_t c, uint32_t &z0,
+                                     uint32_t &z1, uint32_t &z2, uint32_t h1, uint32_t h2,
+                                     uint32_t p, uint32_t pinv) {
+  const uint32_t u = b + c;                                   // [0, 2p)
+  const uint32_t v = b + p - c;                               // (0, 2p)
+  z0 = a + red1(u, p);
+  const uint32_t A = red1(a + red1(mont_mul(u, h1, p, pinv), p), p);
+  const uint32_t B = red1(mont_mul(v, h2, p, pinv), p);
+  z1 = A + B;
+  z2 = A + p - B;
+}
+
+// row of limb t of column s is (s + t mu) mod 3; inversely, row r holds limb
+// t(r) = ((r - s) mu) mod 3 (mu^2 = 1 mod 3)
+__device__ __forceinline__ uint32_t r3_pick(uint32_t r, uint32_t sm, uint32_t mu,
+                                            uint32_t x0, uint32_t x1, uint32_t x2) {
+  const uint32_t t = ((r + 3 - sm) * mu) % 3;
+  return t == 0 ? x0 : (t == 1 ? x1 : x2);
+}
+
+// First forward pass of a radix-3 class: limbs -> residues -> column DFT-3
+// -> the first ELL levels of all three rows. Tiles: 2^ELL rows x 2^LOGTC
+// columns of each of the three length-M rows (three LDS tiles).
+template <int ELL, int LOGTC>
+__global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd3(PrimeArgs pa, JobRef jr, int which,
+                                                              int L) {
+  constexpr int TC = 1 << LOGTC;
+  constexpr int NE = 1 << (ELL + LOGTC);
+  constexpr int E = NE / NT;
+  constexpr int W = Pad<LOGTC>::words(NE);
+  __shared__ uint32_t lds[3 * W];
+  const Job jb = get_job(jr);
+  gc_u64 *limbs = (gc_u64 *)(which ? jb.b : jb.a);
+  g_u32 *spec = (g_u32 *)(which ? jb.bspec : jb.aspec);
+  const uint32_t nlimbs = which ? jb.bn : jb.an;
+  const uint32_t M = 1u << L;
+  const uint32_t S = 1u << (L - ELL);
+  const uint32_t mu = (L & 1) ? 2u : 1u;
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t col0 = tile << LOGTC;
+  const uint32_t nl = nlimbs < 3 * M ? nlimbs : 3 * M;
+  uint32_t *lt = lds + lpadt<LOGTC>(threadIdx.x);
+  const uint32_t toff = (threadIdx.x >> LOGTC) * S + (threadIdx.x & (TC - 1));
+  const bool split = gridDim.y > 1;
+  const uint32_t q0 = split ? blockIdx.y : 0;
+  const uint32_t q1 = split ? q0 + 1 : SBA_K;
+#pragma unroll 1
+  for (uint32_t q = q0; q < q1; ++q) {
+    const uint32_t p = pa.p[q], pinv = pa.pinv[q], r2 = pa.r2[q];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+      const uint32_t s = uoff + toff;
+      uint32_t x[3];
+#pragma unroll
+      for (int t = 0; t < 3; ++t) {
+        const uint32_t g = s + (uint32_t)t * M;
+        x[t] = red1(limb_residue(g < nl ? (limbs + uoff + (uint32_t)t * M)[toff] : 0, p, pinv, r2), p);
+      }
+      const uint32_t sm = s % 3;
+      uint32_t y0, y1, y2;
+      dft3(r3_pick(0, sm, mu, x[0], x[1], x[2]), r3_pick(1, sm, mu, x[0], x[1], x[2]),
+           r3_pick(2, sm, mu, x[0], x[1], x[2]), y0, y1, y2, pa.h1[q], pa.h2[q], p, pinv);
+      lt[i * Pad<LOGTC>::words(NT)] = y0;
+      lt[W + i * Pad<LOGTC>::words(NT)] = y1;
+      lt[2 * W + i * Pad<LOGTC>::words(NT)] = y2;
+    }
+#pragma unroll 1
+    for (int k = 0; k < 3; ++k) tile_fwd<ELL, LOGTC, 5, 0>(lds + k * W, pa.wn[q], 0, p, pinv);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      g_u32 *out = spec + q * 3 * M + (uint32_t)k * M;
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+        (out + uoff)[toff] = lt[k * W + i * Pad<LOGTC>::words(NT)];
+      }
+    }
+  }
+}
+
+// Last inverse pass of a radix-3 class (inverse levels [0, ELL) of the three
+// rows of prime blockIdx.y on the same column tile), then the inverse column
+// DFT-3 (root w^2, unscaled) and the map back to natural order, in place.
+template <int ELL, int LOGTC>
+__global__ __launch_bounds__(NT, SBA_WAVES) void k_inv_first3(PrimeArgs pa, JobRef jr, int L) {
+  constexpr int TC = 1 << LOGTC;
+  constexpr int NE = 1 << (ELL + LOGTC);
+  constexpr int E = NE / NT;
+  constexpr int W = Pad<LOGTC>::words(NE);
+  __shared__ uint32_t lds[3 * W];
+  const Job jb = get_job(jr);
+  const int q = blockIdx.y;
+  const uint32_t M = 1u << L;
+  const uint32_t S = 1u << (L - ELL);
+  const uint32_t mu = (L & 1) ? 2u : 1u;
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t col0 = tile << LOGTC;
+  const uint32_t p = pa.p[q], pinv = pa.pinv[q];
+  g_u32 *buf = (g_u32 *)jb.bspec + q * 3 * M;
+  uint32_t *lt = lds + lpadt<LOGTC>(threadIdx.x);
+  const uint32_t toff = (threadIdx.x >> LOGTC) * S + (threadIdx.x & (TC - 1));
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+      lt[k * W + i * Pad<LOGTC>::words(NT)] = (buf + (uint32_t)k * M + uoff)[toff];
+    }
+#pragma unroll 1
+  for (int k = 0; k < 3; ++k) tile_inv<ELL, LOGTC, 5, 0>(lds + k * W, pa.wni[q], 0, p, pinv);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+    const uint32_t s = uoff + toff;
+    uint32_t r0, r1, r2;
+    dft3(red1(lt[i * Pad<LOGTC>::words(NT)], p), red1(lt[W + i * Pad<LOGTC>::words(NT)], p),
+         red1(lt[2 * W + i * Pad<LOGTC>::words(NT)], p), r0, r1, r2, pa.h1[q], pa.h2i[q], p,
+         pinv);
+    // row r holds coefficient s + t(r) M: write limb position t from row
+    // r(t) = (s + t mu) mod 3
+    const uint32_t sm = s % 3;
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      const uint32_t r = (sm + (uint32_t)t * mu) % 3;
+      const uint32_t v = r == 0 ? r0 : (r == 1 ? r1 : r2);
+      (buf + (uint32_t)t * M + uoff)[toff] = v;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // launch tables
 // ---------------------------------------------------------------------------
 typedef void (*first_fn)(PrimeArgs, JobRef, int, int);
 typedef void (*pass_fn)(PrimeArgs, JobRef, int, int, int, AccArgs);
-typedef void (*fused_fn)(PrimeArgs, JobRef, int);
+typedef void (*fused_fn)(PrimeArgs, JobRef, int, int);
 struct Plan { int last, bm, split; };
 
 #define SBA_ELL_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
@@ -453,6 +657,14 @@ template <int BM> struct KTab {
     }
     return nullptr;
   }
+  static fused_fn fused2_last(int last) {
+    switch (last) {
+      case 11: return k_fused2<11, BM>;
+      case 12: return k_fused2<12, BM>;
+      case 13: return k_fused2<13, BM>;
+    }
+    return nullptr;
+  }
   static fused_fn fused_last(int last, int sqr) {
     switch (last) {
       case 11: return sqr ? k_fused<11, false, true, BM> : k_fused<11, false, false, BM>;
@@ -466,7 +678,7 @@ template <int BM> struct KTab {
 struct Launchers {
   first_fn first;
   pass_fn fwd_last, inv_first;
-  fused_fn fused;
+  fused_fn fused, fused2;          // fused2: both operands' last pass (multiplies)
 };
 template <int BM> static Launchers launchers_bm(int L, int sqr, int last) {
   Launchers k = {};
@@ -477,10 +689,16 @@ template <int BM> static Launchers launchers_bm(int L, int sqr, int last) {
     k.first = KTab<BM>::first_two(L - last);
     k.fwd_last = KTab<BM>::fwd_last(last);
     k.fused = KTab<BM>::fused_last(last, sqr);
+    k.fused2 = sqr ? nullptr : KTab<BM>::fused2_last(last);
     k.inv_first = KTab<BM>::inv_two(L - last);
   }
   return k;
 }
+// radix-3 first / last column passes: 2^11 residues of each row per tile,
+// ELL = L - last first-pass levels (defined in the radix-32 object)
+typedef void (*inv3_fn)(PrimeArgs, JobRef, int);
+first_fn sba_r3_first(int ell);
+inv3_fn sba_r3_inv(int ell);
 // defined in ntt_pass_bm.hip, one object per radix
 Launchers sba_launchers_bm3(int L, int sqr, int last);
 Launchers sba_launchers_bm4(int L, int sqr, int last);
