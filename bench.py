@@ -318,6 +318,23 @@ def configs_leg(L, reps=10):
         "nn": e["nn"], "dn": e["dn"], "ms": round(best, 3),
         "bit_exact": bool(sha(q) == e["q_sha256"] and sha(r) == e["r_sha256"]),
         "note": "host buffers; includes the PCIe copies of every multiply"}
+    # the reference's divrem_u64_dc on one host core, same operands (once)
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libsbref.so")
+    need = {"avx512f", "avx512bw", "avx512vl", "avx512dq", "avx512ifma", "avx512vbmi",
+            "avx512_vbmi2", "avx512_vpopcntdq", "avx512_bitalg", "bmi2", "adx"}
+    if os.path.exists(ref_so) and need <= cpu_flags():
+        u64p = np.ctypeslib.ndpointer(np.uint64, flags="C")
+        fr = ctypes.CDLL(ref_so).sbref_divrem_u64_dc
+        fr.argtypes = [u64p, u64p, u64p, ctypes.c_uint64, u64p, ctypes.c_uint64,
+                       ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
+        fr.restype = None
+        qq = np.zeros(e["nn"] - e["dn"] + 1, np.uint64)
+        rr = np.zeros(e["dn"], np.uint64)
+        qn_, rn_ = ctypes.c_uint64(), ctypes.c_uint64()
+        t0 = time.perf_counter()
+        fr(qq, rr, nn_, e["nn"], dd_, e["dn"], ctypes.byref(qn_), ctypes.byref(rn_))
+        out["cfg3_divrem_2^21/2^20"]["reference_cpu_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        out["cfg3_divrem_2^21/2^20"]["reference_cpu"] = "divrem_u64_dc compiled from source, 1 core"
     log("bench: divrem", out["cfg3_divrem_2^21/2^20"])
     # the metric multiply through the host entry nt_mul (H2D + D2H inside)
     st = [42]
@@ -336,6 +353,40 @@ def configs_leg(L, reps=10):
                                      "note": "PCIe-inclusive: 64 MiB in, 64 MiB out"}
     log("bench: host nt_mul", out["metric_via_host_nt_mul"])
     return out
+
+
+def profile_shape(args, gold, L):
+    """Time `steps` multiplies of one golden shape (operands resident in HBM)
+    for a profiler run; prints {"shape", "ms", "bit_exact"} only."""
+    import numpy as np
+    import torch
+    from simd_bigint_amd.workloads import pattern_fill
+    e = [x for x in gold["large_mul"] if x["name"] == args.shape][0]
+    st = [e["seed"]]
+    a = pattern_fill(e["an"], e["pat"], st)
+    ta = torch.from_numpy(a.view(np.int64)).cuda()
+    if e["sqr"]:
+        tr = torch.empty(2 * e["an"], dtype=torch.int64, device="cuda")
+        f = lambda: L.sba_gpu_sqr_dev(tr.data_ptr(), ta.data_ptr(), e["an"], None)
+    else:
+        b = pattern_fill(e["bn"], e["pat"], st)
+        tb = torch.from_numpy(b.view(np.int64)).cuda()
+        tr = torch.empty(e["an"] + e["bn"], dtype=torch.int64, device="cuda")
+        f = lambda: L.sba_gpu_mul_dev(tr.data_ptr(), ta.data_ptr(), e["an"], tb.data_ptr(),
+                                      e["bn"], None)
+    for _ in range(args.warmup):
+        f()
+    torch.cuda.synchronize()
+    exact = hashlib.sha256(tr.cpu().numpy().tobytes()).hexdigest() == e["sha256"]
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        f()
+    e1.record()
+    torch.cuda.synchronize()
+    print(json.dumps({"shape": args.shape, "an": e["an"], "bn": e["bn"],
+                      "ms": round(e0.elapsed_time(e1) / args.steps, 4), "bit_exact": exact}))
 
 
 def calibrate_copy():
@@ -384,6 +435,9 @@ def main():
                     help="Poisson arrival rate of the serving run, requests/s")
     ap.add_argument("--cpu-budget", type=float, default=12.0,
                     help="seconds of CPU work for the cpu_baseline sample (0 = skip)")
+    ap.add_argument("--shape", default="metric_4Mi",
+                    help="profiling aid: time another golden multiply shape (e.g. "
+                         "cfg2_3x2^20) instead of the metric; prints only that shape's time")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the 1 GiB elementwise kernel used to scale HBM counters")
     args = ap.parse_args()
@@ -406,6 +460,9 @@ def main():
                                       ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_uint64)]
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
+    if args.shape != "metric_4Mi":
+        return profile_shape(args, gold, L)
     n = 1 << 22
     st = [42]
     a = xorshift_fill(n, st)
@@ -427,7 +484,6 @@ def main():
         step()
     torch.cuda.synchronize()
     prod = tr.cpu().numpy().view(np.uint64)
-    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
     want = [e for e in gold["large_mul"] if e["name"] == "metric_4Mi"][0]["sha256"]
     bit_exact = hashlib.sha256(prod.tobytes()).hexdigest() == want
     log("bench: warmup done, bit_exact =", bit_exact)

@@ -834,8 +834,12 @@ static int r3_lg(uint64_t zlen) {
 
 // the power-of-two plan of the same row length, with the first pass kept
 // within 1..9 levels (the radix-3 column tiles hold 2^11 residues per row)
+#ifndef SBA_R3_WIDE_LAST
+#define SBA_R3_WIDE_LAST 0
+#endif
 static Plan r3_plan(int L, int sqr, uint32_t njobs) {
   Plan p = plan_for(L, sqr, njobs);
+  if (SBA_R3_WIDE_LAST) p.last = 13;
   if (p.last >= L) p.last = L - 1;
   if (p.last > 13) p.last = 13;
   if (L - p.last > 9) p.last = L - 9;

@@ -44,6 +44,10 @@ struct GarnerArgs {
 #ifndef SBA_MERGE_LAST
 #define SBA_MERGE_LAST 1
 #endif
+// occupancy hint of the merged last pass (waves per SIMD)
+#ifndef SBA_FUSED2_WAVES
+#define SBA_FUSED2_WAVES 1
+#endif
 // 1: widest chunks even where group addressing is not linear
 #ifndef SBA_CHUNK_GREEDY
 #define SBA_CHUNK_GREEDY 0
@@ -419,7 +423,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
 // levels. a's transformed tile is kept in registers (canonical, the
 // Montgomery operand) while b's is transformed in the same LDS.
 template <int ELL, int BMAX>
-__global__ __launch_bounds__(NT, SBA_WAVES) void k_fused2(PrimeArgs pa, JobRef jr, int L, int r3) {
+__global__ __launch_bounds__(NT, SBA_FUSED2_WAVES) void k_fused2(PrimeArgs pa, JobRef jr, int L, int r3) {
   constexpr int NE = 1 << ELL;
   constexpr int E = NE / NT;
   __shared__ uint32_t lds[Pad<0>::words(NE)];
