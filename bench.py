@@ -142,10 +142,12 @@ def batch_leg(njobs, L):
             (njobs, label, ms, ncls))
         return ms, ncls
 
-    # the hand-written rule for every class of the batch
-    for L_, sq in keys:
+    # the hand-written rule for every power-of-two class of the batch
+    for L_, kind in keys:
+        if kind & 2:
+            continue
         for bk in range(sba.lib().sba_plan_bucket(1 << 30) + 1):
-            tuning.plan_set(L_, sq, bk, tuning.plan_heuristic(L_))
+            tuning.plan_set(L_, kind, bk, tuning.plan_heuristic(L_))
     timed("heuristic")
     tuning.plan_reset()
     timed("shipped")

@@ -175,24 +175,27 @@ int sba_gpu_mul_loop(const sba_job *jobs, int njobs, void *stream);
 /* measured on MI355X, the hand-written rule (sba_plan_heuristic: last =   */
 /* min(L, 13), bm 5, split 1) where a key was not measured. sba_plan_set   */
 /* returns SBA_BAND for an invalid plan.                                   */
+/* kind = (square ? 1 : 0) | (radix-3 ? 2 : 0). A radix-3 class has length */
+/* 3 * 2^L (L = 13..22; its plans: last 11..13, 1..9 first-pass levels);    */
+/* its shipped plans are the power-of-two ones moved into those limits.    */
 /* ---------------------------------------------------------------------- */
 #define SBA_PLAN_BUCKETS 6
 int sba_plan_bucket(int njobs);    /* 1-3 jobs: 0, 4-15: 1, ... >= 1024: 5 */
-int sba_plan_set(int L, int sqr, int bucket, int last, int bm, int split);
-int sba_plan_get(int L, int sqr, int bucket, int *last, int *bm, int *split);
+int sba_plan_set(int L, int kind, int bucket, int last, int bm, int split);
+int sba_plan_get(int L, int kind, int bucket, int *last, int *bm, int *split);
 int sba_plan_heuristic(int L, int *last, int *bm, int *split);
 void sba_plan_reset(void);
-/* valid plans for length 2^L as (last, bm, split) triples; returns the
+/* valid plans of class (L, kind) as (last, bm, split) triples; returns the
  * count (writes at most max triples) */
-int sba_plan_candidates(int L, int *plans, int max);
+int sba_plan_candidates(int L, int kind, int *plans, int max);
 /* time one size class (all jobs the same L and kind) under a plan: one
  * warm-up, then reps launches between HIP events on the stream; *ms = mean
  * per launch. Workspace as sba_batch_launch_ws. Synchronizes the stream. */
 int sba_class_time(const sba_job *jobs, int njobs, int last, int bm, int split, int reps,
                    void *ws, size_t ws_bytes, void *stream, float *ms);
-/* the size class of a job: *L = log2 of its transform length, *sqr = 1 for
- * a square; SBA_BAND outside the band */
-int sba_job_class(const sba_job *j, int *L, int *sqr);
+/* the size class of a batch job: its kind (as above) and L (transform
+ * length 2^L, or 3 * 2^L for radix-3); SBA_BAND outside the batch band */
+int sba_job_class(const sba_job *j, int *L, int *kind);
 
 /* Autotuner: for every size class of the jobs, time candidate plans on the
  * class's own jobs (reps launches each, products written to the jobs' rp)
@@ -201,7 +204,9 @@ int sba_job_class(const sba_job *j, int *L, int *sqr);
  *   SBA_TUNE_EXHAUSTIVE  every valid plan is timed
  *   SBA_TUNE_MODEL       the loaded cost model ranks the plans; the topk
  *                        best-ranked are timed (topk = 1: none is, the
- *                        model's first choice is stored)
+ *                        model's first choice is stored). The model covers
+ *                        power-of-two classes; radix-3 classes keep their
+ *                        plan in this mode.
  * Workspace as sba_batch_launch_ws. *measured = plan timings made.
  * SBA_ERR_WORKSPACE: workspace too small, or no model loaded for MODEL. */
 #define SBA_TUNE_DEFAULT 0

@@ -115,7 +115,7 @@ def lib() -> ctypes.CDLL:
     L.sba_plan_heuristic.restype = ctypes.c_int
     L.sba_plan_reset.argtypes = []
     L.sba_plan_reset.restype = None
-    L.sba_plan_candidates.argtypes = [ctypes.c_int, ip, ctypes.c_int]
+    L.sba_plan_candidates.argtypes = [ctypes.c_int, ctypes.c_int, ip, ctypes.c_int]
     L.sba_plan_candidates.restype = ctypes.c_int
     L.sba_class_time.argtypes = [vp, ctypes.c_int] + [ctypes.c_int] * 4 + [vp, ctypes.c_size_t,
                                                                             vp, fp]

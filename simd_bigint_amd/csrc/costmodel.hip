@@ -320,23 +320,29 @@ extern "C" int sba_autotune(const sba_job *jobs, int njobs, int mode, int topk, 
   if (!ws || ws_bytes < need) return SBA_ERR_WORKSPACE;
   if (mode == SBA_TUNE_MODEL && (!g_cm.loaded || topk < 1)) return SBA_ERR_WORKSPACE;
   const int nlg = sba_gpu_max_lg() + 1;
-  std::vector<std::vector<sba_job>> cls(2 * (size_t)nlg);
+  // classes by (kind = square? | radix-3? << 1, L)
+  std::vector<std::vector<sba_job>> cls(4 * (size_t)nlg);
   for (int i = 0; i < njobs; ++i) {
-    int L, sq;
-    if (sba_job_class(&jobs[i], &L, &sq) != SBA_OK) return SBA_BAND;
-    cls[(size_t)sq * nlg + L].push_back(jobs[i]);
+    int L, kind;
+    if (sba_job_class(&jobs[i], &L, &kind) != SBA_OK) return SBA_BAND;
+    cls[(size_t)kind * nlg + L].push_back(jobs[i]);
   }
   int plans[3 * CM_MAXCAND];
-  for (int sq = 0; sq < 2; ++sq)
+  for (int kind = 0; kind < 4; ++kind)
     for (int L = 0; L < nlg; ++L) {
-      const std::vector<sba_job> &cj = cls[(size_t)sq * nlg + L];
+      const std::vector<sba_job> &cj = cls[(size_t)kind * nlg + L];
       if (cj.empty()) continue;
+      const int sq = kind & 1, r3 = (kind >> 1) & 1;
+      // the cost model describes power-of-two classes; radix-3 classes keep
+      // their plan in model mode and are timed in exhaustive mode
+      if (r3 && mode == SBA_TUNE_MODEL) continue;
       const int n = (int)cj.size();
       double zs = 0;
       for (const sba_job &j : cj) zs += (double)j.an + (double)(j.bp ? j.bn : j.an);
-      const double fill = zs / ((double)n * (double)(1u << L));
-      int nc = sba_plan_candidates(L, plans, CM_MAXCAND);
+      const double fill = zs / ((double)n * (double)((r3 ? 3u : 1u) << L));
+      int nc = sba_plan_candidates(L, kind, plans, CM_MAXCAND);
       if (nc > CM_MAXCAND) nc = CM_MAXCAND;
+      if (nc < 1) continue;
       std::vector<int> order(nc);
       for (int c = 0; c < nc; ++c) order[c] = c;
       int ntime = nc;
@@ -372,7 +378,7 @@ extern "C" int sba_autotune(const sba_job *jobs, int njobs, int mode, int topk, 
           }
         }
       }
-      const int rc = sba_plan_set(L, sq, sba_plan_bucket(n), plans[3 * best],
+      const int rc = sba_plan_set(L, kind, sba_plan_bucket(n), plans[3 * best],
                                   plans[3 * best + 1], plans[3 * best + 2]);
       if (rc != SBA_OK) return rc;
     }

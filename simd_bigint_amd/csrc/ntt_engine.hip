@@ -321,7 +321,7 @@ static int ctx_init(DevCtx &c) {
     for (int i = 0; i < 5; ++i) inv *= 2u - (uint32_t)p * inv;   // p^-1 mod 2^32
     c.pa.p[q] = (uint32_t)p;
     c.pa.pinv[q] = 0u - inv;
-    c.pa.r2[q] = (uint32_t)(((unsigned __int128)1 << 64) % p);
+    c.pa.r1[q] = (uint32_t)(((uint64_t)1 << 32) % p);
     c.ga_base.p[q] = (uint32_t)p;
     c.ga_base.pinv[q] = 0u - inv;
   }
@@ -468,7 +468,9 @@ static Launchers launchers(int L, int sqr, const Plan &p) {
 // MI355X; the rule where a key was not measured) until the autotuner or a
 // caller stores another choice.
 static Plan g_plan[2][SBA_MAX_LG + 1][SBA_PLAN_BUCKETS];
+static Plan g_plan3[2][SBA_MAX_LG + 1][SBA_PLAN_BUCKETS];   // radix-3 classes (row length 2^L)
 static int g_plan_init = 0;
+static Plan r3_fix(int L, Plan p);
 static Plan shipped_plan(int sq, int L, int k) {
   if (L >= 0 && L < SBA_PT_NL && k < SBA_PLAN_BUCKETS) {
     const signed char *e = kPlanTable[sq][L][k];
@@ -480,7 +482,10 @@ static Plan shipped_plan(int sq, int L, int k) {
 static void plan_defaults(void) {
   for (int sq = 0; sq < 2; ++sq)
     for (int L = 0; L <= SBA_MAX_LG; ++L)
-      for (int k = 0; k < SBA_PLAN_BUCKETS; ++k) g_plan[sq][L][k] = shipped_plan(sq, L, k);
+      for (int k = 0; k < SBA_PLAN_BUCKETS; ++k) {
+        g_plan[sq][L][k] = shipped_plan(sq, L, k);
+        g_plan3[sq][L][k] = r3_fix(L, g_plan[sq][L][k]);
+      }
   g_plan_init = 1;
 }
 static int plan_bucket(uint32_t njobs) {
@@ -495,6 +500,11 @@ static Plan plan_for(int L, int sqr, uint32_t njobs) {
   if (!g_plan_init) plan_defaults();
   if (L < 0 || L > SBA_MAX_LG) return heuristic_plan(L);
   return g_plan[sqr ? 1 : 0][L][plan_bucket(njobs)];
+}
+static Plan plan_for3(int L, int sqr, uint32_t njobs) {
+  if (!g_plan_init) plan_defaults();
+  if (L < 0 || L > SBA_MAX_LG) return r3_fix(L, heuristic_plan(L));
+  return g_plan3[sqr ? 1 : 0][L][plan_bucket(njobs)];
 }
 
 // ---------------------------------------------------------------------------
@@ -652,8 +662,9 @@ static int enqueue_garner(DevCtx &c, uint32_t Nt, uint32_t stride, const JobRef 
   for (int q = 0; q < SBA_K; ++q) {
     const uint64_t p = kPrimes[q];
     const uint64_t ninv = h_invm(Nt % p, p);
-    // scale = Nt^-1 * R^2 mod p
-    ga.scale[q] = (uint32_t)((unsigned __int128)ninv * (((unsigned __int128)1 << 64) % p) % p);
+    // scale = Nt^-1 * R^4 mod p: undoes Nt, the pointwise R^-1, each
+    // operand's R^-1 (limb residues) and the Garner product's own R^-1
+    ga.scale[q] = (uint32_t)((unsigned __int128)ninv * h_powm(2, 128, p) % p);
   }
   {
     PROF("garner_maps", 4.0 * SBA_K * zsum + 9.0 * zsum);
@@ -834,17 +845,28 @@ static int r3_lg(uint64_t zlen) {
 
 // the power-of-two plan of the same row length, with the first pass kept
 // within 1..9 levels (the radix-3 column tiles hold 2^11 residues per row)
+// 1: radix-3 classes default to the widest last pass (13 levels), which
+// keeps the first-pass column tiles widest (fixed by measurement)
 #ifndef SBA_R3_WIDE_LAST
-#define SBA_R3_WIDE_LAST 0
+#define SBA_R3_WIDE_LAST 1
 #endif
-static Plan r3_plan(int L, int sqr, uint32_t njobs) {
-  Plan p = plan_for(L, sqr, njobs);
+// a power-of-two plan moved into the radix-3 limits: last pass 11..13
+// levels, first pass 1..9 levels (its column tiles hold 2^11 residues per
+// row)
+static Plan r3_fix(int L, Plan p) {
   if (SBA_R3_WIDE_LAST) p.last = 13;
   if (p.last >= L) p.last = L - 1;
   if (p.last > 13) p.last = 13;
   if (L - p.last > 9) p.last = L - 9;
+  if (p.last < 11) p.last = 11;
   return p;
 }
+static bool plan_ok3(int L, const Plan &p) {
+  if (L < R3_MIN_L || L > R3_MAX_L || p.bm < 3 || p.bm > 5 || (p.split != 0 && p.split != 1))
+    return false;
+  return p.last >= 11 && p.last <= 13 && L - p.last >= 1 && L - p.last <= 9;
+}
+static Plan r3_plan(int L, int sqr, uint32_t njobs) { return plan_for3(L, sqr, njobs); }
 
 static int enqueue_r3(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef &jr,
                       uint32_t njobs, uint32_t nbmax, double asum, double bsum, double zsum,
@@ -1013,34 +1035,48 @@ struct ClassDesc {
   uint32_t n, nbmax;
   double as, bs, zs;
   Plan plan;
+  int r3;                          // radix-3 class: length 3 * 2^L
 };
+#define SBA_MAX_CLASSES (4 * (SBA_MAX_LG + 1))
+
+// class of a batch job: (L, radix-3?) of its transform (never chunked)
+static void job_key(const sba_job &j, int *L, int *r3) {
+  const uint64_t an = (uint64_t)j.an, bn = j.bp ? (uint64_t)j.bn : an;
+  const int L3 = r3_lg(an + bn - 1);
+  *r3 = L3 != 0;
+  *L = L3 ? L3 : lg_for(an + bn - 1);
+}
 
 static size_t job_ws(const sba_job &j) {
-  const uint64_t an = (uint64_t)j.an, bn = j.bp ? (uint64_t)j.bn : an;
-  return (ws_need((size_t)1 << lg_for(an + bn - 1)) + 255) & ~(size_t)255;
+  int L, r3;
+  job_key(j, &L, &r3);
+  return (ws_need((size_t)(r3 ? 3 : 1) << L) + 255) & ~(size_t)255;
 }
 static size_t table_bytes(int njobs) { return ((size_t)njobs * sizeof(Job) + 255) & ~(size_t)255; }
 
 // Orders the jobs by class into hj[], carving each job's buffers from ws in
-// that order (job_ws bytes each), and fills cls[] (at most 2*(SBA_MAX_LG+1)
-// entries). Every job must be in band. Returns the number of classes.
+// that order (job_ws bytes each), and fills cls[] (at most SBA_MAX_CLASSES
+// entries: square?, radix-3?, L). Every job must be in band. Returns the number of classes.
 static int classify_jobs(const sba_job *jobs, int njobs, char *ws, Job *hj, ClassDesc *cls) {
   int ncls = 0, pos = 0;
   size_t off = 0;
-  int count[2][SBA_MAX_LG + 1] = {};
+  int count[2][2][SBA_MAX_LG + 1] = {};
   for (int i = 0; i < njobs; ++i) {
-    const int sq = jobs[i].bp == nullptr;
-    const uint64_t an = (uint64_t)jobs[i].an, bn = sq ? an : (uint64_t)jobs[i].bn;
-    count[sq][lg_for(an + bn - 1)]++;
+    int L, r3;
+    job_key(jobs[i], &L, &r3);
+    count[jobs[i].bp == nullptr][r3][L]++;
   }
   for (int sq = 0; sq < 2; ++sq)
+    for (int r3 = 0; r3 < 2; ++r3)
     for (int L = 0; L <= SBA_MAX_LG; ++L) {
-      if (!count[sq][L]) continue;
-      ClassDesc k = {L, sq, pos, 0, 0, 0, 0, 0, heuristic_plan(L)};
+      if (!count[sq][r3][L]) continue;
+      ClassDesc k = {L, sq, pos, 0, 0, 0, 0, 0, heuristic_plan(L), r3};
       for (int i = 0; i < njobs; ++i) {
         const int isq = jobs[i].bp == nullptr;
         const uint64_t an = (uint64_t)jobs[i].an, bn = isq ? an : (uint64_t)jobs[i].bn;
-        if (isq != sq || lg_for(an + bn - 1) != L) continue;
+        int jL, jr3;
+        job_key(jobs[i], &jL, &jr3);
+        if (isq != sq || jr3 != r3 || jL != L) continue;
         Job &j = hj[pos++];
         // larger operand first (the first pass of b then covers the shorter one)
         const bool swap = !isq && an < bn;
@@ -1052,7 +1088,7 @@ static int classify_jobs(const sba_job *jobs, int njobs, char *ws, Job *hj, Clas
         j.zlen = (uint32_t)(an + bn - 1);
         j.Z = (uint32_t)(an + bn);
         j.nb = carry_blocks(j.Z);
-        job_layout(j, ws + off, 1u << L);
+        job_layout(j, ws + off, (r3 ? 3u : 1u) << L);
         off += job_ws(jobs[i]);
         k.n++;
         if (j.nb > k.nbmax) k.nbmax = j.nb;
@@ -1060,7 +1096,7 @@ static int classify_jobs(const sba_job *jobs, int njobs, char *ws, Job *hj, Clas
         k.bs += j.bn;
         k.zs += j.Z;
       }
-      k.plan = plan_for(L, sq, k.n);
+      k.plan = r3 ? plan_for3(L, sq, k.n) : plan_for(L, sq, k.n);
       cls[ncls++] = k;
     }
   return ncls;
@@ -1088,7 +1124,7 @@ struct sba_batch {
   void *ws;                        // all jobs' workspaces
   Job *djobs_all;
   int nclass;
-  ClassDesc cls[2 * (SBA_MAX_LG + 1)];
+  ClassDesc cls[SBA_MAX_CLASSES];
   hipStream_t cap_stream;          // private stream used for capture
   hipGraph_t graph;
   hipGraphExec_t exec;
@@ -1101,8 +1137,11 @@ static int enqueue_classes(DevCtx &c, Job *djobs, const ClassDesc *cls, int ncls
     JobRef jr;
     jr.arr = djobs + cls[k].start;
     memset(&jr.one, 0, sizeof jr.one);
-    const int rc = enqueue_class(c, cls[k].L, cls[k].sqr, cls[k].plan, jr, cls[k].n,
-                                 cls[k].nbmax, cls[k].as, cls[k].bs, cls[k].zs, st);
+    const int rc =
+        cls[k].r3 ? enqueue_r3(c, cls[k].L, cls[k].sqr, cls[k].plan, jr, cls[k].n, cls[k].nbmax,
+                               cls[k].as, cls[k].bs, cls[k].zs, st)
+                  : enqueue_class(c, cls[k].L, cls[k].sqr, cls[k].plan, jr, cls[k].n,
+                                  cls[k].nbmax, cls[k].as, cls[k].bs, cls[k].zs, st);
     if (rc != SBA_OK) return rc;
   }
   return SBA_OK;
@@ -1228,7 +1267,7 @@ extern "C" int sba_batch_launch_ws(const sba_job *jobs, int njobs, void *ws, siz
   if (rc) return rc;
   if ((rc = ctx_twiddles(*c, jobs_lmax(jobs, njobs), st)) != 0) return rc;
   Job *dj;
-  ClassDesc cls[2 * (SBA_MAX_LG + 1)];
+  ClassDesc cls[SBA_MAX_CLASSES];
   int ncls = 0;
   if ((rc = stage_jobs(jobs, njobs, ws, need, st, &dj, cls, &ncls)) != SBA_OK) return rc;
   return enqueue_classes(*c, dj, cls, ncls, st);
@@ -1237,31 +1276,37 @@ extern "C" int sba_batch_launch_ws(const sba_job *jobs, int njobs, void *ws, siz
 // ---------------------------------------------------------------------------
 // plan table access and plan timing (the autotuner's measurements)
 // ---------------------------------------------------------------------------
-extern "C" int sba_job_class(const sba_job *j, int *L, int *sqr) {
+extern "C" int sba_job_class(const sba_job *j, int *L, int *kind) {
   if (!j || !in_band_direct(j->an, j->bp ? j->bn : j->an)) return SBA_BAND;
-  const uint64_t an = (uint64_t)j->an, bn = j->bp ? (uint64_t)j->bn : an;
-  *L = lg_for(an + bn - 1);
-  *sqr = j->bp == nullptr;
+  int r3;
+  job_key(*j, L, &r3);
+  *kind = (j->bp == nullptr ? 1 : 0) | (r3 ? 2 : 0);
   return SBA_OK;
 }
 
 extern "C" int sba_plan_bucket(int njobs) { return plan_bucket(njobs < 1 ? 1u : (uint32_t)njobs); }
 
-extern "C" int sba_plan_set(int L, int sqr, int bucket, int last, int bm, int split) {
+static bool kind_plan_ok(int L, int kind, const Plan &p) {
+  return (kind & 2) ? plan_ok3(L, p) : plan_ok(L, p);
+}
+
+extern "C" int sba_plan_set(int L, int kind, int bucket, int last, int bm, int split) {
   Plan p = {last, bm, split};
-  if (L < 10 || L > SBA_MAX_LG || bucket < 0 || bucket >= SBA_PLAN_BUCKETS || !plan_ok(L, p))
+  if (L < 10 || L > SBA_MAX_LG || bucket < 0 || bucket >= SBA_PLAN_BUCKETS ||
+      !kind_plan_ok(L, kind, p))
     return SBA_BAND;
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g_plan_init) plan_defaults();
-  g_plan[sqr ? 1 : 0][L][bucket] = p;
+  ((kind & 2) ? g_plan3 : g_plan)[kind & 1][L][bucket] = p;
   return SBA_OK;
 }
 
-extern "C" int sba_plan_get(int L, int sqr, int bucket, int *last, int *bm, int *split) {
+extern "C" int sba_plan_get(int L, int kind, int bucket, int *last, int *bm, int *split) {
   if (L < 10 || L > SBA_MAX_LG || bucket < 0 || bucket >= SBA_PLAN_BUCKETS) return SBA_BAND;
+  if ((kind & 2) && (L < R3_MIN_L || L > R3_MAX_L)) return SBA_BAND;
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g_plan_init) plan_defaults();
-  const Plan &p = g_plan[sqr ? 1 : 0][L][bucket];
+  const Plan &p = ((kind & 2) ? g_plan3 : g_plan)[kind & 1][L][bucket];
   *last = p.last;
   *bm = p.bm;
   *split = p.split;
@@ -1282,13 +1327,13 @@ extern "C" int sba_plan_heuristic(int L, int *last, int *bm, int *split) {
   return SBA_OK;
 }
 
-extern "C" int sba_plan_candidates(int L, int *plans, int max) {
+extern "C" int sba_plan_candidates(int L, int kind, int *plans, int max) {
   int n = 0;
   for (int last = 10; last <= 13; ++last)
     for (int bm = 3; bm <= 5; ++bm)
       for (int split = 0; split <= 1; ++split) {
         const Plan p = {last, bm, split};
-        if (!plan_ok(L, p)) continue;
+        if (!kind_plan_ok(L, kind, p)) continue;
         if (plans && n < max) {
           plans[3 * n] = p.last;
           plans[3 * n + 1] = p.bm;
@@ -1311,12 +1356,12 @@ extern "C" int sba_class_time(const sba_job *jobs, int njobs, int last, int bm, 
   if (rc) return rc;
   if ((rc = ctx_twiddles(*c, jobs_lmax(jobs, njobs), st)) != 0) return rc;
   Job *dj;
-  ClassDesc cls[2 * (SBA_MAX_LG + 1)];
+  ClassDesc cls[SBA_MAX_CLASSES];
   int ncls = 0;
   if ((rc = stage_jobs(jobs, njobs, ws, need, st, &dj, cls, &ncls)) != SBA_OK) return rc;
   if (ncls != 1) return SBA_BAND;                 // one size class per measurement
   cls[0].plan = Plan{last, bm, split};
-  if (!plan_ok(cls[0].L, cls[0].plan)) return SBA_BAND;
+  if (!kind_plan_ok(cls[0].L, cls[0].r3 ? 2 : 0, cls[0].plan)) return SBA_BAND;
   const int saved = g_prof_on;
   g_prof_on = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;

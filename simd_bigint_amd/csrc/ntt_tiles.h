@@ -11,7 +11,7 @@
 // kernel argument block (by value)
 // ---------------------------------------------------------------------------
 struct PrimeArgs {
-  uint32_t p[SBA_K], pinv[SBA_K], r2[SBA_K];
+  uint32_t p[SBA_K], pinv[SBA_K], r1[SBA_K];   // r1 = 2^32 mod p
   // length-3 DFT constants (Montgomery, canonical): h1 = -1/2,
   // h2 = (w - w^2)/2 for the cube root w, h2i = -h2 (the inverse's)
   uint32_t h1[SBA_K], h2[SBA_K], h2i[SBA_K];
@@ -21,7 +21,7 @@ struct PrimeArgs {
 
 struct GarnerArgs {
   uint32_t p[SBA_K], pinv[SBA_K];
-  uint32_t scale[SBA_K];          // N^-1 * R^2 mod p (undoes N and the pointwise R^-1)
+  uint32_t scale[SBA_K];          // N^-1 * R^4 mod p (see enqueue_garner)
   uint32_t cm[SBA_K][SBA_K];      // (prod_{j<k} p_j mod p_q) * R mod p_q
   uint32_t im[SBA_K];             // (prod_{j<q} p_j)^-1 * R mod p_q
 };
@@ -44,9 +44,23 @@ struct GarnerArgs {
 #ifndef SBA_MERGE_LAST
 #define SBA_MERGE_LAST 1
 #endif
-// occupancy hint of the merged last pass (waves per SIMD)
+// occupancy hints (waves per SIMD) of the merged last pass, the first
+// forward pass and the inverse column pass, and the unroll of the first
+// pass's limb loads, fixed by measurement
 #ifndef SBA_FUSED2_WAVES
-#define SBA_FUSED2_WAVES 1
+#define SBA_FUSED2_WAVES 3
+#endif
+#ifndef SBA_FIRST_WAVES
+#define SBA_FIRST_WAVES SBA_WAVES
+#endif
+#ifndef SBA_INV_WAVES
+#define SBA_INV_WAVES SBA_WAVES
+#endif
+#ifndef SBA_FIRST_LOAD_UNROLL
+#define SBA_FIRST_LOAD_UNROLL 8
+#endif
+#ifndef SBA_PASS_LOAD_UNROLL
+#define SBA_PASS_LOAD_UNROLL 32
 #endif
 // 1: widest chunks even where group addressing is not linear
 #ifndef SBA_CHUNK_GREEDY
@@ -75,15 +89,15 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
 template <int X> struct ILog2 { static constexpr int v = 1 + ILog2<X / 2>::v; };
 template <> struct ILog2<1> { static constexpr int v = 0; };
 
-// residue of a limb, lazy in [0, 2p): (hi * 2^32 mod p) + (lo mod p).
-// p > 2^30 so lo < 4p: two folds bring lo into [0, p).
+// residue of a limb times 2^-32, lazy in [0, 2p): t = hi * (2^32 mod p) +
+// lo = x mod p as an integer below 2^32 p, then one Montgomery reduction
+// (t + m p < 2^64 since p < 2^31). Every operand carries the same 2^-32,
+// removed with the rest in the Garner scale.
 __device__ __forceinline__ uint32_t limb_residue(uint64_t x, uint32_t p, uint32_t pinv,
-                                                 uint32_t r2) {
-  uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-  uint32_t h = red1(mont_mul(hi, r2, p, pinv), p);
-  uint32_t l2 = lo - 2u * p;
-  uint32_t l = l2 < lo ? l2 : lo;
-  return h + red1(l, p);
+                                                 uint32_t r1) {
+  const uint64_t t = (uint64_t)(uint32_t)(x >> 32) * r1 + (uint32_t)x;
+  const uint32_t m = (uint32_t)t * pinv;
+  return (uint32_t)(((uint64_t)m * p + t) >> 32);
 }
 
 // ---------------------------------------------------------------------------
@@ -269,7 +283,7 @@ __device__ __forceinline__ Job get_job(const JobRef &jr) { return get_job(jr, bl
 // primes -> ELL levels -> spec[q][.]. Tiles = column tiles (H = 0).
 // which = 0: operand a into aspec; 1: operand b into bspec.
 template <int ELL, int LOGTC, int BMAX>
-__global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd(PrimeArgs pa, JobRef jr, int which,
+__global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa, JobRef jr, int which,
                                                              int L) {
   constexpr int TC = 1 << LOGTC;
   constexpr int NE = 1 << (ELL + LOGTC);
@@ -294,14 +308,14 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd(PrimeArgs pa, JobRe
   const uint32_t q1 = split ? q0 + 1 : SBA_K;
 #pragma unroll 1
   for (uint32_t q = q0; q < q1; ++q) {
-    const uint32_t p = pa.p[q], pinv = pa.pinv[q], r2 = pa.r2[q];
+    const uint32_t p = pa.p[q], pinv = pa.pinv[q], r1 = pa.r1[q];
     __syncthreads();
     // the tile's limbs are re-read per prime (cache-resident after the first)
-#pragma unroll
+#pragma unroll SBA_FIRST_LOAD_UNROLL
     for (int i = 0; i < E; ++i) {
       const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
       gc_u64 *lp = limbs + uoff;
-      lt[i * Pad<LOGTC>::words(NT)] = limb_residue(uoff + toff < nl ? lp[toff] : 0, p, pinv, r2);
+      lt[i * Pad<LOGTC>::words(NT)] = limb_residue(uoff + toff < nl ? lp[toff] : 0, p, pinv, r1);
     }
     tile_fwd<ELL, LOGTC, BMAX, 0>(lds, pa.wn[q], 0, p, pinv);
     __syncthreads();
@@ -317,7 +331,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd(PrimeArgs pa, JobRe
 // In-place pass over global levels [s0, s0 + ELL) of one prime
 // (blockIdx.y), forward or inverse, on aspec (which = 0) or bspec (1).
 template <int ELL, int LOGTC, bool INV, int BMAX>
-__global__ __launch_bounds__(NT, SBA_WAVES) void k_pass(PrimeArgs pa, JobRef jr, int which, int L,
+__global__ __launch_bounds__(NT, INV ? SBA_INV_WAVES : SBA_WAVES) void k_pass(PrimeArgs pa, JobRef jr, int which, int L,
                                                         int s0, AccArgs acc) {
   constexpr int TC = 1 << LOGTC;
   constexpr int NE = 1 << (ELL + LOGTC);
@@ -337,7 +351,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_pass(PrimeArgs pa, JobRef jr,
   const uint32_t p = pa.p[q], pinv = pa.pinv[q];
   uint32_t *lt = lds + lpadt<LOGTC>(threadIdx.x);
   const uint32_t toff = (threadIdx.x >> LOGTC) * S + (threadIdx.x & (TC - 1));
-#pragma unroll
+#pragma unroll SBA_PASS_LOAD_UNROLL
   for (int i = 0; i < E; ++i) {
     const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
     lt[i * Pad<LOGTC>::words(NT)] = (buf + uoff)[toff];
@@ -398,7 +412,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
   for (int i = 0; i < E; ++i) {
     uint32_t v;
     if (LIMB_IN) {
-      v = limb_residue(base + i * NT + tid < nb ? (bl + i * NT)[tid] : 0, p, pinv, pa.r2[q]);
+      v = limb_residue(base + i * NT + tid < nb ? (bl + i * NT)[tid] : 0, p, pinv, pa.r1[q]);
     } else {
       v = (buf + i * NT)[tid];
     }
@@ -522,7 +536,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd3(PrimeArgs pa, JobR
   const uint32_t q1 = split ? q0 + 1 : SBA_K;
 #pragma unroll 1
   for (uint32_t q = q0; q < q1; ++q) {
-    const uint32_t p = pa.p[q], pinv = pa.pinv[q], r2 = pa.r2[q];
+    const uint32_t p = pa.p[q], pinv = pa.pinv[q], r1 = pa.r1[q];
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < E; ++i) {
@@ -532,7 +546,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd3(PrimeArgs pa, JobR
 #pragma unroll
       for (int t = 0; t < 3; ++t) {
         const uint32_t g = s + (uint32_t)t * M;
-        x[t] = red1(limb_residue(g < nl ? (limbs + uoff + (uint32_t)t * M)[toff] : 0, p, pinv, r2), p);
+        x[t] = red1(limb_residue(g < nl ? (limbs + uoff + (uint32_t)t * M)[toff] : 0, p, pinv, r1), p);
       }
       const uint32_t sm = s % 3;
       uint32_t y0, y1, y2;

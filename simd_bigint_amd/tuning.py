@@ -41,9 +41,10 @@ Plan = Tuple[int, int, int]            # (last, bm, split)
 # ---------------------------------------------------------------------------
 # plan table
 # ---------------------------------------------------------------------------
-def plan_candidates(L: int) -> List[Plan]:
+def plan_candidates(L: int, kind: int = 0) -> List[Plan]:
+    """Valid plans of class (L, kind); kind = square? | radix-3? << 1."""
     buf = (ctypes.c_int * (3 * 64))()
-    n = lib().sba_plan_candidates(L, buf, 64)
+    n = lib().sba_plan_candidates(L, kind, buf, 64)
     return [(buf[3 * i], buf[3 * i + 1], buf[3 * i + 2]) for i in range(min(n, 64))]
 
 
@@ -54,16 +55,16 @@ def plan_heuristic(L: int) -> Plan:
     return (a.value, b.value, c.value)
 
 
-def plan_get(L: int, sqr: int, bucket: int) -> Plan:
+def plan_get(L: int, kind: int, bucket: int) -> Plan:
     a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-    if lib().sba_plan_get(L, sqr, bucket, ctypes.byref(a), ctypes.byref(b),
+    if lib().sba_plan_get(L, kind, bucket, ctypes.byref(a), ctypes.byref(b),
                           ctypes.byref(c)) != SBA_OK:
         raise ValueError("bad plan key")
     return (a.value, b.value, c.value)
 
 
-def plan_set(L: int, sqr: int, bucket: int, plan: Plan) -> None:
-    if lib().sba_plan_set(L, sqr, bucket, *plan) != SBA_OK:
+def plan_set(L: int, kind: int, bucket: int, plan: Plan) -> None:
+    if lib().sba_plan_set(L, kind, bucket, *plan) != SBA_OK:
         raise ValueError("invalid plan %r for L=%d" % (plan, L))
 
 
@@ -173,7 +174,9 @@ def default_shapes(max_elems: int = 1 << 24) -> List[Tuple[int, int, int, float]
             for nj in (1, 3, 8, 24, 64, 192, 512, 1024):
                 if nj * (1 << L) > max_elems:
                     continue
-                for fill in (0.6, 0.95):
+                # fills of power-of-two classes (at most 0.75 of 2^L runs as
+                # a radix-3 class of 3 * 2^(L-2) for L >= 15)
+                for fill in (0.8, 0.95):
                     out.append((L, sqr, nj, fill))
     return out
 

@@ -59,13 +59,13 @@ def test_every_plan_bit_exact(L, sqr):
     pool = torch.randint(-2**63, 2**63 - 1, (1 << 20,), dtype=torch.int64, generator=g).to(dev)
     jobs = tuning.make_class_jobs(L, sqr, 3, 0.95, pool, dev)
     # one ragged job of the same class: a short second operand (or a square
-    # just above half the transform length)
+    # filling the transform to its last coefficient)
     if sqr:
-        an = (1 << (L - 2)) + 3
+        an = 1 << (L - 1)
         r = torch.empty(2 * an, dtype=torch.int64, device=dev)
         jobs.append((r, pool[11:11 + an], None))
     else:
-        an, bn = (1 << (L - 1)) - 5, 37
+        an, bn = (1 << L) - 41, 37
         r = torch.empty(an + bn, dtype=torch.int64, device=dev)
         jobs.append((r, pool[7:7 + an], pool[99:99 + bn]))
     want = _expected(jobs)
@@ -79,6 +79,41 @@ def test_every_plan_bit_exact(L, sqr):
         for k, (r, _, _) in enumerate(jobs):
             got = _limbs(r)
             assert np.array_equal(got, want[k]), "plan %r job %d differs" % (plan, k)
+
+
+@pytest.mark.parametrize("L", [13, 14, 16, 19])
+@pytest.mark.parametrize("sqr", [0, 1])
+def test_every_radix3_plan_bit_exact(L, sqr):
+    """Batch classes of length 3 * 2^L: every radix-3 plan, mixed fills,
+    against the oracle."""
+    torch, dev = _torch()
+    g = torch.Generator(device="cpu").manual_seed(200 + L)
+    pool = torch.randint(-2**63, 2**63 - 1, (1 << 21,), dtype=torch.int64, generator=g).to(dev)
+    jobs = []
+    for z in ((3 << L) // 2, ((1 << (L + 1)) + 3) // 2 + 1, (5 << L) // 4):
+        an = z if sqr else z + 7
+        bn = z if sqr else z - 7
+        r = torch.empty(an + bn, dtype=torch.int64, device=dev)
+        jobs.append((r, pool[13:13 + an], None if sqr else pool[501:501 + bn]))
+    kinds = set()
+    for r, a, b in jobs:
+        Lk, kind = ctypes.c_int(), ctypes.c_int()
+        arr = sba.job_array([(r, a, b)])
+        assert sba.lib().sba_job_class(ctypes.cast(arr, ctypes.c_void_p), ctypes.byref(Lk),
+                                       ctypes.byref(kind)) == 1
+        kinds.add((Lk.value, kind.value))
+    assert kinds == {(L, 2 | sqr)}, kinds
+    want = _expected(jobs)
+    plans = tuning.plan_candidates(L, 2 | sqr)
+    say("radix-3 L=%d sqr=%d: %d plans" % (L, sqr, len(plans)))
+    assert plans
+    for plan in plans:
+        for r, _, _ in jobs:
+            r.fill_(0x5A5A)
+        tuning.class_time(jobs, plan, reps=1)
+        torch.cuda.synchronize()
+        for k, (r, _, _) in enumerate(jobs):
+            assert np.array_equal(_limbs(r), want[k]), "plan %r job %d differs" % (plan, k)
 
 
 def test_every_plan_bit_exact_large():
@@ -169,6 +204,8 @@ def _mixed_jobs(torch, dev):
     jobs = []
     for L, sqr, nj in ((11, 0, 40), (13, 1, 12), (15, 0, 6), (18, 0, 2)):
         jobs += tuning.make_class_jobs(L, sqr, nj, 0.8, pool, dev)
+    # and one radix-3 class (length 3 * 2^14)
+    jobs += tuning.make_class_jobs(16, 0, 3, 0.7, pool, dev)
     return jobs
 
 
@@ -178,6 +215,7 @@ def test_autotune_stores_measured_plans_and_stays_exact():
     want = _expected(jobs)
     m = tuning.autotune(jobs, mode=tuning.EXHAUSTIVE, reps=2)
     ncand = sum(len(tuning.plan_candidates(L)) for L in (11, 13, 15, 18))
+    ncand += len(tuning.plan_candidates(14, 2))
     assert m == ncand
     for L, sqr, nj in ((11, 0, 40), (13, 1, 12), (15, 0, 6), (18, 0, 2)):
         say("tuned L=%d sqr=%d n=%d: %r" % (L, sqr, nj,
@@ -203,7 +241,7 @@ def test_autotune_with_model_times_fewer_plans():
     tuning.CostModel.load().install()
     assert sba.lib().sba_cm_loaded() == 1
     m2 = tuning.autotune(jobs, mode=tuning.MODEL, topk=2, reps=2)
-    assert m2 == 2 * 4
+    assert m2 == 2 * 4                        # the radix-3 class keeps its plan
     m1 = tuning.autotune(jobs, mode=tuning.MODEL, topk=1, reps=2)
     assert m1 == 0
     bt = sba.Batch(jobs, use_graph=True)
