@@ -49,6 +49,60 @@ static void grow(uint64_t **p, size_t *cap, size_t n) {
   *cap = want;
 }
 
+/* pinned host staging (grow-only): the caller's pageable buffers are copied
+ * into it by several threads, and the DMA runs from pinned memory at full
+ * link rate */
+static uint64_t *g_ha, *g_hb, *g_hr;
+static size_t g_hna, g_hnb, g_hnr;
+
+static void grow_pinned(uint64_t **p, size_t *cap, size_t n) {
+  if (*cap >= n) return;
+  if (*p) hipHostFree(*p);
+  *p = NULL;
+  *cap = 0;
+  size_t want = n + n / 8 + 64;
+  if (hipHostMalloc((void **)p, want * 8, hipHostMallocDefault) != hipSuccess)
+    die("hipHostMalloc", SBA_ERR_HIP);
+  *cap = want;
+}
+
+#define COPY_THREADS 8
+#define COPY_MIN_PER_THREAD ((size_t)1 << 19)     /* bytes */
+struct copy_job { void *dst; const void *src; size_t n; };
+static void *copy_worker(void *arg) {
+  struct copy_job *j = (struct copy_job *)arg;
+  memcpy(j->dst, j->src, j->n);
+  return NULL;
+}
+/* memcpy split over up to COPY_THREADS threads */
+static void par_copy(void *dst, const void *src, size_t bytes) {
+  size_t nt = bytes / COPY_MIN_PER_THREAD;
+  if (nt > COPY_THREADS) nt = COPY_THREADS;
+  if (nt < 2) {
+    memcpy(dst, src, bytes);
+    return;
+  }
+  pthread_t th[COPY_THREADS];
+  struct copy_job jb[COPY_THREADS];
+  const size_t per = (bytes / nt + 63) & ~(size_t)63;
+  size_t off = 0, started = 0;
+  for (size_t t = 0; t < nt && off < bytes; ++t) {
+    const size_t n = bytes - off < per ? bytes - off : per;
+    jb[t].dst = (char *)dst + off;
+    jb[t].src = (const char *)src + off;
+    jb[t].n = n;
+    off += n;
+    if (t + 1 == nt || off >= bytes) {
+      copy_worker(&jb[t]);              /* the calling thread takes the last piece */
+    } else if (pthread_create(&th[started], NULL, copy_worker, &jb[t]) == 0) {
+      ++started;
+    } else {
+      copy_worker(&jb[t]);
+    }
+  }
+  for (size_t t = 0; t < started; ++t) pthread_join(th[t], NULL);
+}
+
 /* H2D, multiply on the default stream, D2H. Returns the engine's code. */
 static int gpu_in_band(size_t an, size_t bn) {
   return sba_gpu_in_band((int64_t)an, (int64_t)bn);
@@ -62,16 +116,26 @@ static int gpu_mul_host(uint64_t *rp, const uint64_t *ap, size_t an,
   grow(&g_da, &g_na, an);
   if (!sqr) grow(&g_db, &g_nb, bn);
   grow(&g_dr, &g_nr, an + bn);
-  if (hipMemcpy(g_da, ap, an * 8, hipMemcpyHostToDevice) != hipSuccess)
-    die("hipMemcpy H2D", SBA_ERR_HIP);
-  if (!sqr && hipMemcpy(g_db, bp, bn * 8, hipMemcpyHostToDevice) != hipSuccess)
-    die("hipMemcpy H2D", SBA_ERR_HIP);
+  grow_pinned(&g_ha, &g_hna, an);
+  if (!sqr) grow_pinned(&g_hb, &g_hnb, bn);
+  grow_pinned(&g_hr, &g_hnr, an + bn);
+  par_copy(g_ha, ap, an * 8);
+  if (hipMemcpyAsync(g_da, g_ha, an * 8, hipMemcpyHostToDevice, NULL) != hipSuccess)
+    die("hipMemcpyAsync H2D", SBA_ERR_HIP);
+  if (!sqr) {
+    par_copy(g_hb, bp, bn * 8);                 /* overlaps the first DMA */
+    if (hipMemcpyAsync(g_db, g_hb, bn * 8, hipMemcpyHostToDevice, NULL) != hipSuccess)
+      die("hipMemcpyAsync H2D", SBA_ERR_HIP);
+  }
   int rc = sqr ? sba_gpu_sqr_dev(g_dr, g_da, (int64_t)an, NULL)
                : sba_gpu_mul_dev(g_dr, g_da, (int64_t)an, g_db, (int64_t)bn, NULL);
   if (rc < 0) die("GPU multiply", rc);
-  if (rc == SBA_OK &&
-      hipMemcpy(rp, g_dr, (an + bn) * 8, hipMemcpyDeviceToHost) != hipSuccess)
-    die("hipMemcpy D2H", SBA_ERR_HIP);
+  if (rc == SBA_OK) {
+    if (hipMemcpyAsync(g_hr, g_dr, (an + bn) * 8, hipMemcpyDeviceToHost, NULL) != hipSuccess ||
+        hipStreamSynchronize(NULL) != hipSuccess)
+      die("hipMemcpyAsync D2H", SBA_ERR_HIP);
+    par_copy(rp, g_hr, (an + bn) * 8);
+  }
   pthread_mutex_unlock(&g_stage_mu);
   return rc;
 }

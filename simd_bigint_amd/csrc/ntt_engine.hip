@@ -41,7 +41,7 @@ __device__ __forceinline__ void garner_coef(const GarnerArgs &ga, gc_u32 *spec,
   uint32_t u[SBA_K], x[SBA_K];
 #pragma unroll
   for (int q = 0; q < SBA_K; ++q)
-    u[q] = red1(mont_mul(spec[q * N + i], ga.scale[q], ga.p[q], ga.pinv[q]), ga.p[q]);
+    u[q] = red1(spec[q * N + i], ga.p[q]);           // already scaled by the inverse
   x[0] = u[0];
 #pragma unroll
   for (int q = 1; q < SBA_K; ++q) {
@@ -75,8 +75,8 @@ __device__ __forceinline__ void garner_coef(const GarnerArgs &ga, gc_u32 *spec,
 // is thus a map {0..3} -> {0..3} (2 bits per entry, 8 bits); maps compose
 // associatively, so the carries are an exclusive scan of maps:
 //   k_garner_maps  Garner + limb sums + limb maps + block maps (LDS-staged)
-//   k_carry_blocks scan of block maps -> carry into each block
-//   k_carry_fix    per-limb carries from the maps, out_j = v_j + c_j
+//   k_carry_fix    carry into the block (earlier block maps composed),
+//                  per-limb carries from the limb maps, out_j = v_j + c_j
 // ---------------------------------------------------------------------------
 #define CARRY_LPT 8                     // limbs per thread in the map scan
 #define CARRY_LPB (CARRY_LPT * NT)      // limbs per block
@@ -181,27 +181,6 @@ __global__ __launch_bounds__(NT) void k_garner_maps(GarnerArgs ga, JobRef jr, ui
   if (threadIdx.x == 0) ((g_u32 *)jb.bmaps)[blockIdx.x] = tot;
 }
 
-// one block per job: carry into every block = exclusive scan of block maps
-__global__ __launch_bounds__(NT) void k_carry_blocks(JobRef jr) {
-  const Job jb = get_job(jr);
-  const uint32_t nb = jb.nb;
-  gc_u32 *bmaps = (gc_u32 *)jb.bmaps;
-  g_u32 *bcin = (g_u32 *)jb.bcin;
-  const uint32_t per = (nb + NT - 1) / NT;
-  const uint32_t b0 = threadIdx.x * per;
-  uint32_t f = MAP_ID;
-  for (uint32_t k = 0; k < per; ++k)
-    if (b0 + k < nb) f = map_then(f, bmaps[b0 + k]);
-  uint32_t tot;
-  const uint32_t pre = block_scan_maps(f, &tot);
-  uint32_t c = map_apply(pre, 0);
-  for (uint32_t k = 0; k < per; ++k)
-    if (b0 + k < nb) {
-      bcin[b0 + k] = c;
-      c = map_apply(bmaps[b0 + k], c);
-    }
-}
-
 __global__ __launch_bounds__(NT) void k_carry_fix(JobRef jr) {
   __shared__ uint64_t cj64[CARRY_LPB / 8];
   const Job jb = get_job(jr);
@@ -216,7 +195,19 @@ __global__ __launch_bounds__(NT) void k_carry_fix(JobRef jr) {
   for (int k = 0; k < CARRY_LPT; ++k) f = map_then(f, (uint32_t)(m8 >> (8 * k)) & 0xFFu);
   uint32_t tot;
   const uint32_t pre = block_scan_maps(f, &tot);
-  uint32_t c = map_apply(pre, ((gc_u32 *)jb.bcin)[blockIdx.x]);
+  // carry into this block: every earlier block's map, composed in order and
+  // applied to 0 (each thread composes a run of them, the block scan joins
+  // the runs; the block maps are few and cache-resident)
+  gc_u32 *bm = (gc_u32 *)jb.bmaps;
+  const uint32_t nprev = blockIdx.x;
+  const uint32_t per = (nprev + NT - 1) / NT;
+  const uint32_t s0 = threadIdx.x * per;
+  uint32_t g = MAP_ID;
+  for (uint32_t k = 0; k < per; ++k)
+    if (s0 + k < nprev) g = map_then(g, bm[s0 + k]);
+  uint32_t gall;
+  block_scan_maps(g, &gall);
+  uint32_t c = map_apply(pre, map_apply(gall, 0));
 #pragma unroll
   for (int k = 0; k < CARRY_LPT; ++k) {
     cj[threadIdx.x * CARRY_LPT + k] = (uint8_t)c;
@@ -381,11 +372,11 @@ static int ctx_twiddles(DevCtx &c, int L, hipStream_t st) {
 }
 
 // workspace layout for transform length N: aspec, bspec (K*N u32 each),
-// limb carry maps (one byte per limb, padded to whole blocks), block maps and
-// block carry-ins (one u32 per block of CARRY_LPB limbs each)
+// limb carry maps (one byte per limb, padded to whole blocks) and block maps
+// (one u32 per block of CARRY_LPB limbs)
 static size_t ws_need(size_t N) {
   const size_t nb = N / CARRY_LPB + 2;
-  return 2 * SBA_K * N * 4 + nb * CARRY_LPB + 2 * nb * 4 + 4096;
+  return 2 * SBA_K * N * 4 + nb * CARRY_LPB + nb * 4 + 4096;
 }
 
 static int lg_for(uint64_t zlen) {
@@ -596,6 +587,18 @@ static int enqueue_garner(DevCtx &c, uint32_t Nt, uint32_t stride, const JobRef 
                           uint32_t njobs, uint32_t nbmax, double zsum, hipStream_t st);
 static const AccArgs kNoAcc = {nullptr, 0, 0, 0, 0};
 
+// the context's prime constants with the inverse's final scale for
+// transform length Nt: Nt^-1 * R^4 mod p undoes Nt, the pointwise R^-1, the
+// R^-1 each operand's limb residues carry and this multiply's own R^-1
+static PrimeArgs prime_args(const DevCtx &c, uint64_t Nt) {
+  PrimeArgs pa = c.pa;
+  for (int q = 0; q < SBA_K; ++q) {
+    const uint64_t p = kPrimes[q];
+    pa.scale[q] = (uint32_t)((unsigned __int128)h_invm(Nt % p, p) * h_powm(2, 128, p) % p);
+  }
+  return pa;
+}
+
 // carry blocks of a product of Z limbs
 static uint32_t carry_blocks(uint64_t Z) { return (uint32_t)((Z + CARRY_LPB - 1) / CARRY_LPB); }
 
@@ -608,7 +611,7 @@ static int enqueue_class(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef
   if (!plan_ok(L, pl)) return SBA_BAND;
   const Launchers k = launchers(L, sqr, pl);
   const uint32_t N = 1u << L;
-  const PrimeArgs &pa = c.pa;
+  const PrimeArgs pa = prime_args(c, N);
   const double KN4 = 4.0 * SBA_K * (double)N * njobs;
   const uint32_t gy1 = pl.split ? SBA_K : 1;
   if (pl.last == L) {
@@ -622,7 +625,7 @@ static int enqueue_class(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef
     }
   } else {
     const int ell1 = L - pl.last;
-    const uint32_t ntile1 = N >> 13;
+    const uint32_t ntile1 = N >> SBA_FIRST_TILE_LG;
     const uint32_t ntile2 = N >> pl.last;
     if (!sqr) {
       PROF("first_fwd", 8.0 * asum + KN4);
@@ -658,21 +661,11 @@ static int enqueue_class(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef
 // unnormalised inverse
 static int enqueue_garner(DevCtx &c, uint32_t Nt, uint32_t stride, const JobRef &jr,
                           uint32_t njobs, uint32_t nbmax, double zsum, hipStream_t st) {
+  (void)Nt;
   GarnerArgs ga = c.ga_base;
-  for (int q = 0; q < SBA_K; ++q) {
-    const uint64_t p = kPrimes[q];
-    const uint64_t ninv = h_invm(Nt % p, p);
-    // scale = Nt^-1 * R^4 mod p: undoes Nt, the pointwise R^-1, each
-    // operand's R^-1 (limb residues) and the Garner product's own R^-1
-    ga.scale[q] = (uint32_t)((unsigned __int128)ninv * h_powm(2, 128, p) % p);
-  }
   {
     PROF("garner_maps", 4.0 * SBA_K * zsum + 9.0 * zsum);
     hipLaunchKernelGGL(k_garner_maps, dim3(nbmax, 1, njobs), dim3(NT), 0, st, ga, jr, stride);
-  }
-  {
-    PROF("carry_blocks", 8.0 * zsum / CARRY_LPB);
-    hipLaunchKernelGGL(k_carry_blocks, dim3(1, 1, njobs), dim3(NT), 0, st, jr);
   }
   {
     PROF("carry_fix", 17.0 * zsum);
@@ -689,7 +682,6 @@ static void job_layout(Job &j, void *ws, uint32_t N) {
   j.lmaps = (uint8_t *)(j.bspec + (size_t)SBA_K * N);
   const size_t nbcap = N / CARRY_LPB + 2;
   j.bmaps = (uint32_t *)(j.lmaps + nbcap * CARRY_LPB);
-  j.bcin = j.bmaps + nbcap;
 }
 
 static int get_ctx(DevCtx **out) {
@@ -747,7 +739,7 @@ static ChunkChoice chunk_choice(uint64_t an, uint64_t bn, int sqr) {
 // short spectrum, chunk spectrum, accumulator, carry maps of Z limbs
 static size_t ws_chunked(int Lc, uint64_t Z) {
   const size_t Nc = (size_t)1 << Lc, Zs = (Z + 255) & ~(size_t)255, nb = Z / CARRY_LPB + 2;
-  return 2 * SBA_K * Nc * 4 + SBA_K * Zs * 4 + nb * CARRY_LPB + 2 * nb * 4 + 4096;
+  return 2 * SBA_K * Nc * 4 + SBA_K * Zs * 4 + nb * CARRY_LPB + nb * 4 + 4096;
 }
 
 // an >= bn; ws holds ws_chunked(cp.Lc, an + bn) bytes
@@ -759,11 +751,11 @@ static int enqueue_chunked(DevCtx &c, const ChunkChoice &cp, uint64_t *rp, const
   const Plan pl = plan_for(Lc, 0, 1);
   if (!plan_ok(Lc, pl) || pl.last == Lc) return SBA_BAND;
   const Launchers k = launchers(Lc, 0, pl);
-  const PrimeArgs &pa = c.pa;
+  const PrimeArgs pa = prime_args(c, Nc);
   const uint64_t Z = an + bn;
   const uint32_t Zs = (uint32_t)((Z + 255) & ~(uint64_t)255);
   const int ell1 = Lc - pl.last;
-  const uint32_t ntile1 = Nc >> 13, ntile2 = Nc >> pl.last;
+  const uint32_t ntile1 = Nc >> SBA_FIRST_TILE_LG, ntile2 = Nc >> pl.last;
   const uint32_t gy1 = pl.split ? SBA_K : 1;
   const double KN4 = 4.0 * SBA_K * Nc;
   JobRef jr;
@@ -822,7 +814,6 @@ static int enqueue_chunked(DevCtx &c, const ChunkChoice &cp, uint64_t *rp, const
   const size_t nbcap = Z / CARRY_LPB + 2;
   g.lmaps = (uint8_t *)(acc + (size_t)SBA_K * Zs);
   g.bmaps = (uint32_t *)(g.lmaps + nbcap * CARRY_LPB);
-  g.bcin = g.bmaps + nbcap;
   return enqueue_garner(c, Nc, Zs, jg, 1, g.nb, (double)Z, st);
 }
 
@@ -877,7 +868,7 @@ static int enqueue_r3(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef &j
   const first_fn f1 = sba_r3_first(ell1);
   const inv3_fn i1 = sba_r3_inv(ell1);
   const uint32_t M = 1u << L;
-  const PrimeArgs &pa = c.pa;
+  const PrimeArgs pa = prime_args(c, 3 * (uint64_t)M);
   const double KN4 = 4.0 * SBA_K * 3.0 * M * njobs;
   const uint32_t ntile1 = M >> 11, ntile2 = M >> pl.last;
   const uint32_t gy1 = pl.split ? SBA_K : 1;

@@ -15,13 +15,15 @@ struct PrimeArgs {
   // length-3 DFT constants (Montgomery, canonical): h1 = -1/2,
   // h2 = (w - w^2)/2 for the cube root w, h2i = -h2 (the inverse's)
   uint32_t h1[SBA_K], h2[SBA_K], h2i[SBA_K];
+  // applied by the pass that finishes the inverse transform: Nt^-1 * R^4
+  // (Nt the transform length; see the host's prime_args)
+  uint32_t scale[SBA_K];
   const uint32_t *wn[SBA_K];
   const uint32_t *wni[SBA_K];
 };
 
 struct GarnerArgs {
   uint32_t p[SBA_K], pinv[SBA_K];
-  uint32_t scale[SBA_K];          // N^-1 * R^4 mod p (see enqueue_garner)
   uint32_t cm[SBA_K][SBA_K];      // (prod_{j<k} p_j mod p_q) * R mod p_q
   uint32_t im[SBA_K];             // (prod_{j<q} p_j)^-1 * R mod p_q
 };
@@ -62,6 +64,14 @@ struct GarnerArgs {
 #ifndef SBA_PASS_LOAD_UNROLL
 #define SBA_PASS_LOAD_UNROLL 32
 #endif
+// residues per first-pass (and inverse column pass) tile, log2, and the row
+// padding of 16-column tiles
+#ifndef SBA_FIRST_TILE_LG
+#define SBA_FIRST_TILE_LG 13
+#endif
+#ifndef SBA_PAD16
+#define SBA_PAD16 16
+#endif
 // 1: widest chunks even where group addressing is not linear
 #ifndef SBA_CHUNK_GREEDY
 #define SBA_CHUNK_GREEDY 0
@@ -71,7 +81,7 @@ struct GarnerArgs {
 // the TC columns x 32/TC groups of a half-wave on 32 distinct banks -- and
 // by 1 word otherwise (the contiguous tiles).
 template <int LOGTC> struct Pad {
-  static constexpr int PW = LOGTC <= 4 ? (1 << LOGTC) : 1;
+  static constexpr int PW = LOGTC < 4 ? (1 << LOGTC) : (LOGTC == 4 ? SBA_PAD16 : 1);
   static constexpr int words(int ne) { return ne + (ne / 32) * PW; }
 };
 template <int LOGTC> __device__ __forceinline__ uint32_t lpadt(uint32_t i) {
@@ -227,7 +237,7 @@ struct Job {
   uint64_t *r;                    // product limbs (device), an+bn of them
   uint32_t *aspec, *bspec;        // K x N residues each
   uint8_t *lmaps;                 // one carry map per limb, padded to blocks
-  uint32_t *bmaps, *bcin;         // per carry block
+  uint32_t *bmaps;                // one map per carry block
   uint32_t an, bn, zlen, Z, nb;   // limbs, coefficients, product limbs, carry blocks
 };
 // global-address-space views of descriptor pointers: loads through a pointer
@@ -269,7 +279,7 @@ __device__ __forceinline__ Job get_job(const JobRef &jr, uint32_t idx) {
   }
   j.a = uni_ptr(j.a); j.b = uni_ptr(j.b); j.r = uni_ptr(j.r);
   j.aspec = uni_ptr(j.aspec); j.bspec = uni_ptr(j.bspec); j.lmaps = uni_ptr(j.lmaps);
-  j.bmaps = uni_ptr(j.bmaps); j.bcin = uni_ptr(j.bcin);
+  j.bmaps = uni_ptr(j.bmaps);
   j.an = uni_u32(j.an); j.bn = uni_u32(j.bn); j.zlen = uni_u32(j.zlen); j.Z = uni_u32(j.Z);
   j.nb = uni_u32(j.nb);
   return j;
@@ -359,6 +369,9 @@ __global__ __launch_bounds__(NT, INV ? SBA_INV_WAVES : SBA_WAVES) void k_pass(Pr
   if (!INV) tile_fwd<ELL, LOGTC, BMAX, 0>(lds, pa.wn[q], H, p, pinv);
   else tile_inv<ELL, LOGTC, BMAX, 0>(lds, pa.wni[q], H, p, pinv);
   __syncthreads();
+  // the inverse's last levels (s0 == 0) finish the transform: scale here
+  const bool fin = INV && s0 == 0;
+  const uint32_t sc = pa.scale[q];
   if (INV && acc.dst) {
     // one chunk of a chunked product: coefficient x of the chunk lands at
     // acc.off + x of the accumulator; x < acc.ovl adds to the previous
@@ -369,7 +382,7 @@ __global__ __launch_bounds__(NT, INV ? SBA_INV_WAVES : SBA_WAVES) void k_pass(Pr
       const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
       const uint32_t x = base + uoff + toff;
       if (x < acc.lim) {
-        uint32_t v = lt[i * Pad<LOGTC>::words(NT)];
+        uint32_t v = mont_mul(lt[i * Pad<LOGTC>::words(NT)], sc, p, pinv);
         if (x < acc.ovl) v = red1(v, p) + red1((d + uoff)[toff], p);
         (d + uoff)[toff] = v;
       }
@@ -379,7 +392,8 @@ __global__ __launch_bounds__(NT, INV ? SBA_INV_WAVES : SBA_WAVES) void k_pass(Pr
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
-    (buf + uoff)[toff] = lt[i * Pad<LOGTC>::words(NT)];
+    const uint32_t v = lt[i * Pad<LOGTC>::words(NT)];
+    (buf + uoff)[toff] = fin ? mont_mul(v, sc, p, pinv) : v;
   }
 }
 
@@ -428,8 +442,12 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
   }
   tile_inv<ELL, 0, BMAX, 0>(lds, pa.wni[q], H, p, pinv);
   __syncthreads();
+  // LIMB_IN: the single tile is the whole transform, so the inverse ends here
 #pragma unroll
-  for (int i = 0; i < E; ++i) (buf + i * NT)[tid] = lt[i * Pad<0>::words(NT)];
+  for (int i = 0; i < E; ++i) {
+    const uint32_t v = lt[i * Pad<0>::words(NT)];
+    (buf + i * NT)[tid] = LIMB_IN ? mont_mul(v, pa.scale[q], p, pinv) : v;
+  }
 }
 
 // Multiply, two-pass plans: the last forward levels of a AND of b on the same
@@ -607,9 +625,11 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_inv_first3(PrimeArgs pa, JobR
     const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
     const uint32_t s = uoff + toff;
     uint32_t r0, r1, r2;
-    dft3(red1(lt[i * Pad<LOGTC>::words(NT)], p), red1(lt[W + i * Pad<LOGTC>::words(NT)], p),
-         red1(lt[2 * W + i * Pad<LOGTC>::words(NT)], p), r0, r1, r2, pa.h1[q], pa.h2i[q], p,
-         pinv);
+    const uint32_t sc = pa.scale[q];
+    dft3(red1(mont_mul(lt[i * Pad<LOGTC>::words(NT)], sc, p, pinv), p),
+         red1(mont_mul(lt[W + i * Pad<LOGTC>::words(NT)], sc, p, pinv), p),
+         red1(mont_mul(lt[2 * W + i * Pad<LOGTC>::words(NT)], sc, p, pinv), p), r0, r1, r2,
+         pa.h1[q], pa.h2i[q], p, pinv);
     // row r holds coefficient s + t(r) M: write limb position t from row
     // r(t) = (s + t mu) mod 3
     const uint32_t sm = s % 3;
@@ -632,10 +652,10 @@ struct Plan { int last, bm, split; };
 
 #define SBA_ELL_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 template <int BM> struct KTab {
-  // first pass: ELL levels over column tiles of 2^(13 - ELL) columns
+  // first pass: ELL levels over column tiles of 2^(SBA_FIRST_TILE_LG - ELL) columns
   static first_fn first_two(int ell) {
     switch (ell) {
-#define SBA_C(E) case E: return k_first_fwd<E, 13 - E, BM>;
+#define SBA_C(E) case E: return k_first_fwd<E, SBA_FIRST_TILE_LG - E, BM>;
       SBA_ELL_CASES(SBA_C)
 #undef SBA_C
     }
@@ -643,7 +663,7 @@ template <int BM> struct KTab {
   }
   static pass_fn inv_two(int ell) {
     switch (ell) {
-#define SBA_C(E) case E: return k_pass<E, 13 - E, true, BM>;
+#define SBA_C(E) case E: return k_pass<E, SBA_FIRST_TILE_LG - E, true, BM>;
       SBA_ELL_CASES(SBA_C)
 #undef SBA_C
     }
