@@ -216,7 +216,8 @@ int sba_autotune(const sba_job *jobs, int njobs, int mode, int topk, int reps, v
                  size_t ws_bytes, void *stream, int *measured);
 
 /* Cost model: an MLP (SBA_CM_IN -> 32 -> 32 -> 1, ReLU) over plan
- * features, predicting the standardised log launch time of a size class.
+ * features, predicting a plan's standardised log launch time relative to
+ * the other plans of its class (the autotuner only ranks plans).
  * Parameters: W1[32][16], b1[32], W2[32][32], b2[32], w3[32], b3
  * (SBA_CM_NPARAM floats). The training entries take device pointers and
  * enqueue on the stream:
@@ -227,7 +228,7 @@ int sba_autotune(const sba_job *jobs, int njobs, int mode, int topk, int reps, v
  * sba_cm_features writes the raw feature row of (class, plan) (host);
  * sba_cm_load installs host parameters plus the feature / target
  * standardisation for the autotuner; sba_cm_predict returns its estimate
- * in milliseconds. */
+ * as a relative cost (comparable between plans of one class). */
 #define SBA_CM_IN 16
 #define SBA_CM_NPARAM 1633
 int sba_cm_nparams(void);

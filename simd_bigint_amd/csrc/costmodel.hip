@@ -8,8 +8,8 @@
 //
 // Cost model: a multilayer perceptron over CM_IN plan features,
 //   f(x) = w3 . relu(W2 relu(W1 x + b1) + b2) + b3,
-// predicting the standardised log of a class's launch time from its
-// standardised features. Training (mean squared error, Adam) runs on the GPU
+// predicting the standardised log launch time of a plan relative to the
+// other plans of its class, from standardised features. Training (mean squared error, Adam) runs on the GPU
 // through the kernels below: k_cm_forward (activations), k_cm_delta
 // (back-propagated errors per sample), k_cm_grad (parameter gradients, one
 // thread per parameter, samples summed in a fixed order) and k_cm_adam.
@@ -277,7 +277,7 @@ extern "C" int sba_cm_load(const float *params, const float *fmean, const float 
 
 extern "C" int sba_cm_loaded(void) { return g_cm.loaded; }
 
-// predicted launch time (ms) of n (class, plan) rows given as raw features
+// relative predicted cost of n (class, plan) rows given as raw features
 // (n <= 64), evaluated by the forward kernel on `stream` (synchronizes)
 static int cm_predict(const float *feat, int n, double *ms, hipStream_t st) {
   if (!g_cm.loaded) return SBA_ERR_WORKSPACE;

@@ -46,9 +46,10 @@ struct GarnerArgs {
 #ifndef SBA_MERGE_LAST
 #define SBA_MERGE_LAST 1
 #endif
-// occupancy hints (waves per SIMD) of the merged last pass, the first
-// forward pass and the inverse column pass, and the unroll of the first
-// pass's limb loads, fixed by measurement
+// occupancy hints (waves per SIMD) of the merged last pass (radix-32 plans;
+// the radix-8/16 variants need more registers than 3 waves leave), the
+// first forward pass and the inverse column pass, and the unroll of the
+// first pass's limb loads, fixed by measurement
 #ifndef SBA_FUSED2_WAVES
 #define SBA_FUSED2_WAVES 3
 #endif
@@ -455,7 +456,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
 // levels. a's transformed tile is kept in registers (canonical, the
 // Montgomery operand) while b's is transformed in the same LDS.
 template <int ELL, int BMAX>
-__global__ __launch_bounds__(NT, SBA_FUSED2_WAVES) void k_fused2(PrimeArgs pa, JobRef jr, int L, int r3) {
+__global__ __launch_bounds__(NT, BMAX == 5 ? SBA_FUSED2_WAVES : SBA_WAVES) void k_fused2(PrimeArgs pa, JobRef jr, int L, int r3) {
   constexpr int NE = 1 << ELL;
   constexpr int E = NE / NT;
   __shared__ uint32_t lds[Pad<0>::words(NE)];
