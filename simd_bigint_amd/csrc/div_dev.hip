@@ -1,0 +1,365 @@
+// div_dev.hip -- the large division's Newton steps kept in HBM.
+//
+// The host division (host/sba_div.c) computes the quotient from one
+// approximate reciprocal (Brent & Zimmermann, Modern Computer Arithmetic,
+// Algorithm 3.5) and one quotient multiply. Above a size threshold this file
+// runs those steps on device buffers: every multiply is the GPU NTT on
+// device pointers, and the linear steps between them are kernels -- an
+// add/subtract with a carry-lookahead scan and the complement B^k - T (an
+// add of ~T and 1). Only the operands go up and the quotient estimate plus
+// the low dn + 1 limbs of the remainder come back; the reciprocal's
+// recursion below DIV_HOST_RECIP limbs and the rare corrections run on the
+// host.
+#include "sba_internal.h"
+
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <mutex>
+
+typedef uint64_t u64;
+
+#define DNT 256
+#define DLPT 8
+#define DLPB (DNT * DLPT)          // limbs per block
+#define DIV_HOST_RECIP 4096        // reciprocals of at most this many limbs: host
+
+// ---------------------------------------------------------------------------
+// add with carry: r = a + (sub ? ~b : b) + cin over n limbs (a or b may be
+// null = zero; b has bn <= n limbs, zero-extended). Carries are 0/1, so a
+// limb is a map {0,1} -> {0,1}: bit 0 = carry out for carry in 0, bit 1 =
+// for carry in 1; maps compose associatively.
+// ---------------------------------------------------------------------------
+#define M2_ID 2u
+__device__ __forceinline__ uint32_t m2_apply(uint32_t f, uint32_t c) { return (f >> c) & 1u; }
+__device__ __forceinline__ uint32_t m2_then(uint32_t f, uint32_t g) {    // f, then g
+  return m2_apply(g, f & 1u) | (m2_apply(g, (f >> 1) & 1u) << 1);
+}
+
+// block-wide exclusive scan of maps; *total = the block's composition
+__device__ uint32_t m2_block_scan(uint32_t f, uint32_t *total) {
+  __shared__ uint32_t wt[DNT / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t inc = f;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc = m2_then(o, inc);
+  }
+  uint32_t exc = __shfl_up(inc, 1, 64);
+  if (lane == 0) exc = M2_ID;
+  if (lane == 63) wt[wid] = inc;
+  __syncthreads();
+  uint32_t pre = M2_ID, tot = M2_ID;
+  for (int w = 0; w < DNT / 64; ++w) {
+    if (w < wid) pre = m2_then(pre, wt[w]);
+    tot = m2_then(tot, wt[w]);
+  }
+  *total = tot;
+  __syncthreads();
+  return m2_then(pre, exc);
+}
+
+// pass 1: limb sums, one map byte per limb, one map per block
+// (a or b may be the same buffer as s: each limb is read before it is written)
+__global__ __launch_bounds__(DNT) void k_add_maps(const u64 *a, const u64 *b, size_t bn, size_t n,
+                                                  int sub, u64 *s,
+                                                  uint8_t *__restrict__ lm,
+                                                  uint32_t *__restrict__ bm) {
+  __shared__ uint8_t mp[DLPB];
+  const size_t i0 = (size_t)blockIdx.x * DLPB;
+#pragma unroll
+  for (int k = 0; k < DLPT; ++k) {
+    const uint32_t j = threadIdx.x + k * DNT;
+    const size_t i = i0 + j;
+    uint32_t f = M2_ID;
+    if (i < n) {
+      const u64 x = a ? a[i] : 0;
+      u64 y = (b && i < bn) ? b[i] : 0;
+      if (sub) y = ~y;
+      const u64 t = x + y;
+      const uint32_t g = t < x, p = t == ~(u64)0;
+      f = g | ((g | p) << 1);
+      s[i] = t;
+    }
+    mp[j] = (uint8_t)f;
+  }
+  __syncthreads();
+  uint32_t f = M2_ID;
+#pragma unroll
+  for (int k = 0; k < DLPT; ++k) {
+    const uint32_t m = mp[threadIdx.x * DLPT + k];
+    lm[i0 + threadIdx.x * DLPT + k] = (uint8_t)m;
+    f = m2_then(f, m);
+  }
+  uint32_t tot;
+  m2_block_scan(f, &tot);
+  if (threadIdx.x == 0) bm[blockIdx.x] = tot;
+}
+
+// pass 2: carry into the block (earlier block maps composed, applied to
+// cin), carries into the limbs, r = s + c; *cout (if not null) = the carry
+// out of the top limb
+__global__ __launch_bounds__(DNT) void k_add_fix(const u64 *s, const uint8_t *__restrict__ lm,
+                                                 const uint32_t *__restrict__ bm, size_t n,
+                                                 uint32_t cin, u64 *r,
+                                                 uint32_t *__restrict__ cout) {
+  __shared__ uint8_t cj[DLPB];
+  const size_t i0 = (size_t)blockIdx.x * DLPB;
+  const uint32_t nprev = blockIdx.x;
+  const uint32_t per = (nprev + DNT - 1) / DNT;
+  uint32_t g = M2_ID;
+  for (uint32_t k = 0; k < per; ++k) {
+    const uint32_t idx = threadIdx.x * per + k;
+    if (idx < nprev) g = m2_then(g, bm[idx]);
+  }
+  uint32_t gall;
+  m2_block_scan(g, &gall);
+  uint32_t f = M2_ID;
+  uint8_t ms[DLPT];
+#pragma unroll
+  for (int k = 0; k < DLPT; ++k) {
+    const size_t i = i0 + threadIdx.x * DLPT + k;
+    ms[k] = i < n ? lm[i] : (uint8_t)M2_ID;
+    f = m2_then(f, ms[k]);
+  }
+  uint32_t tot;
+  const uint32_t pre = m2_block_scan(f, &tot);
+  uint32_t c = m2_apply(pre, m2_apply(gall, cin));
+#pragma unroll
+  for (int k = 0; k < DLPT; ++k) {
+    cj[threadIdx.x * DLPT + k] = (uint8_t)c;
+    c = m2_apply(ms[k], c);
+  }
+  if (cout && blockIdx.x == gridDim.x - 1 && threadIdx.x == DNT - 1) *cout = c;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < DLPT; ++k) {
+    const uint32_t j = threadIdx.x + k * DNT;
+    const size_t i = i0 + j;
+    if (i < n) r[i] = s[i] + cj[j];
+  }
+}
+
+// *flag |= 1 when any of x[0..n) is nonzero
+__global__ __launch_bounds__(DNT) void k_any_nonzero(const u64 *__restrict__ x, size_t n,
+                                                     uint32_t *__restrict__ flag) {
+  const size_t i = (size_t)blockIdx.x * DNT + threadIdx.x;
+  if (i < n && x[i] != 0) atomicOr(flag, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// host orchestration
+// ---------------------------------------------------------------------------
+#define DCHK(x)                                                                   \
+  do {                                                                            \
+    hipError_t e_ = (x);                                                          \
+    if (e_ != hipSuccess) {                                                       \
+      fprintf(stderr, "simd_bigint_amd: %s failed: %s\n", #x, hipGetErrorString(e_)); \
+      return SBA_ERR_HIP;                                                         \
+    }                                                                             \
+  } while (0)
+
+// bump allocator over one grow-only device buffer (u64 units)
+struct DivPool {
+  u64 *base = nullptr;
+  size_t cap = 0, top = 0;
+  uint8_t *lm = nullptr;            // add scratch: limb maps, block maps, carry word
+  uint32_t *bm = nullptr, *cw = nullptr;
+  size_t lmcap = 0;
+  u64 *take(size_t n) {
+    u64 *p = base + top;
+    top += (n + 63) & ~(size_t)63;
+    return top <= cap ? p : nullptr;
+  }
+};
+static DivPool g_dp;
+static std::mutex g_dp_mu;
+
+// r = a + (sub ? ~b : b) + cin over n limbs; *cout_host (if not null) gets
+// the carry out (synchronizes)
+static int dev_add(u64 *r, const u64 *a, const u64 *b, size_t bn, size_t n, int sub,
+                   uint32_t cin, uint32_t *cout_host, hipStream_t st) {
+  if (n == 0) return SBA_OK;
+  const size_t nb = (n + DLPB - 1) / DLPB;
+  if (nb * DLPB > g_dp.lmcap) return SBA_ERR_WORKSPACE;
+  hipLaunchKernelGGL(k_add_maps, dim3((unsigned)nb), dim3(DNT), 0, st, a, b, bn, n, sub, r,
+                     g_dp.lm, g_dp.bm);
+  hipLaunchKernelGGL(k_add_fix, dim3((unsigned)nb), dim3(DNT), 0, st, (const u64 *)r,
+                     (const uint8_t *)g_dp.lm, (const uint32_t *)g_dp.bm, n, cin, r,
+                     cout_host ? g_dp.cw : nullptr);
+  DCHK(hipGetLastError());
+  if (cout_host) {
+    DCHK(hipMemcpyAsync(cout_host, g_dp.cw, 4, hipMemcpyDeviceToHost, st));
+    DCHK(hipStreamSynchronize(st));
+  }
+  return SBA_OK;
+}
+
+static int dev_mul(u64 *r, const u64 *a, size_t an, const u64 *b, size_t bn, hipStream_t st) {
+  const int rc = sba_gpu_mul_dev(r, a, (int64_t)an, b, (int64_t)bn, st);
+  return rc == SBA_OK ? SBA_OK : (rc == SBA_BAND ? SBA_ERR_WORKSPACE : rc);
+}
+
+static u64 *host_buf(size_t n) {
+  u64 *p = (u64 *)malloc((n ? n : 1) * 8);
+  if (!p) {
+    fprintf(stderr, "simd_bigint_amd: out of host memory in division\n");
+    abort();
+  }
+  return p;
+}
+
+// X[0..n] (device) = approximate reciprocal of A[0..n) (device, top bit set)
+static int recip_dev(u64 *X, const u64 *A, size_t n, hipStream_t st) {
+  if (n <= DIV_HOST_RECIP) {
+    u64 *ha = host_buf(n), *hx = host_buf(n + 1);
+    DCHK(hipMemcpyAsync(ha, A, n * 8, hipMemcpyDeviceToHost, st));
+    DCHK(hipStreamSynchronize(st));
+    sba_approx_recip_host(hx, ha, n);
+    DCHK(hipMemcpyAsync(X, hx, (n + 1) * 8, hipMemcpyHostToDevice, st));
+    DCHK(hipStreamSynchronize(st));
+    free(ha);
+    free(hx);
+    return SBA_OK;
+  }
+  const size_t l = (n - 1) / 2, h = n - l;
+  const size_t mark = g_dp.top;
+  u64 *Xh = g_dp.take(h + 1);
+  u64 *T = g_dp.take(n + h + 1);
+  u64 *U = g_dp.take(3 * h + 1);
+  if (!Xh || !T || !U) return SBA_ERR_WORKSPACE;
+  int rc = recip_dev(Xh, A + l, h, st);
+  if (rc != SBA_OK) return rc;
+  // T = A * Xh (n + h + 1 limbs)
+  if ((rc = dev_mul(T, A, n, Xh, h + 1, st)) != SBA_OK) return rc;
+  u64 top = 0;
+  DCHK(hipMemcpyAsync(&top, T + n + h, 8, hipMemcpyDeviceToHost, st));
+  DCHK(hipStreamSynchronize(st));
+  if (top != 0) {
+    // T >= B^(n+h): Xh -= 1, T -= A until below (rare; on the host)
+    u64 *hx = host_buf(h + 1), *ht = host_buf(n + h + 1), *hA = host_buf(n);
+    DCHK(hipMemcpyAsync(hx, Xh, (h + 1) * 8, hipMemcpyDeviceToHost, st));
+    DCHK(hipMemcpyAsync(ht, T, (n + h + 1) * 8, hipMemcpyDeviceToHost, st));
+    DCHK(hipMemcpyAsync(hA, A, n * 8, hipMemcpyDeviceToHost, st));
+    DCHK(hipStreamSynchronize(st));
+    while (ht[n + h] != 0) {
+      for (size_t i = 0; i <= h && hx[i]-- == 0; ++i) {
+      }
+      u64 bw = 0;
+      for (size_t i = 0; i < n + h + 1; ++i) {
+        const u64 y = i < n ? hA[i] : 0;
+        const u64 x = ht[i];
+        const u64 d = x - y - bw;
+        bw = (x < y) || (x - y < bw);
+        ht[i] = d;
+      }
+    }
+    DCHK(hipMemcpyAsync(Xh, hx, (h + 1) * 8, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(T, ht, (n + h + 1) * 8, hipMemcpyHostToDevice, st));
+    DCHK(hipStreamSynchronize(st));
+    free(hx);
+    free(ht);
+    free(hA);
+  }
+  // T = B^(n+h) - T over n + h limbs: ~T + 1
+  if ((rc = dev_add(T, nullptr, T, n + h, n + h, 1, 1, nullptr, st)) != SBA_OK) return rc;
+  // U = floor(T / B^l) * Xh: 3h + 1 limbs
+  if ((rc = dev_mul(U, T + l, 2 * h, Xh, h + 1, st)) != SBA_OK) return rc;
+  // X = Xh * B^l + floor(U / B^(2h - l)) over n + 1 limbs
+  const size_t sh = 2 * h - l;
+  const size_t ul = 3 * h + 1 - sh < n + 1 ? 3 * h + 1 - sh : n + 1;
+  if (l) DCHK(hipMemcpyAsync(X, U + sh, l * 8, hipMemcpyDeviceToDevice, st));
+  if ((rc = dev_add(X + l, Xh, U + sh + l, ul > l ? ul - l : 0, n + 1 - l, 0, 0, nullptr, st)) !=
+      SBA_OK)
+    return rc;
+  g_dp.top = mark;
+  return SBA_OK;
+}
+
+static int grow_pool(size_t limbs, size_t addmax) {
+  if (g_dp.cap < limbs) {
+    if (g_dp.base) DCHK(hipFree(g_dp.base));
+    g_dp.base = nullptr;
+    g_dp.cap = 0;
+    const size_t want = limbs + limbs / 8 + 4096;
+    DCHK(hipMalloc(&g_dp.base, want * 8));
+    g_dp.cap = want;
+  }
+  const size_t lmneed = ((addmax + DLPB - 1) / DLPB + 1) * DLPB;
+  if (g_dp.lmcap < lmneed) {
+    if (g_dp.lm) DCHK(hipFree(g_dp.lm));
+    g_dp.lm = nullptr;
+    g_dp.lmcap = 0;
+    const size_t nb = lmneed / DLPB;
+    DCHK(hipMalloc(&g_dp.lm, lmneed + nb * 4 + 64));
+    g_dp.bm = (uint32_t *)(g_dp.lm + lmneed);
+    g_dp.cw = g_dp.bm + nb;
+    g_dp.lmcap = lmneed;
+  }
+  g_dp.top = 0;
+  return SBA_OK;
+}
+
+// Quotient estimate and remainder of the normalised division on the device
+// (the Newton branch of the host division): Nn has qn + dn limbs (the
+// dividend shifted so that Dn's top bit is set), Dm has m = qn + 1 limbs
+// (Dn's top limbs plus one, or Dn zero-extended), exact_pow = Dm is B^m - 1.
+// Writes Q[0..qn] (qn + 1 limbs) and R[0..dn] (dn + 1 limbs) = the low limbs
+// of Nn - Q * Dn; returns SBA_OK, or < 0 (device error), or SBA_BAND when the
+// estimate exceeds the dividend (the host then aborts, as it does itself).
+extern "C" int sba_div_dev(u64 *Q, u64 *R, const u64 *Nn, const u64 *Dn, size_t dn,
+                           const u64 *Dm, size_t qn, int exact_pow) {
+  std::lock_guard<std::mutex> lk(g_dp_mu);
+  const size_t m = qn + 1, nt1 = qn + dn;
+  hipStream_t st = nullptr;
+  // device buffers: Nn, Dn, Dm, X, P, QD/R, recursion (about 6m)
+  const size_t need = nt1 + dn + m + (m + 1) + (qn + 1 + m + 1) + nt1 + 8 * m + 64 * 64;
+  int rc = grow_pool(need, nt1 + 2 * m + 2);
+  if (rc != SBA_OK) return rc;
+  u64 *dN = g_dp.take(nt1), *dD = g_dp.take(dn), *dDm = g_dp.take(m), *dX = g_dp.take(m + 1);
+  u64 *dP = g_dp.take(qn + 1 + m + 1), *dQD = g_dp.take(nt1);
+  if (!dN || !dD || !dDm || !dX || !dP || !dQD) return SBA_ERR_WORKSPACE;
+  DCHK(hipMemcpyAsync(dN, Nn, nt1 * 8, hipMemcpyHostToDevice, st));
+  DCHK(hipMemcpyAsync(dD, Dn, dn * 8, hipMemcpyHostToDevice, st));
+  if (exact_pow) {
+    u64 *hx = host_buf(m + 1);
+    memset(hx, 0, (m + 1) * 8);
+    hx[m] = 1;
+    DCHK(hipMemcpyAsync(dX, hx, (m + 1) * 8, hipMemcpyHostToDevice, st));
+    DCHK(hipStreamSynchronize(st));
+    free(hx);
+  } else {
+    DCHK(hipMemcpyAsync(dDm, Dm, m * 8, hipMemcpyHostToDevice, st));
+    if ((rc = recip_dev(dX, dDm, m, st)) != SBA_OK) return rc;
+  }
+  // Qest = floor(Nt * X / B^(qn + 2)), Nt = Nn >> 64 (dn - 1): qn + 1 limbs
+  const size_t ntl = qn + 1;
+  if ((rc = dev_mul(dP, dN + (dn - 1), ntl, dX, m + 1, st)) != SBA_OK) return rc;
+  const u64 *dQ = dP + (qn + 2);
+  DCHK(hipMemcpyAsync(Q, dQ, (qn + 1) * 8, hipMemcpyDeviceToHost, st));
+  DCHK(hipStreamSynchronize(st));
+  if (Q[qn] != 0) return SBA_BAND;
+  // R = Nn - Qest * Dn over qn + dn limbs (the borrow-free case: carry out 1)
+  if ((rc = dev_mul(dQD, dQ, qn, dD, dn, st)) != SBA_OK) return rc;
+  uint32_t cout = 0;
+  if ((rc = dev_add(dQD, dN, dQD, nt1, nt1, 1, 1, &cout, st)) != SBA_OK) return rc;
+  if (!cout) return SBA_BAND;
+  // the remainder of an estimate at most 2 below Q is below 3 Dn < B^(dn+1):
+  // check that the limbs above are zero, then bring back the low ones
+  if (nt1 > dn + 1) {
+    DCHK(hipMemsetAsync(g_dp.cw, 0, 4, st));
+    const size_t hi = nt1 - (dn + 1);
+    hipLaunchKernelGGL(k_any_nonzero, dim3((unsigned)((hi + DNT - 1) / DNT)), dim3(DNT), 0, st,
+                       (const u64 *)(dQD + dn + 1), hi, g_dp.cw);
+    DCHK(hipGetLastError());
+    uint32_t any = 0;
+    DCHK(hipMemcpyAsync(&any, g_dp.cw, 4, hipMemcpyDeviceToHost, st));
+    DCHK(hipStreamSynchronize(st));
+    if (any) return SBA_BAND;
+  }
+  DCHK(hipMemcpyAsync(R, dQD, (dn + 1) * 8, hipMemcpyDeviceToHost, st));
+  DCHK(hipStreamSynchronize(st));
+  return SBA_OK;
+}

@@ -181,6 +181,8 @@ static void approx_recip(u64 *X, const u64 *A, size_t n) {
   free(Xh);
 }
 
+void sba_approx_recip_host(u64 *X, const u64 *A, size_t n) { approx_recip(X, A, n); }
+
 /* r = a << s (0 <= s < 64), n limbs in, writes n limbs + returns the spill */
 static u64 lshift(u64 *r, const u64 *a, size_t n, unsigned s) {
   if (s == 0) {
@@ -208,6 +210,7 @@ static u64 bitlen(const u64 *a, size_t n) {
 }
 
 #define SCHOOL_MAX 32   /* schoolbook when min(qn, dn) is at most this */
+#define DIV_DEV_MIN 8192 /* Newton steps on device buffers from this quotient size (limbs) */
 
 static void divrem_core(u64 *qp, u64 *rp, const u64 *np, u64 nn64, const u64 *dp,
                         u64 dn64, u64 *qn_out, u64 *rn_out) {
@@ -250,35 +253,55 @@ static void divrem_core(u64 *qp, u64 *rp, const u64 *np, u64 nn64, const u64 *dp
       memset(Dm, 0, (m - dn) * 8);
       memcpy(Dm + (m - dn), Dn, dn * 8);
     }
-    if (exact_pow) {
-      memset(X, 0, (m + 1) * 8);
-      X[m] = 1; /* B^2m / B^m */
+    if (m >= DIV_DEV_MIN && 2 * m + 1 >= sba_get_gpu_threshold() &&
+        sba_gpu_in_band((int64_t)m, (int64_t)m + 1) && sba_gpu_in_band((int64_t)qn, (int64_t)dn)) {
+      /* reciprocal, quotient estimate and remainder on device buffers;
+       * R comes back as its low dn + 1 limbs (the rest is zero) */
+      R = (u64 *)xmalloc((qn + dn) * 8);
+      memset(R, 0, (qn + dn) * 8);
+      const int rc = sba_div_dev(Q, R, Nn, Dn, dn, Dm, qn, exact_pow);
+      free(X);
+      free(Dm);
+      if (rc == SBA_BAND) {
+        fprintf(stderr, "simd_bigint_amd: quotient estimate out of bounds\n");
+        abort();
+      }
+      if (rc != SBA_OK) {
+        fprintf(stderr, "simd_bigint_amd: GPU division failed (code %d); the library has no "
+                        "CPU fallback for the GPU path\n", rc);
+        abort();
+      }
     } else {
-      approx_recip(X, Dm, m);
+      if (exact_pow) {
+        memset(X, 0, (m + 1) * 8);
+        X[m] = 1; /* B^2m / B^m */
+      } else {
+        approx_recip(X, Dm, m);
+      }
+      /* Qest = floor(Nt * X / B^(qn+2)), Nt = N' >> 64(dn-1): qn+1 limbs */
+      const size_t ntl = qn + 1;
+      u64 *P = (u64 *)xmalloc((ntl + m + 1) * 8);
+      sba_mul(P, Nn + (dn - 1), ntl, X, m + 1);
+      memcpy(Q, P + (qn + 2), (qn + 1) * 8); /* P has 2qn+3 limbs */
+      free(P);
+      free(X);
+      free(Dm);
+      if (Q[qn] != 0) {
+        fprintf(stderr, "simd_bigint_amd: quotient estimate overflow\n");
+        abort();
+      }
+      /* R = N' - Qest * D' (>= 0), then fix Qest upward */
+      u64 *QD = (u64 *)xmalloc((qn + dn) * 8);
+      size_t qt = trim(Q, qn);
+      if (qt) sba_mul(QD, Q, qt, Dn, dn);
+      memset(QD + (qt ? qt + dn : 0), 0, (qn + dn - (qt ? qt + dn : 0)) * 8);
+      R = (u64 *)xmalloc((qn + dn) * 8);
+      if (sub_n(R, Nn, QD, qn + dn)) {
+        fprintf(stderr, "simd_bigint_amd: quotient estimate too large\n");
+        abort();
+      }
+      free(QD);
     }
-    /* Qest = floor(Nt * X / B^(qn+2)), Nt = N' >> 64(dn-1): qn+1 limbs */
-    const size_t ntl = qn + 1;
-    u64 *P = (u64 *)xmalloc((ntl + m + 1) * 8);
-    sba_mul(P, Nn + (dn - 1), ntl, X, m + 1);
-    memcpy(Q, P + (qn + 2), (qn + 1) * 8); /* P has 2qn+3 limbs */
-    free(P);
-    free(X);
-    free(Dm);
-    if (Q[qn] != 0) {
-      fprintf(stderr, "simd_bigint_amd: quotient estimate overflow\n");
-      abort();
-    }
-    /* R = N' - Qest * D' (>= 0), then fix Qest upward */
-    u64 *QD = (u64 *)xmalloc((qn + dn) * 8);
-    size_t qt = trim(Q, qn);
-    if (qt) sba_mul(QD, Q, qt, Dn, dn);
-    memset(QD + (qt ? qt + dn : 0), 0, (qn + dn - (qt ? qt + dn : 0)) * 8);
-    R = (u64 *)xmalloc((qn + dn) * 8);
-    if (sub_n(R, Nn, QD, qn + dn)) {
-      fprintf(stderr, "simd_bigint_amd: quotient estimate too large\n");
-      abort();
-    }
-    free(QD);
     int fix = 0;
     for (;;) {
       /* R >= D' ? (R fits dn+1 limbs here) */

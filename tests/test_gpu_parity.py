@@ -178,6 +178,40 @@ def test_divrem_invariant_large():
     assert R < D and Q * D + R == N
 
 
+def _div_case(qn, dn, st, top=None, pattern=None):
+    """dividend of qn + dn - 1 limbs and divisor of dn limbs: the quotient
+    has qn limbs before normalisation (m = qn + 1 Newton limbs)."""
+    n = xorshift_fill(qn + dn - 1, st)
+    n[-1] |= np.uint64(1 << 63)
+    if pattern == "ones":          # D = B^dn - 1: the reciprocal is exact
+        d = np.full(dn, ~np.uint64(0), dtype=np.uint64)
+    elif pattern == "ones_m2":     # top qn+1 limbs of D plus one = B^(qn+1) - 2
+        d = np.full(dn, ~np.uint64(0), dtype=np.uint64)
+        d[dn - (qn + 1)] = np.uint64(0xFFFFFFFFFFFFFFFD)
+    else:
+        d = xorshift_fill(dn, st)
+        d[-1] = np.uint64(top) if top is not None else d[-1] | np.uint64(1 << 63)
+    return n, d
+
+
+@pytest.mark.parametrize("qn,dn,top,pattern,dc", [
+    (8191, 33, None, None, False),        # first size on device buffers
+    (8190, 33, None, None, False),        # last size on the host Newton
+    (40000, 40, None, None, False),       # long quotient, short divisor (chunked)
+    (10001, 20000, None, "ones", False),  # Dm + 1 = B^m: no reciprocal
+    (12000, 20000, None, "ones_m2", True),  # reciprocal products overflow: X corrections
+    (20000, 9000, 1, None, False),        # divisor top limb 1: shift by 63
+    (65536, 65536, None, None, True),
+])
+def test_divrem_newton_on_device(qn, dn, top, pattern, dc):
+    say("divrem newton", qn, dn, top, pattern, dc)
+    st = [qn * 7 + dn]
+    n, d = _div_case(qn, dn, st, top, pattern)
+    q, r = sba.divrem_u64(n, d, dc=dc)
+    N, D, Q, R = to_int(n), to_int(d), to_int(q), to_int(r)
+    assert R < D and Q * D + R == N
+
+
 def test_device_resident_matches_host():
     torch = pytest.importorskip("torch")
     st = [5]
