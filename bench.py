@@ -303,8 +303,9 @@ def configs_leg(L, reps=10):
     def sha(x):
         return hashlib.sha256(np.ascontiguousarray(x, dtype=np.uint64).tobytes()).hexdigest()
 
-    # divrem 2^21 / 2^20 limbs (divisor top bit set): Newton reciprocal on the
-    # GPU multiply; host buffers, so every multiply inside copies over PCIe
+    # divrem 2^21 / 2^20 limbs (divisor top bit set): Newton reciprocal,
+    # quotient estimate and remainder on device buffers; host buffers in and
+    # out (the operands go up once, the quotient and low remainder come back)
     e = [x for x in gold["divrem"] if x["nn"] == 1 << 21 and x["fn"].endswith("_dc")][0]
     st = [e["seed"]]
     nn_ = xorshift_fill(e["nn"], st)
@@ -319,7 +320,7 @@ def configs_leg(L, reps=10):
     out["cfg3_divrem_2^21/2^20"] = {
         "nn": e["nn"], "dn": e["dn"], "ms": round(best, 3),
         "bit_exact": bool(sha(q) == e["q_sha256"] and sha(r) == e["r_sha256"]),
-        "note": "host buffers; includes the PCIe copies of every multiply"}
+        "note": "host buffers; includes the operand and result copies over PCIe"}
     # the reference's divrem_u64_dc on one host core, same operands (once)
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libsbref.so")
     need = {"avx512f", "avx512bw", "avx512vl", "avx512dq", "avx512ifma", "avx512vbmi",
@@ -354,6 +355,15 @@ def configs_leg(L, reps=10):
                                      "bit_exact": bool(ok == 1 and sha(r) == want),
                                      "note": "PCIe-inclusive: 64 MiB in, 64 MiB out"}
     log("bench: host nt_mul", out["metric_via_host_nt_mul"])
+    # simd_mpn_mul's CPU/GPU crossover measured on this box (not installed)
+    tu = sba.tune_gpu_threshold(apply=False)
+    out["gpu_threshold"] = {"tuned_total_limbs": tu["threshold"],
+                            "default_total_limbs": sba.get_gpu_threshold(),
+                            "ladder": tu["totals"],
+                            "cpu_ms": [round(x, 4) for x in tu["cpu_ms"]],
+                            "gpu_ms": [round(x, 4) for x in tu["gpu_ms"]],
+                            "note": "balanced products on host buffers, GPU copies included"}
+    log("bench: gpu threshold", out["gpu_threshold"])
     return out
 
 

@@ -121,6 +121,13 @@ int sba_gpu_chunking(int64_t an, int64_t bn, int sqr, int *Lc, uint64_t *chunk,
 /* L when a single product of (an, bn) runs at the radix-3 length 3 * 2^L
  * (2^(L+1) < an+bn-1 <= 3 * 2^L, 13 <= L <= 22, not chunked), else 0 */
 int sba_gpu_radix3(int64_t an, int64_t bn, int sqr);
+/* The carry stage between 2048-limb blocks on its own (no counterpart in
+ * the reference; it lets a test drive the stage with chosen carry chains):
+ * rp[0..Z) holds limbs each finished with block carry-in 0, bmaps[b] the
+ * 2-state carry map of block b (bit c = carry out for carry in c); adds
+ * the carries between blocks in place. Enqueue only; SBA_OK, SBA_BAND (a
+ * null pointer, Z = 0 or Z > 2^29) or < 0. */
+int sba_gpu_carry_blocks(uint64_t *rp, const uint32_t *bmaps, uint64_t Z, void *stream);
 uint64_t sba_gpu_min_operand_cap(void);
 int sba_gpu_max_lg(void);
 int sba_gpu_device_count(void);
@@ -254,6 +261,14 @@ int sba_cm_predict(int L, int sqr, int njobs, double fill, int last, int bm, int
  * limbs to the GPU, smaller ones to the CPU Karatsuba. */
 void sba_set_gpu_threshold(uint64_t min_total_limbs);
 uint64_t sba_get_gpu_threshold(void);
+/* crossover tuner: times balanced products of each total in the ladder
+ * that sba_tune_sizes lists (host buffers; the CPU multiply against the GPU
+ * path with its copies, best of 5) and returns the smallest total from
+ * which the GPU wins at every larger ladder size (twice the largest if it
+ * never does); apply != 0 also sets it as the threshold. cpu_ms / gpu_ms
+ * (may be NULL) receive the times, one per ladder size. */
+int sba_tune_sizes(uint64_t *totals, int max);
+uint64_t sba_tune_gpu_threshold(int apply, double *cpu_ms, double *gpu_ms);
 /* per-kernel timing of the GPU path (HIP events on the launch stream):
  * enable, run, synchronize, collect; then read per-kernel totals (name,
  * milliseconds, algorithmic bytes, launches) by index until it returns 0. */

@@ -155,10 +155,17 @@ def lib() -> ctypes.CDLL:
     L.sba_gpu_chunking.restype = ctypes.c_int
     L.sba_gpu_radix3.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int]
     L.sba_gpu_radix3.restype = ctypes.c_int
+    L.sba_gpu_carry_blocks.argtypes = [vp, vp, u64, vp]
+    L.sba_gpu_carry_blocks.restype = ctypes.c_int
     L.sba_set_gpu_threshold.argtypes = [u64]
     L.sba_set_gpu_threshold.restype = None
     L.sba_get_gpu_threshold.argtypes = []
     L.sba_get_gpu_threshold.restype = u64
+    L.sba_tune_sizes.argtypes = [ctypes.POINTER(u64), ctypes.c_int]
+    L.sba_tune_sizes.restype = ctypes.c_int
+    L.sba_tune_gpu_threshold.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double)]
+    L.sba_tune_gpu_threshold.restype = u64
     _lib = L
     return L
 
@@ -238,6 +245,18 @@ def get_gpu_threshold() -> int:
     return int(lib().sba_get_gpu_threshold())
 
 
+def tune_gpu_threshold(apply: bool = False) -> dict:
+    """Measure the CPU/GPU crossover of simd_mpn_mul (balanced products on
+    host buffers, copies included) and optionally install it."""
+    n = lib().sba_tune_sizes(None, 0)
+    sizes = (ctypes.c_uint64 * n)()
+    lib().sba_tune_sizes(sizes, n)
+    cpu = (ctypes.c_double * n)()
+    gpu = (ctypes.c_double * n)()
+    t = int(lib().sba_tune_gpu_threshold(1 if apply else 0, cpu, gpu))
+    return {"threshold": t, "totals": list(sizes), "cpu_ms": list(cpu), "gpu_ms": list(gpu)}
+
+
 # ---------------------------------------------------------------------------
 # device-resident entries (torch tensors as device buffers)
 # ---------------------------------------------------------------------------
@@ -255,6 +274,19 @@ def mul_dev(r, a, b, stream=None) -> int:
     _check_dev(r, a.numel() + b.numel())
     s = None if stream is None else ctypes.c_void_p(int(stream.cuda_stream))
     return lib().sba_gpu_mul_dev(_ptr(r), _ptr(a), a.numel(), _ptr(b), b.numel(), s)
+
+
+def carry_blocks(r, bmaps, stream=None) -> int:
+    """The carry stage between 2048-limb blocks on device tensors: r holds
+    limbs finished with block carry-in 0, bmaps the blocks' 2-state carry
+    maps (ceil(len(r) / 2048) 4-byte words)."""
+    nb = (r.numel() + 2047) // 2048
+    if not (r.is_cuda and bmaps.is_cuda):
+        raise ValueError("carry_blocks needs device tensors")
+    if bmaps.numel() < nb or bmaps.element_size() != 4 or r.element_size() != 8:
+        raise ValueError("carry_blocks: r of 8-byte limbs, bmaps of nb 4-byte words")
+    s = None if stream is None else ctypes.c_void_p(int(stream.cuda_stream))
+    return lib().sba_gpu_carry_blocks(_ptr(r), _ptr(bmaps), r.numel(), s)
 
 
 def sqr_dev(r, a, stream=None) -> int:

@@ -149,6 +149,33 @@ __global__ __launch_bounds__(DNT) void k_any_nonzero(const u64 *__restrict__ x, 
   if (i < n && x[i] != 0) atomicOr(flag, 1u);
 }
 
+// dst[0..n] = src[0..n) << s (0 <= s < 64); dst[n] = the bits shifted out
+__global__ __launch_bounds__(DNT) void k_shl(u64 *__restrict__ dst, const u64 *__restrict__ src,
+                                             size_t n, unsigned s) {
+  const size_t i = (size_t)blockIdx.x * DNT + threadIdx.x;
+  if (i > n) return;
+  const u64 cur = i < n ? src[i] : 0, prev = i > 0 ? src[i - 1] : 0;
+  dst[i] = s ? (cur << s) | (prev >> (64 - s)) : cur;
+}
+
+// dst[0..n) = src[0..n) >> s (0 <= s < 64)
+__global__ __launch_bounds__(DNT) void k_shr(u64 *__restrict__ dst, const u64 *__restrict__ src,
+                                             size_t n, unsigned s) {
+  const size_t i = (size_t)blockIdx.x * DNT + threadIdx.x;
+  if (i >= n) return;
+  const u64 cur = src[i], next = i + 1 < n ? src[i + 1] : 0;
+  dst[i] = s ? (cur >> s) | (next << (64 - s)) : cur;
+}
+
+// *top = max(*top, i + 1) over the limbs where a[i] != b[i] (b zero-extended
+// from bn limbs): 0 when a and b are equal over n limbs
+__global__ __launch_bounds__(DNT) void k_top_diff(const u64 *__restrict__ a,
+                                                  const u64 *__restrict__ b, size_t bn, size_t n,
+                                                  unsigned long long *__restrict__ top) {
+  const size_t i = (size_t)blockIdx.x * DNT + threadIdx.x;
+  if (i < n && a[i] != (i < bn ? b[i] : 0)) atomicMax(top, (unsigned long long)(i + 1));
+}
+
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
@@ -168,6 +195,9 @@ struct DivPool {
   uint8_t *lm = nullptr;            // add scratch: limb maps, block maps, carry word
   uint32_t *bm = nullptr, *cw = nullptr;
   size_t lmcap = 0;
+  unsigned long long *word = nullptr;   // k_top_diff's result
+  u64 *pin = nullptr;               // pinned host staging (grow-only)
+  size_t pincap = 0;
   u64 *take(size_t n) {
     u64 *p = base + top;
     top += (n + 63) & ~(size_t)63;
@@ -298,68 +328,137 @@ static int grow_pool(size_t limbs, size_t addmax) {
     g_dp.cw = g_dp.bm + nb;
     g_dp.lmcap = lmneed;
   }
+  if (!g_dp.word) DCHK(hipMalloc(&g_dp.word, 64));
   g_dp.top = 0;
   return SBA_OK;
 }
 
-// Quotient estimate and remainder of the normalised division on the device
-// (the Newton branch of the host division): Nn has qn + dn limbs (the
-// dividend shifted so that Dn's top bit is set), Dm has m = qn + 1 limbs
-// (Dn's top limbs plus one, or Dn zero-extended), exact_pow = Dm is B^m - 1.
-// Writes Q[0..qn] (qn + 1 limbs) and R[0..dn] (dn + 1 limbs) = the low limbs
-// of Nn - Q * Dn; returns SBA_OK, or < 0 (device error), or SBA_BAND when the
-// estimate exceeds the dividend (the host then aborts, as it does itself).
-extern "C" int sba_div_dev(u64 *Q, u64 *R, const u64 *Nn, const u64 *Dn, size_t dn,
-                           const u64 *Dm, size_t qn, int exact_pow) {
+static int grow_pinned(size_t limbs) {
+  if (g_dp.pincap >= limbs) return SBA_OK;
+  if (g_dp.pin) DCHK(hipHostFree(g_dp.pin));
+  g_dp.pin = nullptr;
+  g_dp.pincap = 0;
+  const size_t want = limbs + limbs / 8 + 64;
+  DCHK(hipHostMalloc((void **)&g_dp.pin, want * 8, hipHostMallocDefault));
+  g_dp.pincap = want;
+  return SBA_OK;
+}
+
+static unsigned grid_for(size_t n) { return (unsigned)((n + DNT - 1) / DNT); }
+
+// *sign = sign of a[0..n) - b[0..bn) (bn <= n, b zero-extended); synchronizes
+static int dev_cmp(const u64 *a, size_t n, const u64 *b, size_t bn, int *sign, hipStream_t st) {
+  DCHK(hipMemsetAsync(g_dp.word, 0, 8, st));
+  hipLaunchKernelGGL(k_top_diff, dim3(grid_for(n)), dim3(DNT), 0, st, a, b, bn, n, g_dp.word);
+  DCHK(hipGetLastError());
+  unsigned long long top = 0;
+  DCHK(hipMemcpyAsync(&top, g_dp.word, 8, hipMemcpyDeviceToHost, st));
+  DCHK(hipStreamSynchronize(st));
+  if (top == 0) {
+    *sign = 0;
+    return SBA_OK;
+  }
+  const size_t i = (size_t)top - 1;
+  u64 x = 0, y = 0;
+  DCHK(hipMemcpyAsync(&x, a + i, 8, hipMemcpyDeviceToHost, st));
+  if (i < bn) DCHK(hipMemcpyAsync(&y, b + i, 8, hipMemcpyDeviceToHost, st));
+  DCHK(hipStreamSynchronize(st));
+  *sign = x > y ? 1 : -1;
+  return SBA_OK;
+}
+
+// The division's Newton branch on device buffers (host/sba_div.c decides
+// when): N has nt limbs (top limb nonzero), D has dn limbs (top limb
+// nonzero), s = the leading zero bits of D's top limb, qn = nt + 1 - dn.
+// On the device: N' = N << s (nt + 1 limbs), D' = D << s; the top qn + 1
+// limbs of D' plus one (or D' zero-extended) give Dm and its reciprocal X;
+// Qest = floor((N' >> 64 (dn - 1)) X / B^(qn + 2)); R = N' - Qest D'; while
+// R >= D': R -= D', Q += 1; R >>= s. Writes qp[0..min(qn, qm)) and
+// rp[0..dn). Returns SBA_OK, SBA_BAND when an estimate leaves its bounds
+// (the host then aborts, as its own Newton branch does), < 0 on a device
+// error.
+extern "C" int sba_div_dev(u64 *qp, size_t qm, u64 *rp, const u64 *np, size_t nt, const u64 *dp,
+                           size_t dn, unsigned s) {
   std::lock_guard<std::mutex> lk(g_dp_mu);
-  const size_t m = qn + 1, nt1 = qn + dn;
+  const size_t qn = nt + 1 - dn, m = qn + 1, nt1 = nt + 1;
   hipStream_t st = nullptr;
-  // device buffers: Nn, Dn, Dm, X, P, QD/R, recursion (about 6m)
-  const size_t need = nt1 + dn + m + (m + 1) + (qn + 1 + m + 1) + nt1 + 8 * m + 64 * 64;
+  // device buffers: raw N and D, N', D', Dm, X, P, QD / R, R >> s, recursion (about 6m)
+  const size_t need = nt + dn + nt1 + (dn + 1) + m + (m + 1) + (qn + 1 + m + 1) + nt1 + dn +
+                      8 * m + 64 * 64;
   int rc = grow_pool(need, nt1 + 2 * m + 2);
   if (rc != SBA_OK) return rc;
-  u64 *dN = g_dp.take(nt1), *dD = g_dp.take(dn), *dDm = g_dp.take(m), *dX = g_dp.take(m + 1);
-  u64 *dP = g_dp.take(qn + 1 + m + 1), *dQD = g_dp.take(nt1);
-  if (!dN || !dD || !dDm || !dX || !dP || !dQD) return SBA_ERR_WORKSPACE;
-  DCHK(hipMemcpyAsync(dN, Nn, nt1 * 8, hipMemcpyHostToDevice, st));
-  DCHK(hipMemcpyAsync(dD, Dn, dn * 8, hipMemcpyHostToDevice, st));
-  if (exact_pow) {
-    u64 *hx = host_buf(m + 1);
-    memset(hx, 0, (m + 1) * 8);
-    hx[m] = 1;
-    DCHK(hipMemcpyAsync(dX, hx, (m + 1) * 8, hipMemcpyHostToDevice, st));
-    DCHK(hipStreamSynchronize(st));
-    free(hx);
+  const size_t qc = qn < qm ? qn : qm;
+  if ((rc = grow_pinned(nt + dn > qc + dn ? nt + dn : qc + dn)) != SBA_OK) return rc;
+  u64 *dN0 = g_dp.take(nt), *dD0 = g_dp.take(dn), *dN = g_dp.take(nt1), *dD = g_dp.take(dn + 1);
+  u64 *dDm = g_dp.take(m), *dX = g_dp.take(m + 1), *dP = g_dp.take(qn + 1 + m + 1);
+  u64 *dQD = g_dp.take(nt1), *dRs = g_dp.take(dn);
+  if (!dN0 || !dD0 || !dN || !dD || !dDm || !dX || !dP || !dQD || !dRs) return SBA_ERR_WORKSPACE;
+  // operands up through pinned staging (the second host copy overlaps the first DMA)
+  sba_par_copy(g_dp.pin, np, nt * 8);
+  DCHK(hipMemcpyAsync(dN0, g_dp.pin, nt * 8, hipMemcpyHostToDevice, st));
+  sba_par_copy(g_dp.pin + nt, dp, dn * 8);
+  DCHK(hipMemcpyAsync(dD0, g_dp.pin + nt, dn * 8, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_shl, dim3(grid_for(nt + 1)), dim3(DNT), 0, st, dN, (const u64 *)dN0, nt, s);
+  hipLaunchKernelGGL(k_shl, dim3(grid_for(dn + 1)), dim3(DNT), 0, st, dD, (const u64 *)dD0, dn, s);
+  DCHK(hipGetLastError());
+  // Dm = the top m limbs of D' plus one (carry out: Dm + 1 = B^m), or D' zero-extended
+  int exact_pow = 0;
+  if (dn >= m) {
+    uint32_t cout = 0;
+    if ((rc = dev_add(dDm, dD + (dn - m), nullptr, 0, m, 0, 1, &cout, st)) != SBA_OK) return rc;
+    exact_pow = cout != 0;
   } else {
-    DCHK(hipMemcpyAsync(dDm, Dm, m * 8, hipMemcpyHostToDevice, st));
-    if ((rc = recip_dev(dX, dDm, m, st)) != SBA_OK) return rc;
+    DCHK(hipMemsetAsync(dDm, 0, (m - dn) * 8, st));
+    DCHK(hipMemcpyAsync(dDm + (m - dn), dD, dn * 8, hipMemcpyDeviceToDevice, st));
   }
-  // Qest = floor(Nt * X / B^(qn + 2)), Nt = Nn >> 64 (dn - 1): qn + 1 limbs
-  const size_t ntl = qn + 1;
-  if ((rc = dev_mul(dP, dN + (dn - 1), ntl, dX, m + 1, st)) != SBA_OK) return rc;
-  const u64 *dQ = dP + (qn + 2);
-  DCHK(hipMemcpyAsync(Q, dQ, (qn + 1) * 8, hipMemcpyDeviceToHost, st));
+  if (exact_pow) {                   // X = B^2m / B^m = B^m
+    const u64 one = 1;
+    DCHK(hipMemsetAsync(dX, 0, m * 8, st));
+    DCHK(hipMemcpyAsync(dX + m, &one, 8, hipMemcpyHostToDevice, st));
+    DCHK(hipStreamSynchronize(st));
+  } else if ((rc = recip_dev(dX, dDm, m, st)) != SBA_OK) {
+    return rc;
+  }
+  // Qest = floor(Nt * X / B^(qn + 2)), Nt = N' >> 64 (dn - 1): qn + 1 limbs
+  if ((rc = dev_mul(dP, dN + (dn - 1), qn + 1, dX, m + 1, st)) != SBA_OK) return rc;
+  u64 *dQ = dP + (qn + 2);
+  u64 qtop = 0;
+  DCHK(hipMemcpyAsync(&qtop, dQ + qn, 8, hipMemcpyDeviceToHost, st));
   DCHK(hipStreamSynchronize(st));
-  if (Q[qn] != 0) return SBA_BAND;
-  // R = Nn - Qest * Dn over qn + dn limbs (the borrow-free case: carry out 1)
+  if (qtop != 0) return SBA_BAND;
+  // R = N' - Qest D' over nt + 1 limbs (no borrow: carry out 1)
   if ((rc = dev_mul(dQD, dQ, qn, dD, dn, st)) != SBA_OK) return rc;
   uint32_t cout = 0;
   if ((rc = dev_add(dQD, dN, dQD, nt1, nt1, 1, 1, &cout, st)) != SBA_OK) return rc;
   if (!cout) return SBA_BAND;
-  // the remainder of an estimate at most 2 below Q is below 3 Dn < B^(dn+1):
-  // check that the limbs above are zero, then bring back the low ones
+  // the remainder of an estimate at most a few below Q is below B^(dn+1):
+  // the limbs above must be zero
   if (nt1 > dn + 1) {
     DCHK(hipMemsetAsync(g_dp.cw, 0, 4, st));
-    const size_t hi = nt1 - (dn + 1);
-    hipLaunchKernelGGL(k_any_nonzero, dim3((unsigned)((hi + DNT - 1) / DNT)), dim3(DNT), 0, st,
-                       (const u64 *)(dQD + dn + 1), hi, g_dp.cw);
+    hipLaunchKernelGGL(k_any_nonzero, dim3(grid_for(nt1 - (dn + 1))), dim3(DNT), 0, st,
+                       (const u64 *)(dQD + dn + 1), nt1 - (dn + 1), g_dp.cw);
     DCHK(hipGetLastError());
     uint32_t any = 0;
     DCHK(hipMemcpyAsync(&any, g_dp.cw, 4, hipMemcpyDeviceToHost, st));
     DCHK(hipStreamSynchronize(st));
     if (any) return SBA_BAND;
   }
-  DCHK(hipMemcpyAsync(R, dQD, (dn + 1) * 8, hipMemcpyDeviceToHost, st));
+  // while R >= D': R -= D', Q += 1 (R over dn + 1 limbs)
+  for (int fix = 0;; ++fix) {
+    int sign = 0;
+    if ((rc = dev_cmp(dQD, dn + 1, dD, dn, &sign, st)) != SBA_OK) return rc;
+    if (sign < 0) break;
+    if (fix >= 64) return SBA_BAND;
+    if ((rc = dev_add(dQD, dQD, dD, dn, dn + 1, 1, 1, nullptr, st)) != SBA_OK) return rc;
+    if ((rc = dev_add(dQ, dQ, nullptr, 0, qn + 1, 0, 1, nullptr, st)) != SBA_OK) return rc;
+  }
+  hipLaunchKernelGGL(k_shr, dim3(grid_for(dn)), dim3(DNT), 0, st, dRs, (const u64 *)dQD, dn, s);
+  DCHK(hipGetLastError());
+  // results down through pinned staging
+  DCHK(hipMemcpyAsync(g_dp.pin, dQ, qc * 8, hipMemcpyDeviceToHost, st));
+  DCHK(hipMemcpyAsync(g_dp.pin + qc, dRs, dn * 8, hipMemcpyDeviceToHost, st));
   DCHK(hipStreamSynchronize(st));
+  sba_par_copy(qp, g_dp.pin, qc * 8);
+  sba_par_copy(rp, g_dp.pin + qc, dn * 8);
   return SBA_OK;
 }

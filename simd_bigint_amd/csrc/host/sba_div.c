@@ -212,6 +212,14 @@ static u64 bitlen(const u64 *a, size_t n) {
 #define SCHOOL_MAX 32   /* schoolbook when min(qn, dn) is at most this */
 #define DIV_DEV_MIN 8192 /* Newton steps on device buffers from this quotient size (limbs) */
 
+/* the Newton branch runs on device buffers from DIV_DEV_MIN quotient limbs,
+ * when its products are above the GPU threshold and in the GPU's band */
+static int div_on_device(size_t qn, size_t dn) {
+  const size_t m = qn + 1;
+  return m >= DIV_DEV_MIN && 2 * m + 1 >= sba_get_gpu_threshold() &&
+         sba_gpu_in_band((int64_t)m, (int64_t)m + 1) && sba_gpu_in_band((int64_t)qn, (int64_t)dn);
+}
+
 static void divrem_core(u64 *qp, u64 *rp, const u64 *np, u64 nn64, const u64 *dp,
                         u64 dn64, u64 *qn_out, u64 *rn_out) {
   const u64 Dbits = bitlen(dp, dn64), Nbits = bitlen(np, nn64);
@@ -229,6 +237,27 @@ static void divrem_core(u64 *qp, u64 *rp, const u64 *np, u64 nn64, const u64 *dp
   const size_t dn = trim(dp, dn64), nt = trim(np, nn64);
   const unsigned s = (unsigned)__builtin_clzll(dp[dn - 1]);
   const size_t qn = nt + 1 - dn; /* Q < B^qn */
+  const u64 qm = nn64 - dn64 + 1;
+  if (qn > SCHOOL_MAX && dn > SCHOOL_MAX && div_on_device(qn, dn)) {
+    /* the whole Newton branch on device buffers: operands up, Q and R down */
+    const int rc = sba_div_dev(qp, (size_t)qm, rp, np, nt, dp, dn, s);
+    if (rc == SBA_BAND) {
+      fprintf(stderr, "simd_bigint_amd: quotient estimate out of bounds\n");
+      abort();
+    }
+    if (rc != SBA_OK) {
+      fprintf(stderr, "simd_bigint_amd: GPU division failed (code %d); the library has no "
+                      "CPU fallback for the GPU path\n", rc);
+      abort();
+    }
+    for (u64 i = qn; i < qm; ++i) qp[i] = 0;
+    u64 qt = trim(qp, qm);
+    *qn_out = qt ? qt : 1;
+    for (u64 i = dn; i < dn64; ++i) rp[i] = 0;
+    u64 rt = trim(rp, dn64);
+    *rn_out = rt ? rt : 1;
+    return;
+  }
   u64 *Dn = (u64 *)xmalloc(dn * 8);
   lshift(Dn, dp, dn, s);
   u64 *Nn = (u64 *)xmalloc((nt + 2) * 8); /* N << s: nt+1 limbs, +1 for the school path */
@@ -253,55 +282,35 @@ static void divrem_core(u64 *qp, u64 *rp, const u64 *np, u64 nn64, const u64 *dp
       memset(Dm, 0, (m - dn) * 8);
       memcpy(Dm + (m - dn), Dn, dn * 8);
     }
-    if (m >= DIV_DEV_MIN && 2 * m + 1 >= sba_get_gpu_threshold() &&
-        sba_gpu_in_band((int64_t)m, (int64_t)m + 1) && sba_gpu_in_band((int64_t)qn, (int64_t)dn)) {
-      /* reciprocal, quotient estimate and remainder on device buffers;
-       * R comes back as its low dn + 1 limbs (the rest is zero) */
-      R = (u64 *)xmalloc((qn + dn) * 8);
-      memset(R, 0, (qn + dn) * 8);
-      const int rc = sba_div_dev(Q, R, Nn, Dn, dn, Dm, qn, exact_pow);
-      free(X);
-      free(Dm);
-      if (rc == SBA_BAND) {
-        fprintf(stderr, "simd_bigint_amd: quotient estimate out of bounds\n");
-        abort();
-      }
-      if (rc != SBA_OK) {
-        fprintf(stderr, "simd_bigint_amd: GPU division failed (code %d); the library has no "
-                        "CPU fallback for the GPU path\n", rc);
-        abort();
-      }
+    if (exact_pow) {
+      memset(X, 0, (m + 1) * 8);
+      X[m] = 1; /* B^2m / B^m */
     } else {
-      if (exact_pow) {
-        memset(X, 0, (m + 1) * 8);
-        X[m] = 1; /* B^2m / B^m */
-      } else {
-        approx_recip(X, Dm, m);
-      }
-      /* Qest = floor(Nt * X / B^(qn+2)), Nt = N' >> 64(dn-1): qn+1 limbs */
-      const size_t ntl = qn + 1;
-      u64 *P = (u64 *)xmalloc((ntl + m + 1) * 8);
-      sba_mul(P, Nn + (dn - 1), ntl, X, m + 1);
-      memcpy(Q, P + (qn + 2), (qn + 1) * 8); /* P has 2qn+3 limbs */
-      free(P);
-      free(X);
-      free(Dm);
-      if (Q[qn] != 0) {
-        fprintf(stderr, "simd_bigint_amd: quotient estimate overflow\n");
-        abort();
-      }
-      /* R = N' - Qest * D' (>= 0), then fix Qest upward */
-      u64 *QD = (u64 *)xmalloc((qn + dn) * 8);
-      size_t qt = trim(Q, qn);
-      if (qt) sba_mul(QD, Q, qt, Dn, dn);
-      memset(QD + (qt ? qt + dn : 0), 0, (qn + dn - (qt ? qt + dn : 0)) * 8);
-      R = (u64 *)xmalloc((qn + dn) * 8);
-      if (sub_n(R, Nn, QD, qn + dn)) {
-        fprintf(stderr, "simd_bigint_amd: quotient estimate too large\n");
-        abort();
-      }
-      free(QD);
+      approx_recip(X, Dm, m);
     }
+    /* Qest = floor(Nt * X / B^(qn+2)), Nt = N' >> 64(dn-1): qn+1 limbs */
+    const size_t ntl = qn + 1;
+    u64 *P = (u64 *)xmalloc((ntl + m + 1) * 8);
+    sba_mul(P, Nn + (dn - 1), ntl, X, m + 1);
+    memcpy(Q, P + (qn + 2), (qn + 1) * 8); /* P has 2qn+3 limbs */
+    free(P);
+    free(X);
+    free(Dm);
+    if (Q[qn] != 0) {
+      fprintf(stderr, "simd_bigint_amd: quotient estimate overflow\n");
+      abort();
+    }
+    /* R = N' - Qest * D' (>= 0), then fix Qest upward */
+    u64 *QD = (u64 *)xmalloc((qn + dn) * 8);
+    size_t qt = trim(Q, qn);
+    if (qt) sba_mul(QD, Q, qt, Dn, dn);
+    memset(QD + (qt ? qt + dn : 0), 0, (qn + dn - (qt ? qt + dn : 0)) * 8);
+    R = (u64 *)xmalloc((qn + dn) * 8);
+    if (sub_n(R, Nn, QD, qn + dn)) {
+      fprintf(stderr, "simd_bigint_amd: quotient estimate too large\n");
+      abort();
+    }
+    free(QD);
     int fix = 0;
     for (;;) {
       /* R >= D' ? (R fits dn+1 limbs here) */
@@ -319,7 +328,6 @@ static void divrem_core(u64 *qp, u64 *rp, const u64 *np, u64 nn64, const u64 *dp
     }
   }
   /* outputs: Q as nn64-dn64+1 limbs, R >> s as dn64 limbs */
-  const u64 qm = nn64 - dn64 + 1;
   for (u64 i = 0; i < qm; ++i) qp[i] = i < qn ? Q[i] : 0;
   u64 qt = trim(qp, qm);
   *qn_out = qt ? qt : 1;

@@ -11,6 +11,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 typedef unsigned __int128 u128;
 
@@ -102,6 +103,8 @@ static void par_copy(void *dst, const void *src, size_t bytes) {
   }
   for (size_t t = 0; t < started; ++t) pthread_join(th[t], NULL);
 }
+
+void sba_par_copy(void *dst, const void *src, size_t bytes) { par_copy(dst, src, bytes); }
 
 /* H2D, multiply on the default stream, D2H. Returns the engine's code. */
 static int gpu_in_band(size_t an, size_t bn) {
@@ -374,6 +377,66 @@ static uint64_t g_gpu_min_total = SBA_GPU_MIN_TOTAL_DEFAULT;
 
 void sba_set_gpu_threshold(uint64_t min_total_limbs) { g_gpu_min_total = min_total_limbs; }
 uint64_t sba_get_gpu_threshold(void) { return g_gpu_min_total; }
+
+/* ------------------------------------------------------------------ */
+/* crossover tuner for the threshold above (the reference's tier       */
+/* thresholds come from its own sweeps, include/u52-mparam.h:1-19):    */
+/* balanced products on host buffers, this library's CPU multiply      */
+/* against the GPU path (copies included), best of a few runs each.    */
+/* ------------------------------------------------------------------ */
+static const uint64_t kTuneTotals[] = {256,  384,   512,   768,   1024,  1536,  2048,  3072, 4096,
+                                       6144, 8192, 12288, 16384, 24576, 32768, 49152, 65536};
+#define TUNE_N ((int)(sizeof kTuneTotals / sizeof kTuneTotals[0]))
+
+static double now_ms(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec * 1e3 + (double)ts.tv_nsec * 1e-6;
+}
+
+int sba_tune_sizes(uint64_t *totals, int max) {
+  for (int i = 0; i < TUNE_N && i < max; ++i) totals[i] = kTuneTotals[i];
+  return TUNE_N;
+}
+
+uint64_t sba_tune_gpu_threshold(int apply, double *cpu_ms, double *gpu_ms) {
+  const uint64_t top = kTuneTotals[TUNE_N - 1];
+  uint64_t *a = (uint64_t *)malloc(top * 8), *r = (uint64_t *)malloc(top * 8);
+  if (!a || !r) die("malloc", 0);
+  uint64_t x = 0x9E3779B97F4A7C15ull;
+  for (uint64_t i = 0; i < top; ++i) {
+    x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+    a[i] = x;
+  }
+  int gpu_wins_from = TUNE_N;       /* first index from which the GPU always wins */
+  for (int k = 0; k < TUNE_N; ++k) {
+    const uint64_t T = kTuneTotals[k], an = T / 2, bn = T - an;
+    const uint64_t *b = a + an;
+    double bc = 1e30, bg = 1e30;
+    if (gpu_mul_host(r, a, an, b, bn, 0) != SBA_OK) die("GPU multiply in the tuner", SBA_BAND);
+    for (int rep = 0; rep < 5; ++rep) {
+      double t0 = now_ms();
+      cpu_mul(r, a, an, b, bn);
+      double t1 = now_ms();
+      if (gpu_mul_host(r, a, an, b, bn, 0) != SBA_OK) die("GPU multiply in the tuner", SBA_BAND);
+      double t2 = now_ms();
+      if (t1 - t0 < bc) bc = t1 - t0;
+      if (t2 - t1 < bg) bg = t2 - t1;
+    }
+    if (cpu_ms) cpu_ms[k] = bc;
+    if (gpu_ms) gpu_ms[k] = bg;
+    if (bg < bc) {
+      if (gpu_wins_from == TUNE_N) gpu_wins_from = k;
+    } else {
+      gpu_wins_from = TUNE_N;
+    }
+  }
+  free(a);
+  free(r);
+  const uint64_t t = gpu_wins_from < TUNE_N ? kTuneTotals[gpu_wins_from] : 2 * top;
+  if (apply) g_gpu_min_total = t;
+  return t;
+}
 
 void simd_mpn_mul(uint64_t *rp, const uint64_t *ap, uint64_t an, const uint64_t *bp,
                   uint64_t bn) {

@@ -21,13 +21,16 @@ void sba_mul(uint64_t *rp, const uint64_t *ap, size_t an, const uint64_t *bp, si
  * A*X < B^2n <= A*(X+2), B = 2^64 (host Newton, sba_div.c) */
 void sba_approx_recip_host(uint64_t *X, const uint64_t *A, size_t n);
 
-/* the Newton branch of the division on device buffers (div_dev.hip): from
- * the normalised Nn (qn + dn limbs), Dn (dn limbs) and Dm (qn + 1 limbs; or
- * exact_pow when Dm + 1 = B^(qn+1)), Q[0..qn] = the quotient estimate and
- * R[0..dn] = the low limbs of Nn - Q * Dn. SBA_OK; SBA_BAND if the estimate
- * is out of its bounds; < 0 on a device error. */
-int sba_div_dev(uint64_t *Q, uint64_t *R, const uint64_t *Nn, const uint64_t *Dn, size_t dn,
-                const uint64_t *Dm, size_t qn, int exact_pow);
+/* memcpy split over several host threads (sba_host.c) */
+void sba_par_copy(void *dst, const void *src, size_t bytes);
+
+/* the division's Newton branch on device buffers (div_dev.hip): N of nt
+ * limbs and D of dn limbs (top limbs nonzero), s = leading zero bits of D's
+ * top limb, quotient of qn = nt + 1 - dn limbs. Writes qp[0..min(qn, qm))
+ * and rp[0..dn). SBA_OK; SBA_BAND if an estimate leaves its bounds; < 0 on
+ * a device error. */
+int sba_div_dev(uint64_t *qp, size_t qm, uint64_t *rp, const uint64_t *np, size_t nt,
+                const uint64_t *dp, size_t dn, unsigned s);
 
 #ifdef __cplusplus
 }

@@ -22,8 +22,9 @@
 //     Inside the tile, CHUNKS of up to 5 levels run in registers (radix-32
 //     sub-butterflies per thread), with LDS exchanges between chunks.
 //   * Garner over the five residues (ascending primes) composes each
-//     coefficient into 3 u64 words; a 3-kernel carry-lookahead scan over a
-//     4-state carry map adds the words into limbs bit-exactly.
+//     coefficient into 3 u64 words; the words are summed into limbs and a
+//     binary carry-lookahead (ballot masks in a block, a scan of 2-state
+//     block maps across blocks) finishes the limbs bit-exactly.
 #include "ntt_tiles.h"
 #include "plan_table.h"
 
@@ -55,171 +56,220 @@ __device__ __forceinline__ void garner_coef(const GarnerArgs &ga, gc_u32 *spec,
     const uint32_t d = red1(u[q] + p - t, p);
     x[q] = red1(mont_mul(d, ga.im[q], p, pinv), p);
   }
-  // Horner: c = (((x4 p3 + x3) p2 + x2) p1 + x1) p0 + x0  (< 2^155)
-  unsigned __int128 c = x[4];
-  c = c * ga.p[3] + x[3];
-  c = c * ga.p[2] + x[2];
-  c = c * ga.p[1] + x[1];
-  const unsigned __int128 t0 = (unsigned __int128)(uint64_t)c * ga.p[0] + x[0];
-  const unsigned __int128 t1 =
-      (unsigned __int128)(uint64_t)(c >> 64) * ga.p[0] + (uint64_t)(t0 >> 64);
-  c0 = (uint64_t)t0;
-  c1 = (uint64_t)t1;
-  c2 = (uint64_t)(t1 >> 64);
+  // Horner: c = (((x4 p3 + x3) p2 + x2) p1 + x1) p0 + x0  (< 2^155) on
+  // 32-bit words, one 32 x 32 + 64-bit multiply-add per word and step
+  uint32_t w[SBA_K];
+  w[0] = x[4];
+#pragma unroll
+  for (int s = 0; s < SBA_K - 1; ++s) {
+    const uint32_t P = ga.p[3 - s];
+    uint64_t t = x[3 - s];
+#pragma unroll
+    for (int i = 0; i <= s; ++i) {
+      t += (uint64_t)w[i] * P;                   // < 2^63 + 2^32
+      w[i] = (uint32_t)t;
+      t >>= 32;
+    }
+    w[s + 1] = (uint32_t)t;
+  }
+  c0 = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+  c1 = (uint64_t)w[2] | ((uint64_t)w[3] << 32);
+  c2 = w[4];
 }
 
 // ---------------------------------------------------------------------------
 // carry propagation. Limb j receives s_j = w0[j] + w1[j-1] + w2[j-2]
-// (< 3 * 2^64) = v_j + 2^64 h_j. With incoming carry c in {0..3}:
-// out_j = (v_j + c) mod 2^64 and c' = h_j + [c >= 2^64 - v_j] <= 3. A limb
-// is thus a map {0..3} -> {0..3} (2 bits per entry, 8 bits); maps compose
-// associatively, so the carries are an exclusive scan of maps:
-//   k_garner_maps  Garner + limb sums + limb maps + block maps (LDS-staged)
-//   k_carry_fix    carry into the block (earlier block maps composed),
-//                  per-limb carries from the limb maps, out_j = v_j + c_j
+// (< 3 * 2^64) = lo_j + 2^64 hi_j, hi_j <= 2. The high part moves one limb
+// up: t_j = lo_j + hi_{j-1} (mod 2^64) with g_j = [it wrapped]. What is left
+// is a plain binary carry chain, c_{j+1} = g_j | (p_j & c_j) with p_j =
+// [t_j = 2^64 - 1] (g_j and p_j never both hold: a wrapped t_j is < 2), and
+// out_j = t_j + c_j. On a 64-limb run held by one wave the chain is an
+// integer addition of ballot masks: with A = G | P and B = G, the carries
+// into the limbs are (A + B + c_in) ^ A ^ B. Carries leave a run or a block
+// only on a generate followed by all-ones limbs, so for most data every
+// block's carry-in is 0.
+//   k_garner_limbs  Garner + limb words; carries inside each 2048-limb
+//                   block (block carry-in 0); the block's 2-state carry map
+//   k_carry_blocks  the block carry-ins (a scan of the block maps), added
+//                   where they are 1 (usually to one limb)
 // ---------------------------------------------------------------------------
-#define CARRY_LPT 8                     // limbs per thread in the map scan
+#define CARRY_LPT 8                     // limbs per thread
 #define CARRY_LPB (CARRY_LPT * NT)      // limbs per block
-#define MAP_ID 0xE4u                    // c -> c
+#define CARRY_RUNS (CARRY_LPB / 64)     // 64-limb runs per block
+#define SCAN_NT 1024                    // k_carry_blocks threads
+#define CARRY_WGS 256                   // k_carry_blocks workgroups per product (at most)
+#define C2_ID 2u                        // 2-state map c -> c: bit c = carry out for carry in c
 
-__device__ __forceinline__ uint32_t map_apply(uint32_t f, uint32_t c) { return (f >> (2 * c)) & 3u; }
+__device__ __forceinline__ uint32_t c2_apply(uint32_t f, uint32_t c) { return (f >> c) & 1u; }
 // first f, then g
-__device__ __forceinline__ uint32_t map_then(uint32_t f, uint32_t g) {
-  uint32_t r = 0;
-#pragma unroll
-  for (uint32_t c = 0; c < 4; ++c) r |= map_apply(g, map_apply(f, c)) << (2 * c);
-  return r;
-}
-__device__ __forceinline__ uint32_t limb_map(uint64_t v, uint32_t h) {
-  const uint64_t room = ~v;                       // 2^64 - 1 - v
-  const uint32_t z = room < 3 ? (uint32_t)room + 1 : 4u;   // overflow iff c >= z
-  uint32_t f = 0;
-#pragma unroll
-  for (uint32_t c = 0; c < 4; ++c) f |= (h + (c >= z ? 1u : 0u)) << (2 * c);
-  return f;
+__device__ __forceinline__ uint32_t c2_then(uint32_t f, uint32_t g) {
+  return c2_apply(g, f & 1u) | (c2_apply(g, (f >> 1) & 1u) << 1);
 }
 
-// block-wide exclusive scan of maps; returns the exclusive prefix of this
-// thread and writes the block total into *total (all threads).
-__device__ uint32_t block_scan_maps(uint32_t f, uint32_t *total) {
-  __shared__ uint32_t wtot[NT / 64];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  uint32_t inc = f;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t o = __shfl_up(inc, d, 64);
-    if (lane >= d) inc = map_then(o, inc);
-  }
-  uint32_t exc = __shfl_up(inc, 1, 64);
-  if (lane == 0) exc = MAP_ID;
-  if (lane == 63) wtot[wid] = inc;
-  __syncthreads();
-  uint32_t pre = MAP_ID;
-  for (int w = 0; w < wid; ++w) pre = map_then(pre, wtot[w]);
-  uint32_t tot = MAP_ID;
-  for (int w = 0; w < NT / 64; ++w) tot = map_then(tot, wtot[w]);
-  *total = tot;
-  __syncthreads();
-  return map_then(pre, exc);
-}
-
-__global__ __launch_bounds__(NT) void k_garner_maps(GarnerArgs ga, JobRef jr, uint32_t N) {
-  __shared__ uint64_t ws0[CARRY_LPB + 2], ws1[CARRY_LPB + 2], ws2[CARRY_LPB + 2];
-  __shared__ uint64_t mp64[CARRY_LPB / 8];
+__global__ __launch_bounds__(NT) void k_garner_limbs(GarnerArgs ga, JobRef jr, uint32_t N) {
+  // slot j + 3 holds the words of coefficient B0 + j (slots 0..2: B0-3..B0-1)
+  __shared__ uint64_t ws0[CARRY_LPB + 3], ws1[CARRY_LPB + 3], ws2[CARRY_LPB + 3];
+  __shared__ uint32_t rmap[CARRY_RUNS], rcin[CARRY_RUNS];
   const Job jb = get_job(jr);
   if (blockIdx.x >= jb.nb) return;                 // this job has fewer carry blocks
   gc_u32 *spec = (gc_u32 *)jb.bspec;
   const uint32_t zlen = jb.zlen, Z = jb.Z;
   g_u64 *rp = (g_u64 *)jb.r;
-  uint8_t *mp = (uint8_t *)mp64;
   const uint32_t B0 = blockIdx.x * CARRY_LPB;
-  // coefficients B0-2 .. B0+CARRY_LPB-1 -> LDS slot j+2 (coalesced residue reads)
 #pragma unroll 2
   for (int k = 0; k < CARRY_LPT; ++k) {
     const uint32_t j = threadIdx.x + k * NT;
     const uint32_t i = B0 + j;
     uint64_t c0 = 0, c1 = 0, c2 = 0;
     if (i < zlen) garner_coef(ga, spec, N, i, c0, c1, c2);
-    ws0[j + 2] = c0; ws1[j + 2] = c1; ws2[j + 2] = c2;
+    ws0[j + 3] = c0; ws1[j + 3] = c1; ws2[j + 3] = c2;
   }
-  if (threadIdx.x < 2) {
-    const uint32_t j = threadIdx.x;            // slot j holds coefficient B0 - 2 + j
+  if (threadIdx.x < 3) {
+    const uint32_t j = threadIdx.x;            // coefficient B0 - 3 + j
     uint64_t c0 = 0, c1 = 0, c2 = 0;
-    if (B0 >= 2 - j) {
-      const uint32_t i = B0 - 2 + j;
+    if (B0 + j >= 3) {
+      const uint32_t i = B0 + j - 3;
       if (i < zlen) garner_coef(ga, spec, N, i, c0, c1, c2);
     }
     ws0[j] = c0; ws1[j] = c1; ws2[j] = c2;
   }
   __syncthreads();
-  // limb sums (coalesced writes of v_j into the product), limb maps in LDS
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint64_t t[CARRY_LPT], ra[CARRY_LPT], rb[CARRY_LPT];
 #pragma unroll
   for (int k = 0; k < CARRY_LPT; ++k) {
     const uint32_t j = threadIdx.x + k * NT;
-    const uint32_t Lb = B0 + j;
-    uint32_t f = MAP_ID;
-    if (Lb < Z) {
-      const uint64_t a = ws0[j + 2], b = ws1[j + 1], c = ws2[j];
-      const uint64_t s1 = a + b;
-      uint32_t h = s1 < a ? 1u : 0u;
-      const uint64_t s2 = s1 + c;
-      h += s2 < s1 ? 1u : 0u;
-      rp[Lb] = s2;
-      f = limb_map(s2, h);
+    // s_j from slots j+3, j+2, j+1; s_{j-1} from j+2, j+1, j (its high part)
+    const uint64_t a = ws0[j + 3], b = ws1[j + 2], c = ws2[j + 1];
+    const uint64_t s1 = a + b;
+    const uint64_t lo = s1 + c;
+    const uint64_t ap = ws0[j + 2], bp = ws1[j + 1], cp = ws2[j];
+    const uint64_t q1 = ap + bp;
+    const uint32_t hp = (q1 < ap ? 1u : 0u) + (q1 + cp < q1 ? 1u : 0u);
+    const uint64_t tt = lo + hp;
+    bool g = tt < lo, pr = tt == ~(uint64_t)0;
+    if (B0 + j >= Z) g = false, pr = true;        // past the product: carry passes
+    t[k] = tt;
+    const uint64_t G = __ballot(g), P = __ballot(pr);
+    ra[k] = G | P;
+    rb[k] = G;
+    if (lane == 0) {
+      const uint64_t sm = ra[k] + rb[k];
+      const uint32_t c0 = sm < ra[k] ? 1u : 0u;
+      const uint32_t c1 = c0 | (sm == ~(uint64_t)0 ? 1u : 0u);
+      rmap[k * (NT / 64) + wid] = c0 | (c1 << 1);   // runs in limb order
     }
-    mp[j] = (uint8_t)f;
   }
   __syncthreads();
-  // each thread composes its 8 consecutive limb maps; block total
-  const uint64_t m8 = mp64[threadIdx.x];
-  *(g_u64 *)((g_u8 *)jb.lmaps + (size_t)B0 + threadIdx.x * CARRY_LPT) = m8;
-  uint32_t f = MAP_ID;
+  // carry into each run (block carry-in 0), and the block's map
+  if (wid == 0) {
+    const uint32_t f = lane < CARRY_RUNS ? rmap[lane] : C2_ID;
+    uint32_t inc = f;
 #pragma unroll
-  for (int k = 0; k < CARRY_LPT; ++k) f = map_then(f, (uint32_t)(m8 >> (8 * k)) & 0xFFu);
-  uint32_t tot;
-  block_scan_maps(f, &tot);
-  if (threadIdx.x == 0) ((g_u32 *)jb.bmaps)[blockIdx.x] = tot;
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t o = __shfl_up(inc, d, 64);
+      if (lane >= d) inc = c2_then(o, inc);
+    }
+    uint32_t exc = __shfl_up(inc, 1, 64);
+    if (lane == 0) exc = C2_ID;
+    if (lane < CARRY_RUNS) rcin[lane] = c2_apply(exc, 0);
+    if (lane == 63) ((g_u32 *)jb.bmaps)[blockIdx.x] = inc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < CARRY_LPT; ++k) {
+    const uint32_t j = threadIdx.x + k * NT;
+    const uint64_t cin = rcin[k * (NT / 64) + wid];
+    const uint64_t C = (ra[k] + rb[k] + cin) ^ ra[k] ^ rb[k];
+    if (B0 + j < Z) rp[B0 + j] = t[k] + ((C >> lane) & 1u);
+  }
 }
 
-__global__ __launch_bounds__(NT) void k_carry_fix(JobRef jr) {
-  __shared__ uint64_t cj64[CARRY_LPB / 8];
-  const Job jb = get_job(jr);
-  if (blockIdx.x >= jb.nb) return;
-  g_u64 *rp = (g_u64 *)jb.r;
-  const uint32_t Z = jb.Z;
-  uint8_t *cj = (uint8_t *)cj64;
-  const uint32_t B0 = blockIdx.x * CARRY_LPB;
-  const uint64_t m8 = *(gc_u64 *)((g_u8 *)jb.lmaps + (size_t)B0 + threadIdx.x * CARRY_LPT);
-  uint32_t f = MAP_ID;
+// workgroup-wide exclusive scan of 2-state maps in thread order; *total =
+// the composition of all of them
+__device__ uint32_t c2_block_scan(uint32_t f, uint32_t *total) {
+  __shared__ uint32_t wt[SCAN_NT / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t inc = f;
 #pragma unroll
-  for (int k = 0; k < CARRY_LPT; ++k) f = map_then(f, (uint32_t)(m8 >> (8 * k)) & 0xFFu);
-  uint32_t tot;
-  const uint32_t pre = block_scan_maps(f, &tot);
-  // carry into this block: every earlier block's map, composed in order and
-  // applied to 0 (each thread composes a run of them, the block scan joins
-  // the runs; the block maps are few and cache-resident)
-  gc_u32 *bm = (gc_u32 *)jb.bmaps;
-  const uint32_t nprev = blockIdx.x;
-  const uint32_t per = (nprev + NT - 1) / NT;
-  const uint32_t s0 = threadIdx.x * per;
-  uint32_t g = MAP_ID;
-  for (uint32_t k = 0; k < per; ++k)
-    if (s0 + k < nprev) g = map_then(g, bm[s0 + k]);
-  uint32_t gall;
-  block_scan_maps(g, &gall);
-  uint32_t c = map_apply(pre, map_apply(gall, 0));
-#pragma unroll
-  for (int k = 0; k < CARRY_LPT; ++k) {
-    cj[threadIdx.x * CARRY_LPT + k] = (uint8_t)c;
-    c = map_apply((uint32_t)(m8 >> (8 * k)) & 0xFFu, c);
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc = c2_then(o, inc);
   }
+  uint32_t exc = __shfl_up(inc, 1, 64);
+  if (lane == 0) exc = C2_ID;
+  if (lane == 63) wt[wid] = inc;
   __syncthreads();
-#pragma unroll
-  for (int k = 0; k < CARRY_LPT; ++k) {
-    const uint32_t j = threadIdx.x + k * NT;
-    const uint32_t Lb = B0 + j;
-    const uint32_t cc = cj[j];
-    if (Lb < Z && cc) rp[Lb] += cc;
+  uint32_t pre = C2_ID, tot = C2_ID;
+  for (int w = 0; w < SCAN_NT / 64; ++w) {
+    if (w < wid) pre = c2_then(pre, wt[w]);
+    tot = c2_then(tot, wt[w]);
+  }
+  *total = tot;
+  __syncthreads();
+  return c2_then(pre, exc);
+}
+
+// carries between blocks (grid x = up to CARRY_WGS workgroups per product,
+// grid z = job). Workgroup g owns a contiguous range of blocks: it composes
+// the maps of all earlier blocks for the carry into its range, scans its
+// range's maps, and adds each carry-in of 1 to its block's limbs (computed
+// with carry-in 0) -- usually to the first limb only; a block whose first
+// limb is all ones gets the whole workgroup (the limbs below the first one
+// that is not all ones wrap to zero)
+__global__ __launch_bounds__(SCAN_NT) void k_carry_blocks(JobRef jr) {
+  __shared__ uint32_t list[SCAN_NT];
+  __shared__ uint32_t first, nlist;
+  const Job jb = get_job(jr);
+  const uint32_t nb = jb.nb, Z = jb.Z;
+  const uint32_t span = (nb + gridDim.x - 1) / gridDim.x;
+  const uint32_t r0 = blockIdx.x * span;
+  if (r0 >= nb) return;                              // uniform
+  const uint32_t r1 = r0 + span < nb ? r0 + span : nb;
+  gc_u32 *bm = (gc_u32 *)jb.bmaps;
+  g_u64 *rp = (g_u64 *)jb.r;
+  const uint32_t per = (r0 + SCAN_NT - 1) / SCAN_NT;
+  const uint32_t s0 = threadIdx.x * per, s1 = s0 + per < r0 ? s0 + per : r0;
+  uint32_t f = C2_ID;
+  for (uint32_t i = s0; i < s1; ++i) f = c2_then(f, bm[i]);
+  uint32_t tot;
+  c2_block_scan(f, &tot);
+  uint32_t c = c2_apply(tot, 0);                     // carry into block r0
+  for (uint32_t base = r0; base < r1; base += SCAN_NT) {
+    const uint32_t i = base + threadIdx.x;
+    uint32_t ctot;
+    const uint32_t exc = c2_block_scan(i < r1 ? bm[i] : C2_ID, &ctot);
+    const uint32_t cin = c2_apply(exc, c);
+    c = c2_apply(ctot, c);
+    if (threadIdx.x == 0) nlist = 0;
+    __syncthreads();
+    if (i < r1 && cin) {
+      g_u64 *x = rp + (size_t)i * CARRY_LPB;
+      const uint64_t v = *x;
+      if (v != ~(uint64_t)0) *x = v + 1;
+      else list[atomicAdd(&nlist, 1u)] = i;
+    }
+    __syncthreads();
+    const uint32_t nl = nlist;
+    for (uint32_t e = 0; e < nl; ++e) {
+      const uint32_t B0 = list[e] * CARRY_LPB;
+      const uint32_t E = Z - B0 < CARRY_LPB ? Z - B0 : CARRY_LPB;
+      if (threadIdx.x == 0) first = E;
+      __syncthreads();
+      for (uint32_t j = threadIdx.x; j < E; j += SCAN_NT)
+        if (rp[B0 + j] != ~(uint64_t)0) {
+          atomicMin(&first, j);
+          break;
+        }
+      __syncthreads();
+      const uint32_t fz = first;
+      for (uint32_t j = threadIdx.x; j < E && j <= fz; j += SCAN_NT) {
+        if (j < fz) rp[B0 + j] = 0;
+        else rp[B0 + j] += 1;
+      }
+      __syncthreads();
+    }
+    __syncthreads();                                 // list and nlist are free again
   }
 }
 
@@ -372,11 +422,10 @@ static int ctx_twiddles(DevCtx &c, int L, hipStream_t st) {
 }
 
 // workspace layout for transform length N: aspec, bspec (K*N u32 each),
-// limb carry maps (one byte per limb, padded to whole blocks) and block maps
-// (one u32 per block of CARRY_LPB limbs)
+// then one carry map (u32) per block of CARRY_LPB limbs
 static size_t ws_need(size_t N) {
   const size_t nb = N / CARRY_LPB + 2;
-  return 2 * SBA_K * N * 4 + nb * CARRY_LPB + nb * 4 + 4096;
+  return 2 * SBA_K * N * 4 + nb * 4 + 4096;
 }
 
 static int lg_for(uint64_t zlen) {
@@ -627,11 +676,11 @@ static int enqueue_class(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef
     const int ell1 = L - pl.last;
     const uint32_t ntile1 = N >> SBA_FIRST_TILE_LG;
     const uint32_t ntile2 = N >> pl.last;
+    // both operands' first passes in one launch (one kernel boundary fewer)
     if (!sqr) {
-      PROF("first_fwd", 8.0 * asum + KN4);
-      hipLaunchKernelGGL(k.first, dim3(ntile1, gy1, njobs), dim3(NT), 0, st, pa, jr, 0, L);
-    }
-    {
+      PROF("first_fwd", 8.0 * (asum + bsum) + 2 * KN4);
+      hipLaunchKernelGGL(k.first, dim3(ntile1, 2 * gy1, njobs), dim3(NT), 0, st, pa, jr, 2, L);
+    } else {
       PROF("first_fwd", 8.0 * bsum + KN4);
       hipLaunchKernelGGL(k.first, dim3(ntile1, gy1, njobs), dim3(NT), 0, st, pa, jr, 1, L);
     }
@@ -664,12 +713,13 @@ static int enqueue_garner(DevCtx &c, uint32_t Nt, uint32_t stride, const JobRef 
   (void)Nt;
   GarnerArgs ga = c.ga_base;
   {
-    PROF("garner_maps", 4.0 * SBA_K * zsum + 9.0 * zsum);
-    hipLaunchKernelGGL(k_garner_maps, dim3(nbmax, 1, njobs), dim3(NT), 0, st, ga, jr, stride);
+    PROF("garner", 4.0 * SBA_K * zsum + 8.0 * zsum);
+    hipLaunchKernelGGL(k_garner_limbs, dim3(nbmax, 1, njobs), dim3(NT), 0, st, ga, jr, stride);
   }
   {
-    PROF("carry_fix", 17.0 * zsum);
-    hipLaunchKernelGGL(k_carry_fix, dim3(nbmax, 1, njobs), dim3(NT), 0, st, jr);
+    PROF("carry_blocks", 8.0 * zsum / CARRY_LPB);
+    const uint32_t g = nbmax < CARRY_WGS ? nbmax : CARRY_WGS;
+    hipLaunchKernelGGL(k_carry_blocks, dim3(g, 1, njobs), dim3(SCAN_NT), 0, st, jr);
   }
   HIPCHK(hipGetLastError());
   return SBA_OK;
@@ -679,9 +729,7 @@ static int enqueue_garner(DevCtx &c, uint32_t Nt, uint32_t stride, const JobRef 
 static void job_layout(Job &j, void *ws, uint32_t N) {
   j.aspec = (uint32_t *)ws;
   j.bspec = j.aspec + (size_t)SBA_K * N;
-  j.lmaps = (uint8_t *)(j.bspec + (size_t)SBA_K * N);
-  const size_t nbcap = N / CARRY_LPB + 2;
-  j.bmaps = (uint32_t *)(j.lmaps + nbcap * CARRY_LPB);
+  j.bmaps = j.bspec + (size_t)SBA_K * N;
 }
 
 static int get_ctx(DevCtx **out) {
@@ -704,6 +752,25 @@ static int get_ctx(DevCtx **out) {
 // products longer than 2^25 coefficients.
 // ---------------------------------------------------------------------------
 #define SBA_MAX_Z ((uint64_t)1 << 29)   // product limbs: residue offsets stay 32-bit
+
+extern "C" int sba_gpu_carry_blocks(uint64_t *rp, const uint32_t *bmaps, uint64_t Z,
+                                    void *stream) {
+  if (!rp || !bmaps || Z == 0 || Z > SBA_MAX_Z) return SBA_BAND;
+  JobRef jr;
+  jr.arr = nullptr;
+  Job &j = jr.one;
+  memset(&j, 0, sizeof j);
+  j.r = rp;
+  j.bmaps = (uint32_t *)bmaps;
+  j.Z = (uint32_t)Z;
+  j.zlen = (uint32_t)(Z - 1);
+  j.nb = carry_blocks(Z);
+  const uint32_t g = j.nb < CARRY_WGS ? j.nb : CARRY_WGS;
+  hipLaunchKernelGGL(k_carry_blocks, dim3(g, 1, 1), dim3(SCAN_NT), 0, (hipStream_t)stream, jr);
+  HIPCHK(hipGetLastError());
+  return SBA_OK;
+}
+
 #define CHUNK_MIN 8192                  // smallest chunk (limbs)
 #define CHUNK_LAUNCH_EQ 150000.0        // one launch costs about this many element-transforms
 
@@ -739,7 +806,7 @@ static ChunkChoice chunk_choice(uint64_t an, uint64_t bn, int sqr) {
 // short spectrum, chunk spectrum, accumulator, carry maps of Z limbs
 static size_t ws_chunked(int Lc, uint64_t Z) {
   const size_t Nc = (size_t)1 << Lc, Zs = (Z + 255) & ~(size_t)255, nb = Z / CARRY_LPB + 2;
-  return 2 * SBA_K * Nc * 4 + SBA_K * Zs * 4 + nb * CARRY_LPB + nb * 4 + 4096;
+  return 2 * SBA_K * Nc * 4 + SBA_K * Zs * 4 + nb * 4 + 4096;
 }
 
 // an >= bn; ws holds ws_chunked(cp.Lc, an + bn) bytes
@@ -811,9 +878,7 @@ static int enqueue_chunked(DevCtx &c, const ChunkChoice &cp, uint64_t *rp, const
   g.zlen = (uint32_t)(Z - 1);
   g.Z = (uint32_t)Z;
   g.nb = carry_blocks(Z);
-  const size_t nbcap = Z / CARRY_LPB + 2;
-  g.lmaps = (uint8_t *)(acc + (size_t)SBA_K * Zs);
-  g.bmaps = (uint32_t *)(g.lmaps + nbcap * CARRY_LPB);
+  g.bmaps = acc + (size_t)SBA_K * Zs;
   return enqueue_garner(c, Nc, Zs, jg, 1, g.nb, (double)Z, st);
 }
 
@@ -877,10 +942,9 @@ static int enqueue_r3(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef &j
   const uint32_t ntile1 = M >> 11, ntile2 = M >> pl.last;
   const uint32_t gy1 = pl.split ? SBA_K : 1;
   if (!sqr) {
-    PROF("first_fwd", 8.0 * asum + KN4);
-    hipLaunchKernelGGL(f1, dim3(ntile1, gy1, njobs), dim3(NT), 0, st, pa, jr, 0, L);
-  }
-  {
+    PROF("first_fwd", 8.0 * (asum + bsum) + 2 * KN4);
+    hipLaunchKernelGGL(f1, dim3(ntile1, 2 * gy1, njobs), dim3(NT), 0, st, pa, jr, 2, L);
+  } else {
     PROF("first_fwd", 8.0 * bsum + KN4);
     hipLaunchKernelGGL(f1, dim3(ntile1, gy1, njobs), dim3(NT), 0, st, pa, jr, 1, L);
   }

@@ -237,8 +237,7 @@ struct Job {
   const uint64_t *a, *b;          // operand limbs (device)
   uint64_t *r;                    // product limbs (device), an+bn of them
   uint32_t *aspec, *bspec;        // K x N residues each
-  uint8_t *lmaps;                 // one carry map per limb, padded to blocks
-  uint32_t *bmaps;                // one map per carry block
+  uint32_t *bmaps;                // one 2-state carry map per carry block
   uint32_t an, bn, zlen, Z, nb;   // limbs, coefficients, product limbs, carry blocks
 };
 // global-address-space views of descriptor pointers: loads through a pointer
@@ -279,7 +278,7 @@ __device__ __forceinline__ Job get_job(const JobRef &jr, uint32_t idx) {
     j = jr.one;
   }
   j.a = uni_ptr(j.a); j.b = uni_ptr(j.b); j.r = uni_ptr(j.r);
-  j.aspec = uni_ptr(j.aspec); j.bspec = uni_ptr(j.bspec); j.lmaps = uni_ptr(j.lmaps);
+  j.aspec = uni_ptr(j.aspec); j.bspec = uni_ptr(j.bspec);
   j.bmaps = uni_ptr(j.bmaps);
   j.an = uni_u32(j.an); j.bn = uni_u32(j.bn); j.zlen = uni_u32(j.zlen); j.Z = uni_u32(j.Z);
   j.nb = uni_u32(j.nb);
@@ -290,9 +289,24 @@ __device__ __forceinline__ Job get_job(const JobRef &jr) { return get_job(jr, bl
 // ---------------------------------------------------------------------------
 // pass kernels
 // ---------------------------------------------------------------------------
+// Operand and prime of a first-pass block. which = 0 (a) or 1 (b): grid y
+// = K (one prime per block) or 1 (all primes per block); which = 2: both
+// operands in one launch, grid y = 2K or 2 (a in the lower half).
+struct OpSel {
+  int op;
+  bool split;
+  uint32_t q0;
+};
+__device__ __forceinline__ OpSel op_sel(int which) {
+  const uint32_t per = which == 2 ? gridDim.y >> 1 : gridDim.y;
+  const int op = which == 2 ? (int)(blockIdx.y >= per) : which;
+  const uint32_t y = blockIdx.y - (op && which == 2 ? per : 0);
+  return OpSel{op, per > 1, per > 1 ? y : 0u};
+}
+
 // First forward pass (global levels [0, ELL)): limbs -> residues of all K
 // primes -> ELL levels -> spec[q][.]. Tiles = column tiles (H = 0).
-// which = 0: operand a into aspec; 1: operand b into bspec.
+// which = 0: operand a into aspec; 1: operand b into bspec; 2: both (op_sel).
 template <int ELL, int LOGTC, int BMAX>
 __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa, JobRef jr, int which,
                                                              int L) {
@@ -301,15 +315,15 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
   constexpr int E = NE / NT;
   __shared__ uint32_t lds[Pad<LOGTC>::words(NE)];
   const Job jb = get_job(jr);
-  gc_u64 *limbs = (gc_u64 *)(which ? jb.b : jb.a);
-  g_u32 *spec = (g_u32 *)(which ? jb.bspec : jb.aspec);
-  const uint32_t nlimbs = which ? jb.bn : jb.an;
+  const OpSel os = op_sel(which);
+  gc_u64 *limbs = (gc_u64 *)(os.op ? jb.b : jb.a);
+  g_u32 *spec = (g_u32 *)(os.op ? jb.bspec : jb.aspec);
+  const uint32_t nlimbs = os.op ? jb.bn : jb.an;
   const uint32_t N = 1u << L;             // N <= 2^25: 32-bit offsets throughout
   const uint32_t S = 1u << (L - ELL);
-  // grid y = K: one prime per block; grid y = 1: all primes per block
-  const bool split = gridDim.y > 1;
+  const bool split = os.split;
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-  const uint32_t q0 = split ? blockIdx.y : 0;
+  const uint32_t q0 = os.q0;
   const uint32_t col0 = tile << LOGTC;
   const uint32_t nl = nlimbs < N ? nlimbs : N;
   uint32_t *lt = lds + lpadt<LOGTC>(threadIdx.x);
@@ -539,9 +553,10 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd3(PrimeArgs pa, JobR
   constexpr int W = Pad<LOGTC>::words(NE);
   __shared__ uint32_t lds[3 * W];
   const Job jb = get_job(jr);
-  gc_u64 *limbs = (gc_u64 *)(which ? jb.b : jb.a);
-  g_u32 *spec = (g_u32 *)(which ? jb.bspec : jb.aspec);
-  const uint32_t nlimbs = which ? jb.bn : jb.an;
+  const OpSel os = op_sel(which);
+  gc_u64 *limbs = (gc_u64 *)(os.op ? jb.b : jb.a);
+  g_u32 *spec = (g_u32 *)(os.op ? jb.bspec : jb.aspec);
+  const uint32_t nlimbs = os.op ? jb.bn : jb.an;
   const uint32_t M = 1u << L;
   const uint32_t S = 1u << (L - ELL);
   const uint32_t mu = (L & 1) ? 2u : 1u;
@@ -550,8 +565,8 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd3(PrimeArgs pa, JobR
   const uint32_t nl = nlimbs < 3 * M ? nlimbs : 3 * M;
   uint32_t *lt = lds + lpadt<LOGTC>(threadIdx.x);
   const uint32_t toff = (threadIdx.x >> LOGTC) * S + (threadIdx.x & (TC - 1));
-  const bool split = gridDim.y > 1;
-  const uint32_t q0 = split ? blockIdx.y : 0;
+  const bool split = os.split;
+  const uint32_t q0 = os.q0;
   const uint32_t q1 = split ? q0 + 1 : SBA_K;
 #pragma unroll 1
   for (uint32_t q = q0; q < q1; ++q) {

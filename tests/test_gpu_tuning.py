@@ -269,3 +269,21 @@ def test_training_on_gpu_learns_plan_costs():
     say("held-out: model %.4f heuristic %.4f time over best" %
         (qm["time_over_best"], qh["time_over_best"]))
     assert qm["time_over_best"] < qh["time_over_best"]
+
+
+def test_gpu_threshold_tuner():
+    """The crossover tuner times the CPU multiply against the GPU path on the
+    ladder of balanced sizes; from the threshold it returns up, the GPU wins
+    at every ladder size, and apply=False leaves the threshold alone."""
+    before = sba.get_gpu_threshold()
+    res = sba.tune_gpu_threshold(apply=False)
+    print("tuner", res, file=sys.stderr, flush=True)
+    assert sba.get_gpu_threshold() == before
+    tot, cpu, gpu = res["totals"], res["cpu_ms"], res["gpu_ms"]
+    assert len(tot) == len(cpu) == len(gpu) > 4
+    assert all(x > 0 for x in cpu + gpu)
+    assert gpu[-1] < cpu[-1]                    # 64Ki limbs: the GPU path wins
+    i = tot.index(res["threshold"])
+    assert all(g < c for g, c in zip(gpu[i:], cpu[i:]))
+    if i > 0:
+        assert gpu[i - 1] >= cpu[i - 1]
