@@ -13,6 +13,9 @@ Also reported, from the same process:
                 8 TB/s HBM peak;
   roofline_pipeline  the whole multiply's algorithmic bytes Z*(16+32K), K=6
                 (the accounting of the reference-size model) / step time;
+  roofline_valu the dominant kernel's butterflies and pointwise products at
+                their VALU operation counts / its duration, vs the VALU
+                issue peak (the transforms are VALU-bound, not HBM-bound);
   cpu_baseline  the reference's nt_mul compiled from source (oracle/_ref,
                 "reference") when present and the host CPU has its ISA, else
                 the oracle's C restatement ("port"), on the same operands.
@@ -30,6 +33,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: 256 CUs x 4 SIMD16 x 2.4 GHz lane-operations per second
+VALU_PEAK_T = 256 * 4 * 16 * 2.4e9 / 1e12
+VALU_PER_BFLY = 10     # radix-2 butterfly: 3 multiplies, 2 folds (sub + min), 3 add/sub
+VALU_PER_POINT = 5     # pointwise product: a fold and a Montgomery product
 METRIC = "ns/limb at 4Mi×4Mi-limb mul; batched vs per-call muls/s, bit-exact"
 
 
@@ -311,16 +318,22 @@ def configs_leg(L, reps=10):
     nn_ = xorshift_fill(e["nn"], st)
     dd_ = xorshift_fill(e["dn"], st)
     dd_[-1] |= np.uint64(1 << 63)
+    # the C entry on caller buffers allocated once, as the reference is timed below
+    q = np.zeros(e["nn"] - e["dn"] + 1, np.uint64)
+    r = np.zeros(e["dn"], np.uint64)
+    qn_, rn_ = ctypes.c_uint64(), ctypes.c_uint64()
     best = None
-    for _ in range(3):
+    for _ in range(4):
         t0 = time.perf_counter()
-        q, r = sba.divrem_u64(nn_, dd_, dc=True)
+        sba.lib().divrem_u64_dc(sba._arr(q), sba._arr(r), sba._arr(nn_), e["nn"], sba._arr(dd_),
+                                e["dn"], ctypes.byref(qn_), ctypes.byref(rn_))
         dt = (time.perf_counter() - t0) * 1e3
         best = dt if best is None else min(best, dt)
+    q, r = q[:qn_.value], r[:rn_.value]
     out["cfg3_divrem_2^21/2^20"] = {
         "nn": e["nn"], "dn": e["dn"], "ms": round(best, 3),
         "bit_exact": bool(sha(q) == e["q_sha256"] and sha(r) == e["r_sha256"]),
-        "note": "host buffers; includes the operand and result copies over PCIe"}
+        "note": "host buffers (caller-allocated); includes the operand and result copies over PCIe"}
     # the reference's divrem_u64_dc on one host core, same operands (once)
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libsbref.so")
     need = {"avx512f", "avx512bw", "avx512vl", "avx512dq", "avx512ifma", "avx512vbmi",
@@ -543,6 +556,23 @@ def main():
     dom_name = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches_per_step"])
     dom = kernels[dom_name]
 
+    # the dominant kernel against the VALU issue peak (the transforms are
+    # VALU-bound): the merged last pass runs `last` forward levels of a and of
+    # b and `last` inverse levels, N/2 butterflies per level and prime, and
+    # N pointwise products per prime
+    valu = None
+    if dom_name == "fused_last":
+        from simd_bigint_amd import tuning
+        Lg, K = 23, 5
+        last = tuning.plan_get(Lg, 0, tuning.plan_bucket(1))[0]
+        ops = K * (3 * last * ((1 << Lg) // 2) * VALU_PER_BFLY + (1 << Lg) * VALU_PER_POINT)
+        ach = ops / (dom["avg_us"] * 1e-6) / 1e12
+        valu = {"bound": "valu", "kernel": dom_name, "achieved": round(ach, 2),
+                "peak": round(VALU_PEAK_T, 2), "unit": "T lane-ops/s",
+                "frac": round(ach / VALU_PEAK_T, 4),
+                "model": "%d VALU lane-ops per radix-2 butterfly, %d per pointwise product, "
+                         "%d levels x 3 transforms" % (VALU_PER_BFLY, VALU_PER_POINT, last)}
+
     Z = 2 * n
     K_model = 6
     b_alg = Z * (16 + 32 * K_model)
@@ -588,6 +618,7 @@ def main():
         "roofline_pipeline": {"bound": "hbm", "alg_bytes": b_alg, "achieved": round(pipe_gbs, 1),
                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(pipe_gbs / HBM_PEAK_GBS, 4)},
+        "roofline_valu": valu,
         "kernels": kernels,
         "other_configs": others,
         "batched": batched,

@@ -261,6 +261,13 @@ int sba_cm_predict(int L, int sqr, int njobs, double fill, int last, int bm, int
  * limbs to the GPU, smaller ones to the CPU Karatsuba. */
 void sba_set_gpu_threshold(uint64_t min_total_limbs);
 uint64_t sba_get_gpu_threshold(void);
+/* products with an + bn <= 2048 limbs run on the schoolbook kernel (one
+ * workgroup per product) instead of the NTT when an * bn <= max_product *
+ * N / 1024 in a batch and half that in a single call (N = the NTT length
+ * the product would get, at least 1024); 0 turns that tier off. The
+ * default, 32768, is the MI355X-measured crossover. */
+void sba_set_school_max(uint64_t max_product);
+uint64_t sba_get_school_max(void);
 /* crossover tuner: times balanced products of each total in the ladder
  * that sba_tune_sizes lists (host buffers; the CPU multiply against the GPU
  * path with its copies, best of 5) and returns the smallest total from

@@ -161,6 +161,10 @@ def lib() -> ctypes.CDLL:
     L.sba_set_gpu_threshold.restype = None
     L.sba_get_gpu_threshold.argtypes = []
     L.sba_get_gpu_threshold.restype = u64
+    L.sba_set_school_max.argtypes = [u64]
+    L.sba_set_school_max.restype = None
+    L.sba_get_school_max.argtypes = []
+    L.sba_get_school_max.restype = u64
     L.sba_tune_sizes.argtypes = [ctypes.POINTER(u64), ctypes.c_int]
     L.sba_tune_sizes.restype = ctypes.c_int
     L.sba_tune_gpu_threshold.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
@@ -243,6 +247,17 @@ def set_gpu_threshold(min_total_limbs: int) -> None:
 
 def get_gpu_threshold() -> int:
     return int(lib().sba_get_gpu_threshold())
+
+
+def set_school_max(max_product: int) -> None:
+    """Products with an + bn <= 2048 take the schoolbook kernel when
+    an * bn <= max_product * N / 1024 in a batch, half that in a single call
+    (N = their NTT length, at least 1024); 0 turns that tier off."""
+    lib().sba_set_school_max(int(max_product))
+
+
+def get_school_max() -> int:
+    return int(lib().sba_get_school_max())
 
 
 def tune_gpu_threshold(apply: bool = False) -> dict:

@@ -1,15 +1,15 @@
-// div_dev.hip -- the large division's Newton steps kept in HBM.
+// div_dev.hip -- the large division's Newton branch kept in HBM.
 //
 // The host division (host/sba_div.c) computes the quotient from one
 // approximate reciprocal (Brent & Zimmermann, Modern Computer Arithmetic,
 // Algorithm 3.5) and one quotient multiply. Above a size threshold this file
-// runs those steps on device buffers: every multiply is the GPU NTT on
-// device pointers, and the linear steps between them are kernels -- an
-// add/subtract with a carry-lookahead scan and the complement B^k - T (an
-// add of ~T and 1). Only the operands go up and the quotient estimate plus
-// the low dn + 1 limbs of the remainder come back; the reciprocal's
-// recursion below DIV_HOST_RECIP limbs and the rare corrections run on the
-// host.
+// runs the whole branch on device buffers: every multiply is the GPU NTT on
+// device pointers, and the linear steps between them are kernels -- the
+// normalising shifts, an add/subtract with a carry-lookahead scan, the
+// complement B^k - T (an add of ~T and 1), the comparison of the correction
+// loop. The operands go up and the quotient and remainder come back through
+// pinned staging; only the reciprocal's recursion at or below
+// DIV_HOST_RECIP limbs and its rare overflow fix-up run on the host.
 #include "sba_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -23,7 +23,7 @@ typedef uint64_t u64;
 #define DNT 256
 #define DLPT 8
 #define DLPB (DNT * DLPT)          // limbs per block
-#define DIV_HOST_RECIP 4096        // reciprocals of at most this many limbs: host
+#define DIV_HOST_RECIP 1024        // reciprocals of at most this many limbs: host
 
 // ---------------------------------------------------------------------------
 // add with carry: r = a + (sub ? ~b : b) + cin over n limbs (a or b may be

@@ -106,34 +106,15 @@ __device__ __forceinline__ uint32_t c2_then(uint32_t f, uint32_t g) {
   return c2_apply(g, f & 1u) | (c2_apply(g, (f >> 1) & 1u) << 1);
 }
 
-__global__ __launch_bounds__(NT) void k_garner_limbs(GarnerArgs ga, JobRef jr, uint32_t N) {
-  // slot j + 3 holds the words of coefficient B0 + j (slots 0..2: B0-3..B0-1)
-  __shared__ uint64_t ws0[CARRY_LPB + 3], ws1[CARRY_LPB + 3], ws2[CARRY_LPB + 3];
+// The limbs [B0, B0 + CARRY_LPB) of a product from the coefficient words in
+// LDS (slot j + 3 holds the words of coefficient B0 + j, slots 0..2 those of
+// B0-3..B0-1; zero outside the product), finished with block carry-in 0;
+// *bmap (if not null) gets the block's 2-state carry map. Called by the whole
+// workgroup after the words are in place.
+__device__ __forceinline__ void limbs_from_words(const uint64_t *ws0, const uint64_t *ws1,
+                                                 const uint64_t *ws2, uint32_t B0, uint32_t Z,
+                                                 g_u64 *rp, g_u32 *bmap) {
   __shared__ uint32_t rmap[CARRY_RUNS], rcin[CARRY_RUNS];
-  const Job jb = get_job(jr);
-  if (blockIdx.x >= jb.nb) return;                 // this job has fewer carry blocks
-  gc_u32 *spec = (gc_u32 *)jb.bspec;
-  const uint32_t zlen = jb.zlen, Z = jb.Z;
-  g_u64 *rp = (g_u64 *)jb.r;
-  const uint32_t B0 = blockIdx.x * CARRY_LPB;
-#pragma unroll 2
-  for (int k = 0; k < CARRY_LPT; ++k) {
-    const uint32_t j = threadIdx.x + k * NT;
-    const uint32_t i = B0 + j;
-    uint64_t c0 = 0, c1 = 0, c2 = 0;
-    if (i < zlen) garner_coef(ga, spec, N, i, c0, c1, c2);
-    ws0[j + 3] = c0; ws1[j + 3] = c1; ws2[j + 3] = c2;
-  }
-  if (threadIdx.x < 3) {
-    const uint32_t j = threadIdx.x;            // coefficient B0 - 3 + j
-    uint64_t c0 = 0, c1 = 0, c2 = 0;
-    if (B0 + j >= 3) {
-      const uint32_t i = B0 + j - 3;
-      if (i < zlen) garner_coef(ga, spec, N, i, c0, c1, c2);
-    }
-    ws0[j] = c0; ws1[j] = c1; ws2[j] = c2;
-  }
-  __syncthreads();
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   uint64_t t[CARRY_LPT], ra[CARRY_LPT], rb[CARRY_LPT];
 #pragma unroll
@@ -173,7 +154,7 @@ __global__ __launch_bounds__(NT) void k_garner_limbs(GarnerArgs ga, JobRef jr, u
     uint32_t exc = __shfl_up(inc, 1, 64);
     if (lane == 0) exc = C2_ID;
     if (lane < CARRY_RUNS) rcin[lane] = c2_apply(exc, 0);
-    if (lane == 63) ((g_u32 *)jb.bmaps)[blockIdx.x] = inc;
+    if (lane == 63 && bmap) *bmap = inc;
   }
   __syncthreads();
 #pragma unroll
@@ -183,6 +164,106 @@ __global__ __launch_bounds__(NT) void k_garner_limbs(GarnerArgs ga, JobRef jr, u
     const uint64_t C = (ra[k] + rb[k] + cin) ^ ra[k] ^ rb[k];
     if (B0 + j < Z) rp[B0 + j] = t[k] + ((C >> lane) & 1u);
   }
+}
+
+__global__ __launch_bounds__(NT) void k_garner_limbs(GarnerArgs ga, JobRef jr, uint32_t N) {
+  __shared__ uint64_t ws0[CARRY_LPB + 3], ws1[CARRY_LPB + 3], ws2[CARRY_LPB + 3];
+  const Job jb = get_job(jr);
+  if (blockIdx.x >= jb.nb) return;                 // this job has fewer carry blocks
+  gc_u32 *spec = (gc_u32 *)jb.bspec;
+  const uint32_t zlen = jb.zlen, Z = jb.Z;
+  const uint32_t B0 = blockIdx.x * CARRY_LPB;
+#pragma unroll 2
+  for (int k = 0; k < CARRY_LPT; ++k) {
+    const uint32_t j = threadIdx.x + k * NT;
+    const uint32_t i = B0 + j;
+    uint64_t c0 = 0, c1 = 0, c2 = 0;
+    if (i < zlen) garner_coef(ga, spec, N, i, c0, c1, c2);
+    ws0[j + 3] = c0; ws1[j + 3] = c1; ws2[j + 3] = c2;
+  }
+  if (threadIdx.x < 3) {
+    const uint32_t j = threadIdx.x;            // coefficient B0 - 3 + j
+    uint64_t c0 = 0, c1 = 0, c2 = 0;
+    if (B0 + j >= 3) {
+      const uint32_t i = B0 + j - 3;
+      if (i < zlen) garner_coef(ga, spec, N, i, c0, c1, c2);
+    }
+    ws0[j] = c0; ws1[j] = c1; ws2[j] = c2;
+  }
+  __syncthreads();
+  limbs_from_words(ws0, ws1, ws2, B0, Z, (g_u64 *)jb.r, (g_u32 *)jb.bmaps + blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// small products: schoolbook on one workgroup per product (an + bn <= 2048
+// limbs, one carry block). Column k's sum of a_i b_(k-i) is kept as three
+// words (< 2^(128 + 11)), which is exactly the coefficient form the Garner
+// stage produces, so the same limb finishing applies. Grid x = job.
+// ---------------------------------------------------------------------------
+#define SCHOOL_ZMAX CARRY_LPB
+#define SCHOOL_COLS (SCHOOL_ZMAX / NT)        // columns per thread
+#ifndef SBA_SCHOOL_PMAX_DEFAULT
+#define SBA_SCHOOL_PMAX_DEFAULT 32768         // schoolbook limit on an * bn at N = 1024
+#endif
+
+__global__ __launch_bounds__(NT) void k_school(JobRef jr) {
+  // operands first (a at [0, an), b at [an, an + bn)), then the column words
+  __shared__ uint64_t sm[3 * (SCHOOL_ZMAX + 3)];
+  const Job jb = get_job(jr, blockIdx.x);
+  const uint32_t an = jb.an, bn = jb.bn, Z = jb.Z;
+  gc_u64 *ga_ = (gc_u64 *)jb.a;
+  gc_u64 *gb_ = (gc_u64 *)jb.b;
+  for (uint32_t i = threadIdx.x; i < an; i += NT) sm[i] = ga_[i];
+  for (uint32_t i = threadIdx.x; i < bn; i += NT) sm[an + i] = gb_[i];
+  __syncthreads();
+  const uint64_t *sa = sm, *sb = sm + an;
+  uint64_t w0[SCHOOL_COLS], w1[SCHOOL_COLS], w2[SCHOOL_COLS];
+#pragma unroll
+  for (int m = 0; m < SCHOOL_COLS; ++m) {
+    const uint32_t k = threadIdx.x + m * NT;       // column (coefficient) k < Z - 1
+    uint64_t lo = 0, mid = 0, hi = 0;
+    if (k + 1 < Z) {
+      const uint32_t i0 = k + 1 > bn ? k + 1 - bn : 0;
+      const uint32_t i1 = k < an - 1 ? k : an - 1;
+      for (uint32_t i = i0; i <= i1; ++i) {
+        const uint64_t x = sa[i], y = sb[k - i];
+        const uint64_t pl = x * y, ph = __umul64hi(x, y);
+        lo += pl;
+        const uint64_t c1 = lo < pl ? 1u : 0u;
+        const uint64_t m1 = mid + ph;
+        hi += m1 < mid ? 1u : 0u;
+        mid = m1 + c1;
+        hi += mid < c1 ? 1u : 0u;
+      }
+    }
+    w0[m] = lo; w1[m] = mid; w2[m] = hi;
+  }
+  __syncthreads();                                 // the operands are no longer read
+  uint64_t *ws0 = sm, *ws1 = sm + (SCHOOL_ZMAX + 3), *ws2 = sm + 2 * (SCHOOL_ZMAX + 3);
+#pragma unroll
+  for (int m = 0; m < SCHOOL_COLS; ++m) {
+    const uint32_t j = threadIdx.x + m * NT;
+    ws0[j + 3] = w0[m]; ws1[j + 3] = w1[m]; ws2[j + 3] = w2[m];
+  }
+  if (threadIdx.x < 3) ws0[threadIdx.x] = ws1[threadIdx.x] = ws2[threadIdx.x] = 0;
+  __syncthreads();
+  limbs_from_words(ws0, ws1, ws2, 0, Z, (g_u64 *)jb.r, nullptr);
+}
+
+// products with an + bn <= SCHOOL_ZMAX take the schoolbook kernel when
+// an * bn <= g_school_pmax * N / 1024 in a batch (N = the NTT length they
+// would get, >= 1024: the NTT's cost grows with N, the schoolbook's with
+// an * bn), half that for a single call (where the NTT's few launches of
+// latency dominate). Measured on MI355X: 512-job batches and single calls of
+// 16x16 .. 1024x1024 and 16x2032 .. 256x1792 limbs. sba_set_school_max sets
+// g_school_pmax; 0 turns the tier off.
+static uint64_t g_school_pmax = SBA_SCHOOL_PMAX_DEFAULT;
+static bool school_job(uint64_t an, uint64_t bn, bool single) {
+  if (an + bn > SCHOOL_ZMAX) return false;
+  uint64_t N = 1024;
+  while (N < an + bn - 1) N <<= 1;
+  const uint64_t lim = (single ? g_school_pmax / 2 : g_school_pmax) * (N / 1024);
+  return an * bn <= lim;
 }
 
 // workgroup-wide exclusive scan of 2-state maps in thread order; *total =
@@ -1017,6 +1098,28 @@ static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64
     const uint64_t *t = ap; ap = bp; bp = t;
     int64_t tn = an; an = bn; bn = tn;
   }
+  if (school_job((uint64_t)an, (uint64_t)bn, true)) {
+    hipStream_t st = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lk(g_mu);
+    JobRef jr;
+    jr.arr = nullptr;
+    Job &j = jr.one;
+    memset(&j, 0, sizeof j);
+    j.a = ap;
+    j.b = sqr ? ap : bp;
+    j.r = rp;
+    j.an = (uint32_t)an;
+    j.bn = (uint32_t)bn;
+    j.zlen = (uint32_t)(an + bn - 1);
+    j.Z = (uint32_t)(an + bn);
+    j.nb = 1;
+    {
+      PROF("school", 16.0 * (double)j.Z);
+      hipLaunchKernelGGL(k_school, dim3(1, 1, 1), dim3(NT), 0, st, jr);
+    }
+    HIPCHK(hipGetLastError());
+    return SBA_OK;
+  }
   const ChunkChoice cp = chunk_choice((uint64_t)an, (uint64_t)bn, sqr);
   const int L3 = cp.n ? 0 : r3_lg((uint64_t)an + (uint64_t)bn - 1);
   const int L = cp.n ? cp.Lc : (L3 ? L3 : lg_for((uint64_t)an + (uint64_t)bn - 1));
@@ -1095,8 +1198,9 @@ struct ClassDesc {
   double as, bs, zs;
   Plan plan;
   int r3;                          // radix-3 class: length 3 * 2^L
+  int school;                      // the schoolbook class (every small product)
 };
-#define SBA_MAX_CLASSES (4 * (SBA_MAX_LG + 1))
+#define SBA_MAX_CLASSES (4 * (SBA_MAX_LG + 1) + 1)
 
 // class of a batch job: (L, radix-3?) of its transform (never chunked)
 static void job_key(const sba_job &j, int *L, int *r3) {
@@ -1113,14 +1217,43 @@ static size_t job_ws(const sba_job &j) {
 }
 static size_t table_bytes(int njobs) { return ((size_t)njobs * sizeof(Job) + 255) & ~(size_t)255; }
 
+static bool job_school(const sba_job &j) {
+  return school_job((uint64_t)j.an, j.bp ? (uint64_t)j.bn : (uint64_t)j.an, false);
+}
+
 // Orders the jobs by class into hj[], carving each job's buffers from ws in
 // that order (job_ws bytes each), and fills cls[] (at most SBA_MAX_CLASSES
-// entries: square?, radix-3?, L). Every job must be in band. Returns the number of classes.
-static int classify_jobs(const sba_job *jobs, int njobs, char *ws, Job *hj, ClassDesc *cls) {
+// entries: the schoolbook class when school != 0, then square?, radix-3?,
+// L). Every job must be in band. Returns the number of classes.
+static int classify_jobs(const sba_job *jobs, int njobs, char *ws, Job *hj, ClassDesc *cls,
+                         int school) {
   int ncls = 0, pos = 0;
   size_t off = 0;
   int count[2][2][SBA_MAX_LG + 1] = {};
+  if (school) {
+    ClassDesc k = {0, 0, pos, 0, 0, 0, 0, 0, Plan{0, 0, 0}, 0, 1};
+    for (int i = 0; i < njobs; ++i) {
+      if (!job_school(jobs[i])) continue;
+      const int isq = jobs[i].bp == nullptr;
+      const uint64_t an = (uint64_t)jobs[i].an, bn = isq ? an : (uint64_t)jobs[i].bn;
+      Job &j = hj[pos++];
+      memset(&j, 0, sizeof j);
+      j.a = jobs[i].ap;
+      j.b = isq ? jobs[i].ap : jobs[i].bp;
+      j.an = (uint32_t)an;
+      j.bn = (uint32_t)bn;
+      j.r = jobs[i].rp;
+      j.zlen = (uint32_t)(an + bn - 1);
+      j.Z = (uint32_t)(an + bn);
+      j.nb = 1;
+      off += job_ws(jobs[i]);
+      k.n++;
+      k.zs += j.Z;
+    }
+    if (k.n) cls[ncls++] = k;
+  }
   for (int i = 0; i < njobs; ++i) {
+    if (school && job_school(jobs[i])) continue;
     int L, r3;
     job_key(jobs[i], &L, &r3);
     count[jobs[i].bp == nullptr][r3][L]++;
@@ -1129,8 +1262,9 @@ static int classify_jobs(const sba_job *jobs, int njobs, char *ws, Job *hj, Clas
     for (int r3 = 0; r3 < 2; ++r3)
     for (int L = 0; L <= SBA_MAX_LG; ++L) {
       if (!count[sq][r3][L]) continue;
-      ClassDesc k = {L, sq, pos, 0, 0, 0, 0, 0, heuristic_plan(L), r3};
+      ClassDesc k = {L, sq, pos, 0, 0, 0, 0, 0, heuristic_plan(L), r3, 0};
       for (int i = 0; i < njobs; ++i) {
+        if (school && job_school(jobs[i])) continue;
         const int isq = jobs[i].bp == nullptr;
         const uint64_t an = (uint64_t)jobs[i].an, bn = isq ? an : (uint64_t)jobs[i].bn;
         int jL, jr3;
@@ -1196,6 +1330,14 @@ static int enqueue_classes(DevCtx &c, Job *djobs, const ClassDesc *cls, int ncls
     JobRef jr;
     jr.arr = djobs + cls[k].start;
     memset(&jr.one, 0, sizeof jr.one);
+    if (cls[k].school) {
+      {
+        PROF("school", 16.0 * cls[k].zs);
+        hipLaunchKernelGGL(k_school, dim3(cls[k].n, 1, 1), dim3(NT), 0, st, jr);
+      }
+      HIPCHK(hipGetLastError());
+      continue;
+    }
     const int rc =
         cls[k].r3 ? enqueue_r3(c, cls[k].L, cls[k].sqr, cls[k].plan, jr, cls[k].n, cls[k].nbmax,
                                cls[k].as, cls[k].bs, cls[k].zs, st)
@@ -1247,7 +1389,7 @@ extern "C" sba_batch *sba_batch_create(const sba_job *jobs, int njobs, int use_g
   }
   Job *hj = (Job *)calloc((size_t)njobs, sizeof(Job));
   if (!hj) { sba_batch_destroy(b); *status = SBA_ERR_HIP; return nullptr; }
-  b->nclass = classify_jobs(jobs, njobs, (char *)b->ws, hj, b->cls);
+  b->nclass = classify_jobs(jobs, njobs, (char *)b->ws, hj, b->cls, 1);
   hipError_t e = hipMemcpy(b->djobs_all, hj, sizeof(Job) * (size_t)njobs, hipMemcpyHostToDevice);
   free(hj);
   if (e != hipSuccess) { sba_batch_destroy(b); *status = SBA_ERR_HIP; return nullptr; }
@@ -1303,11 +1445,11 @@ extern "C" size_t sba_batch_ws_bytes(const sba_job *jobs, int njobs) {
 // stage the job table of `jobs` (classified) at the workspace tail; on
 // success *dj is the device table and cls/ncls describe the classes
 static int stage_jobs(const sba_job *jobs, int njobs, void *ws, size_t need, hipStream_t st,
-                      Job **dj, ClassDesc *cls, int *ncls) {
+                      Job **dj, ClassDesc *cls, int *ncls, int school) {
   *dj = (Job *)((char *)ws + need - table_bytes(njobs));
   Job *hj = (Job *)calloc((size_t)njobs, sizeof(Job));
   if (!hj) return SBA_ERR_HIP;
-  *ncls = classify_jobs(jobs, njobs, (char *)ws, hj, cls);
+  *ncls = classify_jobs(jobs, njobs, (char *)ws, hj, cls, school);
   hipError_t e = hipMemcpyAsync(*dj, hj, sizeof(Job) * (size_t)njobs, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);   // hj is pageable and freed below
   free(hj);
@@ -1328,7 +1470,7 @@ extern "C" int sba_batch_launch_ws(const sba_job *jobs, int njobs, void *ws, siz
   Job *dj;
   ClassDesc cls[SBA_MAX_CLASSES];
   int ncls = 0;
-  if ((rc = stage_jobs(jobs, njobs, ws, need, st, &dj, cls, &ncls)) != SBA_OK) return rc;
+  if ((rc = stage_jobs(jobs, njobs, ws, need, st, &dj, cls, &ncls, 1)) != SBA_OK) return rc;
   return enqueue_classes(*c, dj, cls, ncls, st);
 }
 
@@ -1342,6 +1484,12 @@ extern "C" int sba_job_class(const sba_job *j, int *L, int *kind) {
   *kind = (j->bp == nullptr ? 1 : 0) | (r3 ? 2 : 0);
   return SBA_OK;
 }
+
+extern "C" void sba_set_school_max(uint64_t max_product) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_school_pmax = max_product;
+}
+extern "C" uint64_t sba_get_school_max(void) { return g_school_pmax; }
 
 extern "C" int sba_plan_bucket(int njobs) { return plan_bucket(njobs < 1 ? 1u : (uint32_t)njobs); }
 
@@ -1417,7 +1565,8 @@ extern "C" int sba_class_time(const sba_job *jobs, int njobs, int last, int bm, 
   Job *dj;
   ClassDesc cls[SBA_MAX_CLASSES];
   int ncls = 0;
-  if ((rc = stage_jobs(jobs, njobs, ws, need, st, &dj, cls, &ncls)) != SBA_OK) return rc;
+  // plans are NTT plans: small jobs are timed on the NTT here, not the schoolbook
+  if ((rc = stage_jobs(jobs, njobs, ws, need, st, &dj, cls, &ncls, 0)) != SBA_OK) return rc;
   if (ncls != 1) return SBA_BAND;                 // one size class per measurement
   cls[0].plan = Plan{last, bm, split};
   if (!kind_plan_ok(cls[0].L, cls[0].r3 ? 2 : 0, cls[0].plan)) return SBA_BAND;

@@ -53,6 +53,9 @@ struct GarnerArgs {
 #ifndef SBA_FUSED2_WAVES
 #define SBA_FUSED2_WAVES 3
 #endif
+#ifndef SBA_ZERO_LEVEL
+#define SBA_ZERO_LEVEL 1     // first level of a half-empty operand as a copy
+#endif
 #ifndef SBA_FIRST_WAVES
 #define SBA_FIRST_WAVES SBA_WAVES
 #endif
@@ -128,7 +131,9 @@ template <int ELL, int LOGTC, int T0, int B> struct ChunkGeom {
       LOGD >= 5 ? (1 << LOGD) + (1 << (LOGD - 5)) * Pad<LOGTC>::PW : (1 << LOGD);
 };
 
-template <int ELL, int LOGTC, int T0, int B, bool INV>
+// ZL (forward only, first pass of a zero-padded operand): the levels below
+// ZL pair every element with a zero and become copies (x, 0) -> (x, x)
+template <int ELL, int LOGTC, int T0, int B, bool INV, int ZL = 0>
 __device__ __forceinline__ void tile_chunk(uint32_t *lds, const uint32_t *__restrict__ tw,
                                            uint32_t H, uint32_t p, uint32_t pinv) {
   constexpr int TC = 1 << LOGTC;
@@ -157,6 +162,12 @@ __device__ __forceinline__ void tile_chunk(uint32_t *lds, const uint32_t *__rest
 #pragma unroll
       for (int u = 0; u < B; ++u) {
         const int half = 1 << (B - u - 1);
+        if (T0 + u < ZL) {
+#pragma unroll
+          for (int m = 0; m < (1 << B); ++m)
+            if ((m >> (B - u - 1)) & 1) v[m] = v[m - half];
+          continue;
+        }
 #pragma unroll
         for (int m2 = 0; m2 < (1 << u); ++m2) {
           const uint32_t w = tw[(hj << u) | (uint32_t)m2];
@@ -208,15 +219,27 @@ template <int ELL, int LOGTC, int BMAX> struct ChunkPlan {
   }
 };
 
-template <int ELL, int LOGTC, int BMAX, int T0>
+template <int ELL, int LOGTC, int BMAX, int T0, int ZL = 0>
 __device__ __forceinline__ void tile_fwd(uint32_t *lds, const uint32_t *tw, uint32_t H,
                                          uint32_t p, uint32_t pinv) {
   if constexpr (T0 < ELL) {
     constexpr int B = ChunkPlan<ELL, LOGTC, BMAX>::pick(T0);
-    tile_chunk<ELL, LOGTC, T0, B, false>(lds, tw, H, p, pinv);
-    tile_fwd<ELL, LOGTC, BMAX, T0 + B>(lds, tw, H, p, pinv);
+    tile_chunk<ELL, LOGTC, T0, B, false, ZL>(lds, tw, H, p, pinv);
+    tile_fwd<ELL, LOGTC, BMAX, T0 + B, ZL>(lds, tw, H, p, pinv);
   }
 }
+
+// the forward levels of a first pass, with its first level a copy when the
+// operand fills at most half of the transform (SBA_ZERO_LEVEL; one uniform
+// branch per block, each side straight-line code)
+template <int ELL, int LOGTC, int BMAX>
+__device__ __forceinline__ void tile_fwd_first(uint32_t *lds, const uint32_t *tw, uint32_t p,
+                                               uint32_t pinv, bool half_zero) {
+  if (SBA_ZERO_LEVEL && half_zero) tile_fwd<ELL, LOGTC, BMAX, 0, 1>(lds, tw, 0, p, pinv);
+  else tile_fwd<ELL, LOGTC, BMAX, 0, 0>(lds, tw, 0, p, pinv);
+}
+
+
 
 template <int ELL, int LOGTC, int BMAX, int T0>
 __device__ __forceinline__ void tile_inv(uint32_t *lds, const uint32_t *tw, uint32_t H,
@@ -331,6 +354,9 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
   // and a uniform part (scalar base + 32-bit lane offset addressing)
   const uint32_t toff = (threadIdx.x >> LOGTC) * S + (threadIdx.x & (TC - 1));
   const uint32_t q1 = split ? q0 + 1 : SBA_K;
+  // an operand of at most N/2 limbs pairs every element with a zero in the
+  // first level (x, 0) -> (x, x)
+  const bool half_zero = nl <= N / 2;
 #pragma unroll 1
   for (uint32_t q = q0; q < q1; ++q) {
     const uint32_t p = pa.p[q], pinv = pa.pinv[q], r1 = pa.r1[q];
@@ -342,7 +368,7 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
       gc_u64 *lp = limbs + uoff;
       lt[i * Pad<LOGTC>::words(NT)] = limb_residue(uoff + toff < nl ? lp[toff] : 0, p, pinv, r1);
     }
-    tile_fwd<ELL, LOGTC, BMAX, 0>(lds, pa.wn[q], 0, p, pinv);
+    tile_fwd_first<ELL, LOGTC, BMAX>(lds, pa.wn[q], p, pinv, half_zero);
     __syncthreads();
     g_u32 *out = spec + q * N;
 #pragma unroll
@@ -447,7 +473,9 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
     }
     lt[i * Pad<0>::words(NT)] = v;
   }
-  tile_fwd<ELL, 0, BMAX, 0>(lds, pa.wn[q], H, p, pinv);
+  // LIMB_IN: one tile (H = 0) holds the whole transform of the padded limbs
+  if (LIMB_IN) tile_fwd_first<ELL, 0, BMAX>(lds, pa.wn[q], p, pinv, nb <= N / 2);
+  else tile_fwd<ELL, 0, BMAX, 0>(lds, pa.wn[q], H, p, pinv);
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < E; ++i) {
