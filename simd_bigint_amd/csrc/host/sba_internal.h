@@ -11,8 +11,11 @@ extern "C" {
 #endif
 
 /* total limbs (an+bn) from which simd_mpn_mul and the division's
- * multiplies go to the GPU; below, the library's CPU Karatsuba */
-#define SBA_GPU_MIN_TOTAL_DEFAULT 12288u
+ * multiplies go to the GPU; below, the library's CPU Karatsuba. The
+ * crossover tuner (sba_tune_gpu_threshold) measured 768 on an MI355X box
+ * (balanced products, host buffers, copies included: 384 x 384 limbs in
+ * 36 us on either side); rounded up to 1024 (57 us CPU, 37 us GPU). */
+#define SBA_GPU_MIN_TOTAL_DEFAULT 1024u
 
 /* rp[0..an+bn) = a * b through the threshold dispatch (either order) */
 void sba_mul(uint64_t *rp, const uint64_t *ap, size_t an, const uint64_t *bp, size_t bn);

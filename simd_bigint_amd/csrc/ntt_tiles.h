@@ -53,6 +53,9 @@ struct GarnerArgs {
 #ifndef SBA_FUSED2_WAVES
 #define SBA_FUSED2_WAVES 3
 #endif
+#ifndef SBA_COL_REGS
+#define SBA_COL_REGS 1       // column passes: first / last chunk straight from / to registers
+#endif
 #ifndef SBA_ZERO_LEVEL
 #define SBA_ZERO_LEVEL 1     // first level of a half-empty operand as a copy
 #endif
@@ -132,17 +135,31 @@ template <int ELL, int LOGTC, int T0, int B> struct ChunkGeom {
 };
 
 // ZL (forward only, first pass of a zero-padded operand): the levels below
-// ZL pair every element with a zero and become copies (x, 0) -> (x, x)
-template <int ELL, int LOGTC, int T0, int B, bool INV, int ZL = 0>
-__device__ __forceinline__ void tile_chunk(uint32_t *lds, const uint32_t *__restrict__ tw,
-                                           uint32_t H, uint32_t p, uint32_t pinv) {
+// ZL pair every element with a zero and become copies (x, 0) -> (x, x).
+// IO: where the chunk's groups come from and go to -- CHUNK_LDS (LDS to
+// LDS), CHUNK_TO_REGS (LDS in, results left in vv), CHUNK_FROM_REGS (vv in,
+// results to LDS). When a pass's level-0 chunk holds all E of a thread's
+// elements (B = log2 E), its groups are exactly the elements the thread
+// loads and stores (k = tid / TC + m * NT / TC, column tid % TC), so that
+// chunk can start from the loaded registers or end in the stored ones.
+#define CHUNK_LDS 0
+#define CHUNK_TO_REGS 1
+#define CHUNK_FROM_REGS 2
+template <int ELL, int LOGTC, int T0, int B> struct ChunkRegs {
+  static constexpr int GPT = (1 << (ELL + LOGTC)) / ((1 << B) * NT);
+  typedef uint32_t type[GPT][1 << B];
+};
+
+template <int ELL, int LOGTC, int T0, int B, bool INV, int ZL = 0, int IO = CHUNK_LDS>
+__device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__restrict__ tw,
+                                              uint32_t H, uint32_t p, uint32_t pinv,
+                                              typename ChunkRegs<ELL, LOGTC, T0, B>::type &vv) {
   constexpr int TC = 1 << LOGTC;
-  constexpr int NE = 1 << (ELL + LOGTC);
-  constexpr int GPT = NE / ((1 << B) * NT);
+  constexpr int GPT = ChunkRegs<ELL, LOGTC, T0, B>::GPT;
   using G = ChunkGeom<ELL, LOGTC, T0, B>;
   constexpr int LB = G::LB;
   static_assert(GPT >= 1, "chunk too wide for the tile");
-  __syncthreads();
+  if (IO != CHUNK_FROM_REGS) __syncthreads();
 #pragma unroll
   for (int gi = 0; gi < GPT; ++gi) {
     const uint32_t g = threadIdx.x + gi * NT;
@@ -157,7 +174,9 @@ __device__ __forceinline__ void tile_chunk(uint32_t *lds, const uint32_t *__rest
     uint32_t v[1 << B];
 #pragma unroll
     for (int m = 0; m < (1 << B); ++m)
-      v[m] = G::LINEAR ? l0[m * G::DP] : lds[lpadt<LOGTC>(i0 + ((uint32_t)m << G::LOGD))];
+      v[m] = IO == CHUNK_FROM_REGS
+                 ? vv[gi][m]
+                 : (G::LINEAR ? l0[m * G::DP] : lds[lpadt<LOGTC>(i0 + ((uint32_t)m << G::LOGD))]);
     if (!INV) {
 #pragma unroll
       for (int u = 0; u < B; ++u) {
@@ -195,10 +214,18 @@ __device__ __forceinline__ void tile_chunk(uint32_t *lds, const uint32_t *__rest
     }
 #pragma unroll
     for (int m = 0; m < (1 << B); ++m) {
-      if (G::LINEAR) l0[m * G::DP] = v[m];
+      if (IO == CHUNK_TO_REGS) vv[gi][m] = v[m];
+      else if (G::LINEAR) l0[m * G::DP] = v[m];
       else lds[lpadt<LOGTC>(i0 + ((uint32_t)m << G::LOGD))] = v[m];
     }
   }
+}
+
+template <int ELL, int LOGTC, int T0, int B, bool INV, int ZL = 0>
+__device__ __forceinline__ void tile_chunk(uint32_t *lds, const uint32_t *__restrict__ tw,
+                                           uint32_t H, uint32_t p, uint32_t pinv) {
+  typename ChunkRegs<ELL, LOGTC, T0, B>::type unused;
+  tile_chunk_io<ELL, LOGTC, T0, B, INV, ZL, CHUNK_LDS>(lds, tw, H, p, pinv, unused);
 }
 
 // chunk sizes: the largest B <= BM whose group addressing is linear,
@@ -226,6 +253,17 @@ __device__ __forceinline__ void tile_fwd(uint32_t *lds, const uint32_t *tw, uint
     constexpr int B = ChunkPlan<ELL, LOGTC, BMAX>::pick(T0);
     tile_chunk<ELL, LOGTC, T0, B, false, ZL>(lds, tw, H, p, pinv);
     tile_fwd<ELL, LOGTC, BMAX, T0 + B, ZL>(lds, tw, H, p, pinv);
+  }
+}
+
+// inverse chunks of the levels from T0 below TEND (highest first), LDS to LDS
+template <int ELL, int LOGTC, int BMAX, int T0, int TEND>
+__device__ __forceinline__ void tile_inv_upto(uint32_t *lds, const uint32_t *tw, uint32_t H,
+                                              uint32_t p, uint32_t pinv) {
+  if constexpr (T0 < TEND) {
+    constexpr int B = ChunkPlan<ELL, LOGTC, BMAX>::pick(T0);
+    tile_inv_upto<ELL, LOGTC, BMAX, T0 + B, TEND>(lds, tw, H, p, pinv);
+    tile_chunk<ELL, LOGTC, T0, B, true>(lds, tw, H, p, pinv);
   }
 }
 
@@ -357,18 +395,36 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
   // an operand of at most N/2 limbs pairs every element with a zero in the
   // first level (x, 0) -> (x, x)
   const bool half_zero = nl <= N / 2;
+  constexpr int B0 = ChunkPlan<ELL, LOGTC, BMAX>::pick(0);
+  constexpr bool DIRECT = SBA_COL_REGS && B0 == ChunkPlan<ELL, LOGTC, BMAX>::LE;
 #pragma unroll 1
   for (uint32_t q = q0; q < q1; ++q) {
     const uint32_t p = pa.p[q], pinv = pa.pinv[q], r1 = pa.r1[q];
     __syncthreads();
     // the tile's limbs are re-read per prime (cache-resident after the first)
+    if constexpr (DIRECT) {
+      // the level-0 chunk starts from the residues in registers
+      typename ChunkRegs<ELL, LOGTC, 0, B0>::type vv;
 #pragma unroll SBA_FIRST_LOAD_UNROLL
-    for (int i = 0; i < E; ++i) {
-      const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
-      gc_u64 *lp = limbs + uoff;
-      lt[i * Pad<LOGTC>::words(NT)] = limb_residue(uoff + toff < nl ? lp[toff] : 0, p, pinv, r1);
+      for (int i = 0; i < E; ++i) {
+        const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+        gc_u64 *lp = limbs + uoff;
+        vv[0][i] = limb_residue(uoff + toff < nl ? lp[toff] : 0, p, pinv, r1);
+      }
+      if (SBA_ZERO_LEVEL && half_zero)
+        tile_chunk_io<ELL, LOGTC, 0, B0, false, 1, CHUNK_FROM_REGS>(lds, pa.wn[q], 0, p, pinv, vv);
+      else
+        tile_chunk_io<ELL, LOGTC, 0, B0, false, 0, CHUNK_FROM_REGS>(lds, pa.wn[q], 0, p, pinv, vv);
+      tile_fwd<ELL, LOGTC, BMAX, B0>(lds, pa.wn[q], 0, p, pinv);
+    } else {
+#pragma unroll SBA_FIRST_LOAD_UNROLL
+      for (int i = 0; i < E; ++i) {
+        const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+        gc_u64 *lp = limbs + uoff;
+        lt[i * Pad<LOGTC>::words(NT)] = limb_residue(uoff + toff < nl ? lp[toff] : 0, p, pinv, r1);
+      }
+      tile_fwd_first<ELL, LOGTC, BMAX>(lds, pa.wn[q], p, pinv, half_zero);
     }
-    tile_fwd_first<ELL, LOGTC, BMAX>(lds, pa.wn[q], p, pinv, half_zero);
     __syncthreads();
     g_u32 *out = spec + q * N;
 #pragma unroll
@@ -402,14 +458,37 @@ __global__ __launch_bounds__(NT, INV ? SBA_INV_WAVES : SBA_WAVES) void k_pass(Pr
   const uint32_t p = pa.p[q], pinv = pa.pinv[q];
   uint32_t *lt = lds + lpadt<LOGTC>(threadIdx.x);
   const uint32_t toff = (threadIdx.x >> LOGTC) * S + (threadIdx.x & (TC - 1));
+  constexpr int B0 = ChunkPlan<ELL, LOGTC, BMAX>::pick(0);
+  constexpr bool DIRECT = SBA_COL_REGS && B0 == ChunkPlan<ELL, LOGTC, BMAX>::LE;
+  // DIRECT: the level-0 chunk starts from the loaded registers (forward) or
+  // ends in the registers that are stored (inverse)
+  typename ChunkRegs<ELL, LOGTC, 0, B0>::type vv;
+  if (DIRECT && !INV) {
 #pragma unroll SBA_PASS_LOAD_UNROLL
-  for (int i = 0; i < E; ++i) {
-    const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
-    lt[i * Pad<LOGTC>::words(NT)] = (buf + uoff)[toff];
+    for (int i = 0; i < E; ++i) {
+      const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+      vv[0][i] = (buf + uoff)[toff];
+    }
+    tile_chunk_io<ELL, LOGTC, 0, B0, false, 0, CHUNK_FROM_REGS>(lds, pa.wn[q], H, p, pinv, vv);
+    tile_fwd<ELL, LOGTC, BMAX, B0>(lds, pa.wn[q], H, p, pinv);
+  } else {
+#pragma unroll SBA_PASS_LOAD_UNROLL
+    for (int i = 0; i < E; ++i) {
+      const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+      lt[i * Pad<LOGTC>::words(NT)] = (buf + uoff)[toff];
+    }
+    if (!INV) {
+      tile_fwd<ELL, LOGTC, BMAX, 0>(lds, pa.wn[q], H, p, pinv);
+    } else if (DIRECT) {
+      tile_inv_upto<ELL, LOGTC, BMAX, B0, ELL>(lds, pa.wni[q], H, p, pinv);
+      tile_chunk_io<ELL, LOGTC, 0, B0, true, 0, CHUNK_TO_REGS>(lds, pa.wni[q], H, p, pinv, vv);
+    } else {
+      tile_inv<ELL, LOGTC, BMAX, 0>(lds, pa.wni[q], H, p, pinv);
+    }
   }
-  if (!INV) tile_fwd<ELL, LOGTC, BMAX, 0>(lds, pa.wn[q], H, p, pinv);
-  else tile_inv<ELL, LOGTC, BMAX, 0>(lds, pa.wni[q], H, p, pinv);
-  __syncthreads();
+  // the inverse's element i: in registers (DIRECT) or in LDS
+  const bool inreg = DIRECT && INV;
+  if (!inreg) __syncthreads();
   // the inverse's last levels (s0 == 0) finish the transform: scale here
   const bool fin = INV && s0 == 0;
   const uint32_t sc = pa.scale[q];
@@ -423,7 +502,7 @@ __global__ __launch_bounds__(NT, INV ? SBA_INV_WAVES : SBA_WAVES) void k_pass(Pr
       const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
       const uint32_t x = base + uoff + toff;
       if (x < acc.lim) {
-        uint32_t v = mont_mul(lt[i * Pad<LOGTC>::words(NT)], sc, p, pinv);
+        uint32_t v = mont_mul(inreg ? vv[0][i] : lt[i * Pad<LOGTC>::words(NT)], sc, p, pinv);
         if (x < acc.ovl) v = red1(v, p) + red1((d + uoff)[toff], p);
         (d + uoff)[toff] = v;
       }
@@ -433,7 +512,7 @@ __global__ __launch_bounds__(NT, INV ? SBA_INV_WAVES : SBA_WAVES) void k_pass(Pr
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
-    const uint32_t v = lt[i * Pad<LOGTC>::words(NT)];
+    const uint32_t v = inreg ? vv[0][i] : lt[i * Pad<LOGTC>::words(NT)];
     (buf + uoff)[toff] = fin ? mont_mul(v, sc, p, pinv) : v;
   }
 }
