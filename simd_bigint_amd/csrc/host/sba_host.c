@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime_api.h>
 #include <pthread.h>
+#include <stdatomic.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -373,10 +374,14 @@ static void cpu_mul(uint64_t *rp, const uint64_t *ap, size_t an, const uint64_t 
 /* simd_mpn_mul: include/u64cvt.h:94-132 semantics (swap to an >= bn,  */
 /* smallest sizes on the basecase, threshold dispatch above).          */
 /* ------------------------------------------------------------------ */
-static uint64_t g_gpu_min_total = SBA_GPU_MIN_TOTAL_DEFAULT;
+static _Atomic uint64_t g_gpu_min_total = SBA_GPU_MIN_TOTAL_DEFAULT;
 
-void sba_set_gpu_threshold(uint64_t min_total_limbs) { g_gpu_min_total = min_total_limbs; }
-uint64_t sba_get_gpu_threshold(void) { return g_gpu_min_total; }
+void sba_set_gpu_threshold(uint64_t min_total_limbs) {
+  atomic_store_explicit(&g_gpu_min_total, min_total_limbs, memory_order_relaxed);
+}
+uint64_t sba_get_gpu_threshold(void) {
+  return atomic_load_explicit(&g_gpu_min_total, memory_order_relaxed);
+}
 
 /* ------------------------------------------------------------------ */
 /* crossover tuner for the threshold above (the reference's tier       */
@@ -434,7 +439,7 @@ uint64_t sba_tune_gpu_threshold(int apply, double *cpu_ms, double *gpu_ms) {
   free(a);
   free(r);
   const uint64_t t = gpu_wins_from < TUNE_N ? kTuneTotals[gpu_wins_from] : 2 * top;
-  if (apply) g_gpu_min_total = t;
+  if (apply) sba_set_gpu_threshold(t);
   return t;
 }
 
@@ -448,7 +453,7 @@ void simd_mpn_mul(uint64_t *rp, const uint64_t *ap, uint64_t an, const uint64_t 
     if (an) memset(rp, 0, an * 8);
     return;
   }
-  if (an + bn >= g_gpu_min_total) {
+  if (an + bn >= sba_get_gpu_threshold()) {
     int rc = gpu_mul_host(rp, ap, an, bp, bn, ap == bp && an == bn);
     if (rc == SBA_OK) return;
     /* beyond the GPU band (operands of > 43M limbs): split a in halves */

@@ -31,6 +31,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <atomic>
 #include <mutex>
 
 // ---------------------------------------------------------------------------
@@ -257,12 +258,13 @@ __global__ __launch_bounds__(NT) void k_school(JobRef jr) {
 // latency dominate). Measured on MI355X: 512-job batches and single calls of
 // 16x16 .. 1024x1024 and 16x2032 .. 256x1792 limbs. sba_set_school_max sets
 // g_school_pmax; 0 turns the tier off.
-static uint64_t g_school_pmax = SBA_SCHOOL_PMAX_DEFAULT;
+static std::atomic<uint64_t> g_school_pmax{SBA_SCHOOL_PMAX_DEFAULT};
 static bool school_job(uint64_t an, uint64_t bn, bool single) {
   if (an + bn > SCHOOL_ZMAX) return false;
   uint64_t N = 1024;
   while (N < an + bn - 1) N <<= 1;
-  const uint64_t lim = (single ? g_school_pmax / 2 : g_school_pmax) * (N / 1024);
+  const uint64_t pmax = g_school_pmax.load(std::memory_order_relaxed);
+  const uint64_t lim = (single ? pmax / 2 : pmax) * (N / 1024);
   return an * bn <= lim;
 }
 
@@ -1486,10 +1488,11 @@ extern "C" int sba_job_class(const sba_job *j, int *L, int *kind) {
 }
 
 extern "C" void sba_set_school_max(uint64_t max_product) {
-  std::lock_guard<std::mutex> lk(g_mu);
-  g_school_pmax = max_product;
+  g_school_pmax.store(max_product, std::memory_order_relaxed);
 }
-extern "C" uint64_t sba_get_school_max(void) { return g_school_pmax; }
+extern "C" uint64_t sba_get_school_max(void) {
+  return g_school_pmax.load(std::memory_order_relaxed);
+}
 
 extern "C" int sba_plan_bucket(int njobs) { return plan_bucket(njobs < 1 ? 1u : (uint32_t)njobs); }
 
