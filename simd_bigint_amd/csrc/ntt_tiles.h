@@ -54,7 +54,10 @@ struct GarnerArgs {
 #define SBA_FUSED2_WAVES 3
 #endif
 #ifndef SBA_COL_REGS
-#define SBA_COL_REGS 1       // column passes: first / last chunk straight from / to registers
+#define SBA_COL_REGS 1       // column passes: level-0 chunk straight from / to registers
+#endif
+#ifndef SBA_COL_EDGE
+#define SBA_COL_EDGE 1       // column passes of >= 8 columns: the chunk at the other end too
 #endif
 #ifndef SBA_ZERO_LEVEL
 #define SBA_ZERO_LEVEL 1     // first level of a half-empty operand as a copy
@@ -145,6 +148,7 @@ template <int ELL, int LOGTC, int T0, int B> struct ChunkGeom {
 #define CHUNK_LDS 0
 #define CHUNK_TO_REGS 1
 #define CHUNK_FROM_REGS 2
+#define CHUNK_REGS 3          // vv in, results left in vv (a pass of one chunk)
 template <int ELL, int LOGTC, int T0, int B> struct ChunkRegs {
   static constexpr int GPT = (1 << (ELL + LOGTC)) / ((1 << B) * NT);
   typedef uint32_t type[GPT][1 << B];
@@ -159,7 +163,9 @@ __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__r
   using G = ChunkGeom<ELL, LOGTC, T0, B>;
   constexpr int LB = G::LB;
   static_assert(GPT >= 1, "chunk too wide for the tile");
-  if (IO != CHUNK_FROM_REGS) __syncthreads();
+  constexpr bool IN_REGS = IO == CHUNK_FROM_REGS || IO == CHUNK_REGS;
+  constexpr bool OUT_REGS = IO == CHUNK_TO_REGS || IO == CHUNK_REGS;
+  if (!IN_REGS) __syncthreads();
 #pragma unroll
   for (int gi = 0; gi < GPT; ++gi) {
     const uint32_t g = threadIdx.x + gi * NT;
@@ -174,7 +180,7 @@ __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__r
     uint32_t v[1 << B];
 #pragma unroll
     for (int m = 0; m < (1 << B); ++m)
-      v[m] = IO == CHUNK_FROM_REGS
+      v[m] = IN_REGS
                  ? vv[gi][m]
                  : (G::LINEAR ? l0[m * G::DP] : lds[lpadt<LOGTC>(i0 + ((uint32_t)m << G::LOGD))]);
     if (!INV) {
@@ -214,7 +220,7 @@ __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__r
     }
 #pragma unroll
     for (int m = 0; m < (1 << B); ++m) {
-      if (IO == CHUNK_TO_REGS) vv[gi][m] = v[m];
+      if (OUT_REGS) vv[gi][m] = v[m];
       else if (G::LINEAR) l0[m * G::DP] = v[m];
       else lds[lpadt<LOGTC>(i0 + ((uint32_t)m << G::LOGD))] = v[m];
     }
@@ -266,6 +272,54 @@ __device__ __forceinline__ void tile_inv_upto(uint32_t *lds, const uint32_t *tw,
     tile_chunk<ELL, LOGTC, T0, B, true>(lds, tw, H, p, pinv);
   }
 }
+
+// forward chunks from T0 up to (not including) level TEND, LDS to LDS
+template <int ELL, int LOGTC, int BMAX, int T0, int TEND>
+__device__ __forceinline__ void tile_fwd_range(uint32_t *lds, const uint32_t *tw, uint32_t H,
+                                               uint32_t p, uint32_t pinv) {
+  if constexpr (T0 < TEND) {
+    constexpr int B = ChunkPlan<ELL, LOGTC, BMAX>::pick(T0);
+    tile_chunk<ELL, LOGTC, T0, B, false>(lds, tw, H, p, pinv);
+    tile_fwd_range<ELL, LOGTC, BMAX, T0 + B, TEND>(lds, tw, H, p, pinv);
+  }
+}
+
+// the last chunk of a tile transform (the highest levels): its first level
+template <int ELL, int LOGTC, int BMAX> struct LastChunk {
+  static constexpr int t0() {
+    int t = 0;
+    for (;;) {
+      const int b = ChunkPlan<ELL, LOGTC, BMAX>::pick(t);
+      if (t + b >= ELL) return t;
+      t += b;
+    }
+  }
+  static constexpr int T0 = t0();
+  static constexpr int B = ELL - T0;
+};
+
+// Global offsets of the elements of chunk (T0, B)'s groups, for a pass that
+// starts or ends with that chunk straight from or to HBM: element m of the
+// thread's group gi is tile row k, column c, at k * S + c. With R = NT / TC
+// rows per group sweep and 2^LB | R, k splits into a per-thread part and a
+// part uniform over the workgroup: off = thr + (gi * (R >> LB) * 2^(ELL -
+// T0) + m * 2^LB) * S, so every access is a scalar base plus one 32-bit
+// lane offset (as the passes' own loads and stores are).
+template <int ELL, int LOGTC, int T0, int B> struct ChunkIO {
+  using G = ChunkGeom<ELL, LOGTC, T0, B>;
+  static constexpr int R = NT >> LOGTC;
+  static constexpr int lg(int x) { return x > 1 ? 1 + lg(x >> 1) : 0; }
+  static constexpr bool OK = LOGTC >= 3 && R >= 1 && G::LB <= lg(R);
+  static __device__ __forceinline__ uint32_t thr(uint32_t S) {
+    const uint32_t row = threadIdx.x >> LOGTC;
+    const uint32_t k = ((row >> G::LB) << (ELL - T0)) + (row & ((1u << G::LB) - 1));
+    return k * S + (threadIdx.x & ((1u << LOGTC) - 1));
+  }
+  static __device__ __forceinline__ uint32_t uni(int gi, int m, uint32_t S) {
+    return ((uint32_t)gi * (uint32_t)(R >> G::LB) * (1u << (ELL - T0)) + ((uint32_t)m << G::LB)) *
+           S;
+  }
+};
 
 // the forward levels of a first pass, with its first level a copy when the
 // operand fills at most half of the transform (SBA_ZERO_LEVEL; one uniform
@@ -397,6 +451,10 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
   const bool half_zero = nl <= N / 2;
   constexpr int B0 = ChunkPlan<ELL, LOGTC, BMAX>::pick(0);
   constexpr bool DIRECT = SBA_COL_REGS && B0 == ChunkPlan<ELL, LOGTC, BMAX>::LE;
+  // EDGE: the last chunk's groups go straight to HBM as well (TC >= 8
+  // columns keep 32-byte row segments in that layout too)
+  using LC = LastChunk<ELL, LOGTC, BMAX>;
+  constexpr bool EDGE = DIRECT && SBA_COL_EDGE && ChunkIO<ELL, LOGTC, LC::T0, LC::B>::OK;
 #pragma unroll 1
   for (uint32_t q = q0; q < q1; ++q) {
     const uint32_t p = pa.p[q], pinv = pa.pinv[q], r1 = pa.r1[q];
@@ -411,10 +469,38 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
         gc_u64 *lp = limbs + uoff;
         vv[0][i] = limb_residue(uoff + toff < nl ? lp[toff] : 0, p, pinv, r1);
       }
+      if constexpr (EDGE && LC::T0 == 0) {
+        // one chunk: registers in, registers out
+        if (SBA_ZERO_LEVEL && half_zero)
+          tile_chunk_io<ELL, LOGTC, 0, B0, false, 1, CHUNK_REGS>(lds, pa.wn[q], 0, p, pinv, vv);
+        else
+          tile_chunk_io<ELL, LOGTC, 0, B0, false, 0, CHUNK_REGS>(lds, pa.wn[q], 0, p, pinv, vv);
+        using IO = ChunkIO<ELL, LOGTC, 0, B0>;
+        g_u32 *out = spec + q * N + col0;
+        const uint32_t t = IO::thr(S);
+#pragma unroll
+        for (int m = 0; m < (1 << B0); ++m) (out + IO::uni(0, m, S))[t] = vv[0][m];
+        continue;
+      }
       if (SBA_ZERO_LEVEL && half_zero)
         tile_chunk_io<ELL, LOGTC, 0, B0, false, 1, CHUNK_FROM_REGS>(lds, pa.wn[q], 0, p, pinv, vv);
       else
         tile_chunk_io<ELL, LOGTC, 0, B0, false, 0, CHUNK_FROM_REGS>(lds, pa.wn[q], 0, p, pinv, vv);
+      if constexpr (EDGE) {
+        tile_fwd_range<ELL, LOGTC, BMAX, B0, LC::T0>(lds, pa.wn[q], 0, p, pinv);
+        typename ChunkRegs<ELL, LOGTC, LC::T0, LC::B>::type vl;
+        tile_chunk_io<ELL, LOGTC, LC::T0, LC::B, false, 0, CHUNK_TO_REGS>(lds, pa.wn[q], 0, p, pinv,
+                                                                       vl);
+        using IO = ChunkIO<ELL, LOGTC, LC::T0, LC::B>;
+        g_u32 *out = spec + q * N + col0;
+        const uint32_t t = IO::thr(S);
+        constexpr int GL = ChunkRegs<ELL, LOGTC, LC::T0, LC::B>::GPT;
+#pragma unroll
+        for (int gi = 0; gi < GL; ++gi)
+#pragma unroll
+          for (int m = 0; m < (1 << LC::B); ++m) (out + IO::uni(gi, m, S))[t] = vl[gi][m];
+        continue;
+      }
       tile_fwd<ELL, LOGTC, BMAX, B0>(lds, pa.wn[q], 0, p, pinv);
     } else {
 #pragma unroll SBA_FIRST_LOAD_UNROLL
@@ -461,9 +547,34 @@ __global__ __launch_bounds__(NT, INV ? SBA_INV_WAVES : SBA_WAVES) void k_pass(Pr
   constexpr int B0 = ChunkPlan<ELL, LOGTC, BMAX>::pick(0);
   constexpr bool DIRECT = SBA_COL_REGS && B0 == ChunkPlan<ELL, LOGTC, BMAX>::LE;
   // DIRECT: the level-0 chunk starts from the loaded registers (forward) or
-  // ends in the registers that are stored (inverse)
+  // ends in the registers that are stored (inverse); EDGE (inverse, TC >= 8
+  // columns): the inverse's first chunk, the tile's highest levels, also
+  // loads its groups straight from HBM
+  using LC = LastChunk<ELL, LOGTC, BMAX>;
+  constexpr bool EDGE =
+      DIRECT && INV && SBA_COL_EDGE && ChunkIO<ELL, LOGTC, LC::T0, LC::B>::OK;
   typename ChunkRegs<ELL, LOGTC, 0, B0>::type vv;
-  if (DIRECT && !INV) {
+  if constexpr (EDGE && LC::T0 == 0) {
+    // one chunk: registers in, registers out
+    using IO = ChunkIO<ELL, LOGTC, 0, B0>;
+    const uint32_t t = IO::thr(S);
+#pragma unroll
+    for (int m = 0; m < (1 << B0); ++m) vv[0][m] = (buf + IO::uni(0, m, S))[t];
+    tile_chunk_io<ELL, LOGTC, 0, B0, true, 0, CHUNK_REGS>(lds, pa.wni[q], H, p, pinv, vv);
+  } else if constexpr (EDGE) {
+    using IO = ChunkIO<ELL, LOGTC, LC::T0, LC::B>;
+    typename ChunkRegs<ELL, LOGTC, LC::T0, LC::B>::type vf;
+    constexpr int GL = ChunkRegs<ELL, LOGTC, LC::T0, LC::B>::GPT;
+    const uint32_t t = IO::thr(S);
+#pragma unroll
+    for (int gi = 0; gi < GL; ++gi)
+#pragma unroll
+      for (int m = 0; m < (1 << LC::B); ++m) vf[gi][m] = (buf + IO::uni(gi, m, S))[t];
+    tile_chunk_io<ELL, LOGTC, LC::T0, LC::B, true, 0, CHUNK_FROM_REGS>(lds, pa.wni[q], H, p, pinv,
+                                                                     vf);
+    tile_inv_upto<ELL, LOGTC, BMAX, B0, LC::T0>(lds, pa.wni[q], H, p, pinv);
+    tile_chunk_io<ELL, LOGTC, 0, B0, true, 0, CHUNK_TO_REGS>(lds, pa.wni[q], H, p, pinv, vv);
+  } else if (DIRECT && !INV) {
 #pragma unroll SBA_PASS_LOAD_UNROLL
     for (int i = 0; i < E; ++i) {
       const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
