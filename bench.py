@@ -448,8 +448,8 @@ def profiled_traffic(name):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--batch-jobs", type=int, default=4096,
                     help="jobs in the batched vs per-call comparison (0 = skip)")
     ap.add_argument("--configs", type=int, default=1,
@@ -523,6 +523,7 @@ def main():
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / args.steps
     log("bench: %.4f ms per multiply" % ms)
+
 
     # per-kernel timing pass (events around each launch on the same stream)
     L.sba_gpu_profile_reset()
