@@ -7,9 +7,10 @@
 // device pointers, and the linear steps between them are kernels -- the
 // normalising shifts, an add/subtract with a carry-lookahead scan, the
 // complement B^k - T (an add of ~T and 1), the comparison of the correction
-// loop. The operands go up and the quotient and remainder come back through
-// pinned staging; only the reciprocal's recursion at or below
-// DIV_HOST_RECIP limbs and its rare overflow fix-up run on the host.
+// loop, the reciprocal's overflow fix-up. The operands go up and the
+// quotient and remainder come back through pinned staging, small words
+// through a pinned mailbox; only the reciprocal's recursion at or below
+// DIV_HOST_RECIP limbs runs on the host.
 #include "sba_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -168,12 +169,17 @@ __global__ __launch_bounds__(DNT) void k_shr(u64 *__restrict__ dst, const u64 *_
 }
 
 // *top = max(*top, i + 1) over the limbs where a[i] != b[i] (b zero-extended
-// from bn limbs): 0 when a and b are equal over n limbs
+// from bn limbs): 0 when a and b are equal over n limbs; one atomic per block
 __global__ __launch_bounds__(DNT) void k_top_diff(const u64 *__restrict__ a,
                                                   const u64 *__restrict__ b, size_t bn, size_t n,
                                                   unsigned long long *__restrict__ top) {
+  __shared__ unsigned long long bmax;
+  if (threadIdx.x == 0) bmax = 0;
+  __syncthreads();
   const size_t i = (size_t)blockIdx.x * DNT + threadIdx.x;
-  if (i < n && a[i] != (i < bn ? b[i] : 0)) atomicMax(top, (unsigned long long)(i + 1));
+  if (i < n && a[i] != (i < bn ? b[i] : 0)) atomicMax(&bmax, (unsigned long long)(i + 1));
+  __syncthreads();
+  if (threadIdx.x == 0 && bmax) atomicMax(top, bmax);
 }
 
 // ---------------------------------------------------------------------------
@@ -196,6 +202,7 @@ struct DivPool {
   uint32_t *bm = nullptr, *cw = nullptr;
   size_t lmcap = 0;
   unsigned long long *word = nullptr;   // k_top_diff's result
+  u64 *mbox = nullptr;                  // pinned host words for small read-backs
   u64 *pin = nullptr;               // pinned host staging (grow-only)
   size_t pincap = 0;
   u64 *take(size_t n) {
@@ -221,8 +228,9 @@ static int dev_add(u64 *r, const u64 *a, const u64 *b, size_t bn, size_t n, int 
                      cout_host ? g_dp.cw : nullptr);
   DCHK(hipGetLastError());
   if (cout_host) {
-    DCHK(hipMemcpyAsync(cout_host, g_dp.cw, 4, hipMemcpyDeviceToHost, st));
+    DCHK(hipMemcpyAsync(g_dp.mbox + 1, g_dp.cw, 4, hipMemcpyDeviceToHost, st));
     DCHK(hipStreamSynchronize(st));
+    *cout_host = (uint32_t)(g_dp.mbox[1] & 1u);
   }
   return SBA_OK;
 }
@@ -232,26 +240,18 @@ static int dev_mul(u64 *r, const u64 *a, size_t an, const u64 *b, size_t bn, hip
   return rc == SBA_OK ? SBA_OK : (rc == SBA_BAND ? SBA_ERR_WORKSPACE : rc);
 }
 
-static u64 *host_buf(size_t n) {
-  u64 *p = (u64 *)malloc((n ? n : 1) * 8);
-  if (!p) {
-    fprintf(stderr, "simd_bigint_amd: out of host memory in division\n");
-    abort();
-  }
-  return p;
-}
-
 // X[0..n] (device) = approximate reciprocal of A[0..n) (device, top bit set)
 static int recip_dev(u64 *X, const u64 *A, size_t n, hipStream_t st) {
   if (n <= DIV_HOST_RECIP) {
-    u64 *ha = host_buf(n), *hx = host_buf(n + 1);
+    // through the pinned staging buffer (idle here: the uploads before this
+    // point have completed once the stream is synchronized)
+    u64 *ha = g_dp.pin, *hx = g_dp.pin + n;
+    DCHK(hipStreamSynchronize(st));
     DCHK(hipMemcpyAsync(ha, A, n * 8, hipMemcpyDeviceToHost, st));
     DCHK(hipStreamSynchronize(st));
     sba_approx_recip_host(hx, ha, n);
     DCHK(hipMemcpyAsync(X, hx, (n + 1) * 8, hipMemcpyHostToDevice, st));
     DCHK(hipStreamSynchronize(st));
-    free(ha);
-    free(hx);
     return SBA_OK;
   }
   const size_t l = (n - 1) / 2, h = n - l;
@@ -264,34 +264,15 @@ static int recip_dev(u64 *X, const u64 *A, size_t n, hipStream_t st) {
   if (rc != SBA_OK) return rc;
   // T = A * Xh (n + h + 1 limbs)
   if ((rc = dev_mul(T, A, n, Xh, h + 1, st)) != SBA_OK) return rc;
-  u64 top = 0;
-  DCHK(hipMemcpyAsync(&top, T + n + h, 8, hipMemcpyDeviceToHost, st));
-  DCHK(hipStreamSynchronize(st));
-  if (top != 0) {
-    // T >= B^(n+h): Xh -= 1, T -= A until below (rare; on the host)
-    u64 *hx = host_buf(h + 1), *ht = host_buf(n + h + 1), *hA = host_buf(n);
-    DCHK(hipMemcpyAsync(hx, Xh, (h + 1) * 8, hipMemcpyDeviceToHost, st));
-    DCHK(hipMemcpyAsync(ht, T, (n + h + 1) * 8, hipMemcpyDeviceToHost, st));
-    DCHK(hipMemcpyAsync(hA, A, n * 8, hipMemcpyDeviceToHost, st));
+  // while T >= B^(n+h): Xh -= 1, T -= A, on the device. T < B^(n+h) + 2 B^n
+  // (A_h Xh < B^2h, Xh < 2 B^h) and A >= B^n / 2, so at most 4 rounds.
+  for (int round = 0;; ++round) {
+    DCHK(hipMemcpyAsync(g_dp.mbox, T + n + h, 8, hipMemcpyDeviceToHost, st));
     DCHK(hipStreamSynchronize(st));
-    while (ht[n + h] != 0) {
-      for (size_t i = 0; i <= h && hx[i]-- == 0; ++i) {
-      }
-      u64 bw = 0;
-      for (size_t i = 0; i < n + h + 1; ++i) {
-        const u64 y = i < n ? hA[i] : 0;
-        const u64 x = ht[i];
-        const u64 d = x - y - bw;
-        bw = (x < y) || (x - y < bw);
-        ht[i] = d;
-      }
-    }
-    DCHK(hipMemcpyAsync(Xh, hx, (h + 1) * 8, hipMemcpyHostToDevice, st));
-    DCHK(hipMemcpyAsync(T, ht, (n + h + 1) * 8, hipMemcpyHostToDevice, st));
-    DCHK(hipStreamSynchronize(st));
-    free(hx);
-    free(ht);
-    free(hA);
+    if (g_dp.mbox[0] == 0) break;
+    if (round == 4) return SBA_BAND;
+    if ((rc = dev_add(Xh, Xh, nullptr, 0, h + 1, 1, 0, nullptr, st)) != SBA_OK) return rc;
+    if ((rc = dev_add(T, T, A, n, n + h + 1, 1, 1, nullptr, st)) != SBA_OK) return rc;
   }
   // T = B^(n+h) - T over n + h limbs: ~T + 1
   if ((rc = dev_add(T, nullptr, T, n + h, n + h, 1, 1, nullptr, st)) != SBA_OK) return rc;
@@ -329,6 +310,7 @@ static int grow_pool(size_t limbs, size_t addmax) {
     g_dp.lmcap = lmneed;
   }
   if (!g_dp.word) DCHK(hipMalloc(&g_dp.word, 64));
+  if (!g_dp.mbox) DCHK(hipHostMalloc((void **)&g_dp.mbox, 64, hipHostMallocDefault));
   g_dp.top = 0;
   return SBA_OK;
 }
@@ -351,19 +333,19 @@ static int dev_cmp(const u64 *a, size_t n, const u64 *b, size_t bn, int *sign, h
   DCHK(hipMemsetAsync(g_dp.word, 0, 8, st));
   hipLaunchKernelGGL(k_top_diff, dim3(grid_for(n)), dim3(DNT), 0, st, a, b, bn, n, g_dp.word);
   DCHK(hipGetLastError());
-  unsigned long long top = 0;
-  DCHK(hipMemcpyAsync(&top, g_dp.word, 8, hipMemcpyDeviceToHost, st));
+  DCHK(hipMemcpyAsync(g_dp.mbox + 2, g_dp.word, 8, hipMemcpyDeviceToHost, st));
   DCHK(hipStreamSynchronize(st));
+  const u64 top = g_dp.mbox[2];
   if (top == 0) {
     *sign = 0;
     return SBA_OK;
   }
   const size_t i = (size_t)top - 1;
-  u64 x = 0, y = 0;
-  DCHK(hipMemcpyAsync(&x, a + i, 8, hipMemcpyDeviceToHost, st));
-  if (i < bn) DCHK(hipMemcpyAsync(&y, b + i, 8, hipMemcpyDeviceToHost, st));
+  g_dp.mbox[3] = g_dp.mbox[4] = 0;
+  DCHK(hipMemcpyAsync(g_dp.mbox + 3, a + i, 8, hipMemcpyDeviceToHost, st));
+  if (i < bn) DCHK(hipMemcpyAsync(g_dp.mbox + 4, b + i, 8, hipMemcpyDeviceToHost, st));
   DCHK(hipStreamSynchronize(st));
-  *sign = x > y ? 1 : -1;
+  *sign = g_dp.mbox[3] > g_dp.mbox[4] ? 1 : -1;
   return SBA_OK;
 }
 
@@ -412,9 +394,9 @@ extern "C" int sba_div_dev(u64 *qp, size_t qm, u64 *rp, const u64 *np, size_t nt
     DCHK(hipMemcpyAsync(dDm + (m - dn), dD, dn * 8, hipMemcpyDeviceToDevice, st));
   }
   if (exact_pow) {                   // X = B^2m / B^m = B^m
-    const u64 one = 1;
+    g_dp.mbox[6] = 1;
     DCHK(hipMemsetAsync(dX, 0, m * 8, st));
-    DCHK(hipMemcpyAsync(dX + m, &one, 8, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(dX + m, g_dp.mbox + 6, 8, hipMemcpyHostToDevice, st));
     DCHK(hipStreamSynchronize(st));
   } else if ((rc = recip_dev(dX, dDm, m, st)) != SBA_OK) {
     return rc;
@@ -422,10 +404,9 @@ extern "C" int sba_div_dev(u64 *qp, size_t qm, u64 *rp, const u64 *np, size_t nt
   // Qest = floor(Nt * X / B^(qn + 2)), Nt = N' >> 64 (dn - 1): qn + 1 limbs
   if ((rc = dev_mul(dP, dN + (dn - 1), qn + 1, dX, m + 1, st)) != SBA_OK) return rc;
   u64 *dQ = dP + (qn + 2);
-  u64 qtop = 0;
-  DCHK(hipMemcpyAsync(&qtop, dQ + qn, 8, hipMemcpyDeviceToHost, st));
+  DCHK(hipMemcpyAsync(g_dp.mbox, dQ + qn, 8, hipMemcpyDeviceToHost, st));
   DCHK(hipStreamSynchronize(st));
-  if (qtop != 0) return SBA_BAND;
+  if (g_dp.mbox[0] != 0) return SBA_BAND;
   // R = N' - Qest D' over nt + 1 limbs (no borrow: carry out 1)
   if ((rc = dev_mul(dQD, dQ, qn, dD, dn, st)) != SBA_OK) return rc;
   uint32_t cout = 0;
@@ -438,10 +419,9 @@ extern "C" int sba_div_dev(u64 *qp, size_t qm, u64 *rp, const u64 *np, size_t nt
     hipLaunchKernelGGL(k_any_nonzero, dim3(grid_for(nt1 - (dn + 1))), dim3(DNT), 0, st,
                        (const u64 *)(dQD + dn + 1), nt1 - (dn + 1), g_dp.cw);
     DCHK(hipGetLastError());
-    uint32_t any = 0;
-    DCHK(hipMemcpyAsync(&any, g_dp.cw, 4, hipMemcpyDeviceToHost, st));
+    DCHK(hipMemcpyAsync(g_dp.mbox + 5, g_dp.cw, 4, hipMemcpyDeviceToHost, st));
     DCHK(hipStreamSynchronize(st));
-    if (any) return SBA_BAND;
+    if (g_dp.mbox[5] & 0xFFFFFFFFu) return SBA_BAND;
   }
   // while R >= D': R -= D', Q += 1 (R over dn + 1 limbs)
   for (int fix = 0;; ++fix) {
