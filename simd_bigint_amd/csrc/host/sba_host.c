@@ -107,6 +107,53 @@ static void par_copy(void *dst, const void *src, size_t bytes) {
 
 void sba_par_copy(void *dst, const void *src, size_t bytes) { par_copy(dst, src, bytes); }
 
+/* Staged copies in pieces: the host copy of piece k+1 (caller memory <->
+ * pinned staging, several threads) overlaps the DMA of piece k. */
+#define STAGE_PIECE ((size_t)8 << 20)            /* bytes */
+#define STAGE_EVENTS 64
+static hipEvent_t g_stage_ev[STAGE_EVENTS];
+
+static size_t stage_piece(size_t bytes) {
+  size_t piece = STAGE_PIECE;
+  while (bytes / piece >= STAGE_EVENTS) piece *= 2;
+  return piece;
+}
+
+/* dev <- host through the pinned buffer pin, on the default stream */
+static void stage_up(void *dev, const void *host, void *pin, size_t bytes) {
+  const size_t piece = stage_piece(bytes);
+  for (size_t off = 0; off < bytes; off += piece) {
+    const size_t n = bytes - off < piece ? bytes - off : piece;
+    par_copy((char *)pin + off, (const char *)host + off, n);
+    if (hipMemcpyAsync((char *)dev + off, (char *)pin + off, n, hipMemcpyHostToDevice, NULL) !=
+        hipSuccess)
+      die("hipMemcpyAsync H2D", SBA_ERR_HIP);
+  }
+}
+
+/* host <- dev through the pinned buffer pin: every piece's DMA is queued with
+ * an event behind it, and each piece is copied out as soon as it lands */
+static void stage_down(void *host, const void *dev, void *pin, size_t bytes) {
+  const size_t piece = stage_piece(bytes);
+  int k = 0;
+  for (size_t off = 0; off < bytes; off += piece, ++k) {
+    const size_t n = bytes - off < piece ? bytes - off : piece;
+    if (!g_stage_ev[k] && hipEventCreateWithFlags(&g_stage_ev[k], hipEventDisableTiming) !=
+                              hipSuccess)
+      die("hipEventCreate", SBA_ERR_HIP);
+    if (hipMemcpyAsync((char *)pin + off, (const char *)dev + off, n, hipMemcpyDeviceToHost,
+                       NULL) != hipSuccess ||
+        hipEventRecord(g_stage_ev[k], NULL) != hipSuccess)
+      die("hipMemcpyAsync D2H", SBA_ERR_HIP);
+  }
+  k = 0;
+  for (size_t off = 0; off < bytes; off += piece, ++k) {
+    const size_t n = bytes - off < piece ? bytes - off : piece;
+    if (hipEventSynchronize(g_stage_ev[k]) != hipSuccess) die("hipEventSynchronize", SBA_ERR_HIP);
+    par_copy((char *)host + off, (const char *)pin + off, n);
+  }
+}
+
 /* H2D, multiply on the default stream, D2H. Returns the engine's code. */
 static int gpu_in_band(size_t an, size_t bn) {
   return sba_gpu_in_band((int64_t)an, (int64_t)bn);
@@ -123,23 +170,14 @@ static int gpu_mul_host(uint64_t *rp, const uint64_t *ap, size_t an,
   grow_pinned(&g_ha, &g_hna, an);
   if (!sqr) grow_pinned(&g_hb, &g_hnb, bn);
   grow_pinned(&g_hr, &g_hnr, an + bn);
-  par_copy(g_ha, ap, an * 8);
-  if (hipMemcpyAsync(g_da, g_ha, an * 8, hipMemcpyHostToDevice, NULL) != hipSuccess)
-    die("hipMemcpyAsync H2D", SBA_ERR_HIP);
-  if (!sqr) {
-    par_copy(g_hb, bp, bn * 8);                 /* overlaps the first DMA */
-    if (hipMemcpyAsync(g_db, g_hb, bn * 8, hipMemcpyHostToDevice, NULL) != hipSuccess)
-      die("hipMemcpyAsync H2D", SBA_ERR_HIP);
-  }
+  stage_up(g_da, ap, g_ha, an * 8);
+  if (!sqr) stage_up(g_db, bp, g_hb, bn * 8);    /* overlaps a's last DMA */
   int rc = sqr ? sba_gpu_sqr_dev(g_dr, g_da, (int64_t)an, NULL)
                : sba_gpu_mul_dev(g_dr, g_da, (int64_t)an, g_db, (int64_t)bn, NULL);
   if (rc < 0) die("GPU multiply", rc);
-  if (rc == SBA_OK) {
-    if (hipMemcpyAsync(g_hr, g_dr, (an + bn) * 8, hipMemcpyDeviceToHost, NULL) != hipSuccess ||
-        hipStreamSynchronize(NULL) != hipSuccess)
-      die("hipMemcpyAsync D2H", SBA_ERR_HIP);
-    par_copy(rp, g_hr, (an + bn) * 8);
-  }
+  if (rc == SBA_OK) stage_down(rp, g_dr, g_hr, (an + bn) * 8);
+  else if (hipStreamSynchronize(NULL) != hipSuccess)   /* staging idle before reuse */
+    die("hipStreamSynchronize", SBA_ERR_HIP);
   pthread_mutex_unlock(&g_stage_mu);
   return rc;
 }
