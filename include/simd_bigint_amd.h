@@ -121,12 +121,13 @@ int sba_gpu_chunking(int64_t an, int64_t bn, int sqr, int *Lc, uint64_t *chunk,
 /* L when a single product of (an, bn) runs at the radix-3 length 3 * 2^L
  * (2^(L+1) < an+bn-1 <= 3 * 2^L, 13 <= L <= 22, not chunked), else 0 */
 int sba_gpu_radix3(int64_t an, int64_t bn, int sqr);
-/* The carry stage between 2048-limb blocks on its own (no counterpart in
- * the reference; it lets a test drive the stage with chosen carry chains):
- * rp[0..Z) holds limbs each finished with block carry-in 0, bmaps[b] the
- * 2-state carry map of block b (bit c = carry out for carry in c); adds
- * the carries between blocks in place. Enqueue only; SBA_OK, SBA_BAND (a
- * null pointer, Z = 0 or Z > 2^29) or < 0. */
+/* The carry stage between blocks of sba_gpu_carry_block_limbs() limbs on
+ * its own (no counterpart in the reference; it lets a test drive the stage
+ * with chosen carry chains): rp[0..Z) holds limbs each finished with block
+ * carry-in 0, bmaps[b] the 2-state carry map of block b (bit c = carry out
+ * for carry in c); adds the carries between blocks in place. Enqueue only;
+ * SBA_OK, SBA_BAND (a null pointer, Z = 0 or Z > 2^29) or < 0. */
+uint32_t sba_gpu_carry_block_limbs(void);
 int sba_gpu_carry_blocks(uint64_t *rp, const uint32_t *bmaps, uint64_t Z, void *stream);
 uint64_t sba_gpu_min_operand_cap(void);
 int sba_gpu_max_lg(void);

@@ -157,6 +157,8 @@ def lib() -> ctypes.CDLL:
     L.sba_gpu_radix3.restype = ctypes.c_int
     L.sba_gpu_carry_blocks.argtypes = [vp, vp, u64, vp]
     L.sba_gpu_carry_blocks.restype = ctypes.c_int
+    L.sba_gpu_carry_block_limbs.argtypes = []
+    L.sba_gpu_carry_block_limbs.restype = ctypes.c_uint32
     L.sba_set_gpu_threshold.argtypes = [u64]
     L.sba_set_gpu_threshold.restype = None
     L.sba_get_gpu_threshold.argtypes = []
@@ -291,11 +293,17 @@ def mul_dev(r, a, b, stream=None) -> int:
     return lib().sba_gpu_mul_dev(_ptr(r), _ptr(a), a.numel(), _ptr(b), b.numel(), s)
 
 
+def carry_block_limbs() -> int:
+    """Limbs per block of the carry stage (the Garner stage's block)."""
+    return int(lib().sba_gpu_carry_block_limbs())
+
+
 def carry_blocks(r, bmaps, stream=None) -> int:
-    """The carry stage between 2048-limb blocks on device tensors: r holds
-    limbs finished with block carry-in 0, bmaps the blocks' 2-state carry
-    maps (ceil(len(r) / 2048) 4-byte words)."""
-    nb = (r.numel() + 2047) // 2048
+    """The carry stage between blocks of carry_block_limbs() limbs on device
+    tensors: r holds limbs finished with block carry-in 0, bmaps the blocks'
+    2-state carry maps (one 4-byte word per block)."""
+    lpb = carry_block_limbs()
+    nb = (r.numel() + lpb - 1) // lpb
     if not (r.is_cuda and bmaps.is_cuda):
         raise ValueError("carry_blocks needs device tensors")
     if bmaps.numel() < nb or bmaps.element_size() != 4 or r.element_size() != 8:

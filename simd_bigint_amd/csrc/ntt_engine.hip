@@ -94,9 +94,11 @@ __device__ __forceinline__ void garner_coef(const GarnerArgs &ga, gc_u32 *spec,
 //   k_carry_blocks  the block carry-ins (a scan of the block maps), added
 //                   where they are 1 (usually to one limb)
 // ---------------------------------------------------------------------------
-#define CARRY_LPT 8                     // limbs per thread
-#define CARRY_LPB (CARRY_LPT * NT)      // limbs per block
-#define CARRY_RUNS (CARRY_LPB / 64)     // 64-limb runs per block
+#ifndef SBA_GARNER_LPT
+#define SBA_GARNER_LPT 4                // limbs per thread of the Garner stage (measured: 8 and 2 slower)
+#endif
+#define CARRY_LPT SBA_GARNER_LPT
+#define CARRY_LPB (CARRY_LPT * NT)      // limbs per (carry) block
 #define SCAN_NT 1024                    // k_carry_blocks threads
 #define CARRY_WGS 256                   // k_carry_blocks workgroups per product (at most)
 #define C2_ID 2u                        // 2-state map c -> c: bit c = carry out for carry in c
@@ -107,19 +109,22 @@ __device__ __forceinline__ uint32_t c2_then(uint32_t f, uint32_t g) {
   return c2_apply(g, f & 1u) | (c2_apply(g, (f >> 1) & 1u) << 1);
 }
 
-// The limbs [B0, B0 + CARRY_LPB) of a product from the coefficient words in
+// The limbs [B0, B0 + LPT * NT) of a product from the coefficient words in
 // LDS (slot j + 3 holds the words of coefficient B0 + j, slots 0..2 those of
 // B0-3..B0-1; zero outside the product), finished with block carry-in 0;
 // *bmap (if not null) gets the block's 2-state carry map. Called by the whole
-// workgroup after the words are in place.
+// workgroup after the words are in place. W2: the high word's type (its
+// values stay below 2^32 for Garner's coefficients).
+template <int LPT, typename W2>
 __device__ __forceinline__ void limbs_from_words(const uint64_t *ws0, const uint64_t *ws1,
-                                                 const uint64_t *ws2, uint32_t B0, uint32_t Z,
+                                                 const W2 *ws2, uint32_t B0, uint32_t Z,
                                                  g_u64 *rp, g_u32 *bmap) {
-  __shared__ uint32_t rmap[CARRY_RUNS], rcin[CARRY_RUNS];
+  constexpr int RUNS = LPT * NT / 64;             // 64-limb runs
+  __shared__ uint32_t rmap[RUNS], rcin[RUNS];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  uint64_t t[CARRY_LPT], ra[CARRY_LPT], rb[CARRY_LPT];
+  uint64_t t[LPT], ra[LPT], rb[LPT];
 #pragma unroll
-  for (int k = 0; k < CARRY_LPT; ++k) {
+  for (int k = 0; k < LPT; ++k) {
     const uint32_t j = threadIdx.x + k * NT;
     // s_j from slots j+3, j+2, j+1; s_{j-1} from j+2, j+1, j (its high part)
     const uint64_t a = ws0[j + 3], b = ws1[j + 2], c = ws2[j + 1];
@@ -145,7 +150,7 @@ __device__ __forceinline__ void limbs_from_words(const uint64_t *ws0, const uint
   __syncthreads();
   // carry into each run (block carry-in 0), and the block's map
   if (wid == 0) {
-    const uint32_t f = lane < CARRY_RUNS ? rmap[lane] : C2_ID;
+    const uint32_t f = lane < RUNS ? rmap[lane] : C2_ID;
     uint32_t inc = f;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -154,12 +159,12 @@ __device__ __forceinline__ void limbs_from_words(const uint64_t *ws0, const uint
     }
     uint32_t exc = __shfl_up(inc, 1, 64);
     if (lane == 0) exc = C2_ID;
-    if (lane < CARRY_RUNS) rcin[lane] = c2_apply(exc, 0);
+    if (lane < RUNS) rcin[lane] = c2_apply(exc, 0);
     if (lane == 63 && bmap) *bmap = inc;
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < CARRY_LPT; ++k) {
+  for (int k = 0; k < LPT; ++k) {
     const uint32_t j = threadIdx.x + k * NT;
     const uint64_t cin = rcin[k * (NT / 64) + wid];
     const uint64_t C = (ra[k] + rb[k] + cin) ^ ra[k] ^ rb[k];
@@ -168,7 +173,9 @@ __device__ __forceinline__ void limbs_from_words(const uint64_t *ws0, const uint
 }
 
 __global__ __launch_bounds__(NT) void k_garner_limbs(GarnerArgs ga, JobRef jr, uint32_t N) {
-  __shared__ uint64_t ws0[CARRY_LPB + 3], ws1[CARRY_LPB + 3], ws2[CARRY_LPB + 3];
+  // the high word of a coefficient (< 2^155) is below 2^27
+  __shared__ uint64_t ws0[CARRY_LPB + 3], ws1[CARRY_LPB + 3];
+  __shared__ uint32_t ws2[CARRY_LPB + 3];
   const Job jb = get_job(jr);
   if (blockIdx.x >= jb.nb) return;                 // this job has fewer carry blocks
   gc_u32 *spec = (gc_u32 *)jb.bspec;
@@ -180,7 +187,7 @@ __global__ __launch_bounds__(NT) void k_garner_limbs(GarnerArgs ga, JobRef jr, u
     const uint32_t i = B0 + j;
     uint64_t c0 = 0, c1 = 0, c2 = 0;
     if (i < zlen) garner_coef(ga, spec, N, i, c0, c1, c2);
-    ws0[j + 3] = c0; ws1[j + 3] = c1; ws2[j + 3] = c2;
+    ws0[j + 3] = c0; ws1[j + 3] = c1; ws2[j + 3] = (uint32_t)c2;
   }
   if (threadIdx.x < 3) {
     const uint32_t j = threadIdx.x;            // coefficient B0 - 3 + j
@@ -189,10 +196,11 @@ __global__ __launch_bounds__(NT) void k_garner_limbs(GarnerArgs ga, JobRef jr, u
       const uint32_t i = B0 + j - 3;
       if (i < zlen) garner_coef(ga, spec, N, i, c0, c1, c2);
     }
-    ws0[j] = c0; ws1[j] = c1; ws2[j] = c2;
+    ws0[j] = c0; ws1[j] = c1; ws2[j] = (uint32_t)c2;
   }
   __syncthreads();
-  limbs_from_words(ws0, ws1, ws2, B0, Z, (g_u64 *)jb.r, (g_u32 *)jb.bmaps + blockIdx.x);
+  limbs_from_words<CARRY_LPT, uint32_t>(ws0, ws1, ws2, B0, Z, (g_u64 *)jb.r,
+                                        (g_u32 *)jb.bmaps + blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -201,8 +209,9 @@ __global__ __launch_bounds__(NT) void k_garner_limbs(GarnerArgs ga, JobRef jr, u
 // words (< 2^(128 + 11)), which is exactly the coefficient form the Garner
 // stage produces, so the same limb finishing applies. Grid x = job.
 // ---------------------------------------------------------------------------
-#define SCHOOL_ZMAX CARRY_LPB
-#define SCHOOL_COLS (SCHOOL_ZMAX / NT)        // columns per thread
+#define SCHOOL_LPT 8
+#define SCHOOL_ZMAX (SCHOOL_LPT * NT)         // one workgroup's limbs
+#define SCHOOL_COLS SCHOOL_LPT                // columns per thread
 #ifndef SBA_SCHOOL_PMAX_DEFAULT
 #define SBA_SCHOOL_PMAX_DEFAULT 32768         // schoolbook limit on an * bn at N = 1024
 #endif
@@ -248,7 +257,7 @@ __global__ __launch_bounds__(NT) void k_school(JobRef jr) {
   }
   if (threadIdx.x < 3) ws0[threadIdx.x] = ws1[threadIdx.x] = ws2[threadIdx.x] = 0;
   __syncthreads();
-  limbs_from_words(ws0, ws1, ws2, 0, Z, (g_u64 *)jb.r, nullptr);
+  limbs_from_words<SCHOOL_LPT, uint64_t>(ws0, ws1, ws2, 0, Z, (g_u64 *)jb.r, nullptr);
 }
 
 // products with an + bn <= SCHOOL_ZMAX take the schoolbook kernel when
@@ -835,6 +844,8 @@ static int get_ctx(DevCtx **out) {
 // products longer than 2^25 coefficients.
 // ---------------------------------------------------------------------------
 #define SBA_MAX_Z ((uint64_t)1 << 29)   // product limbs: residue offsets stay 32-bit
+
+extern "C" uint32_t sba_gpu_carry_block_limbs(void) { return CARRY_LPB; }
 
 extern "C" int sba_gpu_carry_blocks(uint64_t *rp, const uint32_t *bmaps, uint64_t Z,
                                     void *stream) {

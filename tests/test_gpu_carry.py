@@ -1,4 +1,4 @@
-"""The carry stage between 2048-limb blocks, driven with chosen carry
+"""The carry stage between limb blocks, driven with chosen carry
 chains: blocks whose carry-in is 1 and whose leading limbs are all ones
 (short runs, runs across most of a block, whole blocks), many such blocks
 per workgroup of the stage. Random products almost never reach
@@ -13,11 +13,10 @@ import simd_bigint_amd as sba
 from simd_bigint_amd.workloads import xorshift_fill
 
 pytestmark = pytest.mark.gpu
-LPB = 2048
 ONES = np.uint64(0xFFFFFFFFFFFFFFFF)
 
 
-def expected(limbs, maps, Z):
+def expected(limbs, maps, Z, LPB):
     """Carry c into block b from the maps (bit c of maps[b] = carry out for
     carry in c); a block with carry-in 1 gains 1 as a multi-limb number."""
     out = limbs.copy()
@@ -38,7 +37,9 @@ def expected(limbs, maps, Z):
 @pytest.mark.parametrize("nb,tail", [(3, 5), (1500, 777), (3000, 1)])
 def test_carry_blocks_chosen_chains(nb, tail):
     torch = pytest.importorskip("torch")
-    print("carry_blocks", nb, tail, file=sys.stderr, flush=True)
+    LPB = sba.carry_block_limbs()
+    tail = tail % LPB or 1
+    print("carry_blocks", nb, tail, LPB, file=sys.stderr, flush=True)
     Z = nb * LPB - tail
     st = [nb * 31 + tail]
     limbs = xorshift_fill(Z, st)
@@ -56,7 +57,7 @@ def test_carry_blocks_chosen_chains(nb, tail):
             limbs[B0:B0 + E] = ONES                                  # the whole block
         elif kind == 4:
             limbs[B0] = ONES - np.uint64(1)                          # one below all ones
-    want = expected(limbs, maps, Z)
+    want = expected(limbs, maps, Z, LPB)
     r = torch.from_numpy(limbs.view(np.int64).copy()).cuda()
     m = torch.from_numpy(maps.view(np.int32).copy()).cuda()
     assert sba.carry_blocks(r, m) == 1
