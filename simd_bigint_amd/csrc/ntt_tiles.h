@@ -786,10 +786,19 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd3(PrimeArgs pa, JobR
   const bool split = os.split;
   const uint32_t q0 = os.q0;
   const uint32_t q1 = split ? q0 + 1 : SBA_K;
+  // as in k_first_fwd: the rows' level-0 chunks start from the registers
+  // and (EDGE) their last chunks go straight to HBM
+  constexpr int B0 = ChunkPlan<ELL, LOGTC, 5>::pick(0);
+  constexpr bool DIRECT = SBA_COL_REGS && B0 == ChunkPlan<ELL, LOGTC, 5>::LE;
+  using LC = LastChunk<ELL, LOGTC, 5>;
+  constexpr bool EDGE =
+      DIRECT && SBA_COL_EDGE && LC::T0 > 0 && ChunkIO<ELL, LOGTC, LC::T0, LC::B>::OK;
+  typedef typename ChunkRegs<ELL, LOGTC, 0, B0>::type Regs0;
 #pragma unroll 1
   for (uint32_t q = q0; q < q1; ++q) {
     const uint32_t p = pa.p[q], pinv = pa.pinv[q], r1 = pa.r1[q];
     __syncthreads();
+    Regs0 v0, v1, v2;
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
@@ -804,12 +813,45 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd3(PrimeArgs pa, JobR
       uint32_t y0, y1, y2;
       dft3(r3_pick(0, sm, mu, x[0], x[1], x[2]), r3_pick(1, sm, mu, x[0], x[1], x[2]),
            r3_pick(2, sm, mu, x[0], x[1], x[2]), y0, y1, y2, pa.h1[q], pa.h2[q], p, pinv);
-      lt[i * Pad<LOGTC>::words(NT)] = y0;
-      lt[W + i * Pad<LOGTC>::words(NT)] = y1;
-      lt[2 * W + i * Pad<LOGTC>::words(NT)] = y2;
+      if constexpr (DIRECT) {
+        v0[0][i] = y0;
+        v1[0][i] = y1;
+        v2[0][i] = y2;
+      } else {
+        lt[i * Pad<LOGTC>::words(NT)] = y0;
+        lt[W + i * Pad<LOGTC>::words(NT)] = y1;
+        lt[2 * W + i * Pad<LOGTC>::words(NT)] = y2;
+      }
     }
+    if constexpr (DIRECT) {
+      auto row = [&](int k, Regs0 &vk) {
+        uint32_t *lk = lds + k * W;
+        tile_chunk_io<ELL, LOGTC, 0, B0, false, 0, CHUNK_FROM_REGS>(lk, pa.wn[q], 0, p, pinv, vk);
+        if constexpr (EDGE) {
+          using IO = ChunkIO<ELL, LOGTC, LC::T0, LC::B>;
+          tile_fwd_range<ELL, LOGTC, 5, B0, LC::T0>(lk, pa.wn[q], 0, p, pinv);
+          typename ChunkRegs<ELL, LOGTC, LC::T0, LC::B>::type vl;
+          tile_chunk_io<ELL, LOGTC, LC::T0, LC::B, false, 0, CHUNK_TO_REGS>(lk, pa.wn[q], 0, p,
+                                                                         pinv, vl);
+          g_u32 *out = spec + q * 3 * M + (uint32_t)k * M + col0;
+          const uint32_t t = IO::thr(S);
+          constexpr int GL = ChunkRegs<ELL, LOGTC, LC::T0, LC::B>::GPT;
+#pragma unroll
+          for (int gi = 0; gi < GL; ++gi)
+#pragma unroll
+            for (int m = 0; m < (1 << LC::B); ++m) (out + IO::uni(gi, m, S))[t] = vl[gi][m];
+        } else {
+          tile_fwd<ELL, LOGTC, 5, B0>(lk, pa.wn[q], 0, p, pinv);
+        }
+      };
+      row(0, v0);
+      row(1, v1);
+      row(2, v2);
+      if constexpr (EDGE) continue;
+    } else {
 #pragma unroll 1
-    for (int k = 0; k < 3; ++k) tile_fwd<ELL, LOGTC, 5, 0>(lds + k * W, pa.wn[q], 0, p, pinv);
+      for (int k = 0; k < 3; ++k) tile_fwd<ELL, LOGTC, 5, 0>(lds + k * W, pa.wn[q], 0, p, pinv);
+    }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -844,26 +886,58 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_inv_first3(PrimeArgs pa, JobR
   g_u32 *buf = (g_u32 *)jb.bspec + q * 3 * M;
   uint32_t *lt = lds + lpadt<LOGTC>(threadIdx.x);
   const uint32_t toff = (threadIdx.x >> LOGTC) * S + (threadIdx.x & (TC - 1));
+  // EDGE: each row's first inverse chunk loads straight from HBM and its
+  // level-0 chunk ends in the registers the column DFT-3 takes
+  constexpr int B0 = ChunkPlan<ELL, LOGTC, 5>::pick(0);
+  constexpr bool DIRECT = SBA_COL_REGS && B0 == ChunkPlan<ELL, LOGTC, 5>::LE;
+  using LC = LastChunk<ELL, LOGTC, 5>;
+  constexpr bool EDGE =
+      DIRECT && SBA_COL_EDGE && LC::T0 > 0 && ChunkIO<ELL, LOGTC, LC::T0, LC::B>::OK;
+  typedef typename ChunkRegs<ELL, LOGTC, 0, B0>::type Regs0;
+  Regs0 v0, v1, v2;
+  if constexpr (EDGE) {
+    auto row = [&](int k, Regs0 &vk) {
+      using IO = ChunkIO<ELL, LOGTC, LC::T0, LC::B>;
+      uint32_t *lk = lds + k * W;
+      typename ChunkRegs<ELL, LOGTC, LC::T0, LC::B>::type vf;
+      constexpr int GL = ChunkRegs<ELL, LOGTC, LC::T0, LC::B>::GPT;
+      g_u32 *in = buf + (uint32_t)k * M + col0;
+      const uint32_t t = IO::thr(S);
 #pragma unroll
-  for (int k = 0; k < 3; ++k)
+      for (int gi = 0; gi < GL; ++gi)
 #pragma unroll
-    for (int i = 0; i < E; ++i) {
-      const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
-      lt[k * W + i * Pad<LOGTC>::words(NT)] = (buf + (uint32_t)k * M + uoff)[toff];
-    }
+        for (int m = 0; m < (1 << LC::B); ++m) vf[gi][m] = (in + IO::uni(gi, m, S))[t];
+      tile_chunk_io<ELL, LOGTC, LC::T0, LC::B, true, 0, CHUNK_FROM_REGS>(lk, pa.wni[q], 0, p, pinv,
+                                                                       vf);
+      tile_inv_upto<ELL, LOGTC, 5, B0, LC::T0>(lk, pa.wni[q], 0, p, pinv);
+      tile_chunk_io<ELL, LOGTC, 0, B0, true, 0, CHUNK_TO_REGS>(lk, pa.wni[q], 0, p, pinv, vk);
+    };
+    row(0, v0);
+    row(1, v1);
+    row(2, v2);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+        lt[k * W + i * Pad<LOGTC>::words(NT)] = (buf + (uint32_t)k * M + uoff)[toff];
+      }
 #pragma unroll 1
-  for (int k = 0; k < 3; ++k) tile_inv<ELL, LOGTC, 5, 0>(lds + k * W, pa.wni[q], 0, p, pinv);
-  __syncthreads();
+    for (int k = 0; k < 3; ++k) tile_inv<ELL, LOGTC, 5, 0>(lds + k * W, pa.wni[q], 0, p, pinv);
+    __syncthreads();
+  }
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
     const uint32_t s = uoff + toff;
     uint32_t r0, r1, r2;
     const uint32_t sc = pa.scale[q];
-    dft3(red1(mont_mul(lt[i * Pad<LOGTC>::words(NT)], sc, p, pinv), p),
-         red1(mont_mul(lt[W + i * Pad<LOGTC>::words(NT)], sc, p, pinv), p),
-         red1(mont_mul(lt[2 * W + i * Pad<LOGTC>::words(NT)], sc, p, pinv), p), r0, r1, r2,
-         pa.h1[q], pa.h2i[q], p, pinv);
+    const uint32_t a0 = EDGE ? v0[0][i] : lt[i * Pad<LOGTC>::words(NT)];
+    const uint32_t a1 = EDGE ? v1[0][i] : lt[W + i * Pad<LOGTC>::words(NT)];
+    const uint32_t a2 = EDGE ? v2[0][i] : lt[2 * W + i * Pad<LOGTC>::words(NT)];
+    dft3(red1(mont_mul(a0, sc, p, pinv), p), red1(mont_mul(a1, sc, p, pinv), p),
+         red1(mont_mul(a2, sc, p, pinv), p), r0, r1, r2, pa.h1[q], pa.h2i[q], p, pinv);
     // row r holds coefficient s + t(r) M: write limb position t from row
     // r(t) = (s + t mu) mod 3
     const uint32_t sm = s % 3;
