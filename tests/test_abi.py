@@ -67,3 +67,31 @@ def test_out_of_band_returns_zero_without_device():
     # p30 band (include/p30.h:1851-1858): min operand cap 2,344,425
     assert L.p30_mul(p(r), p(a), 2344426, p(a), 2344426) == 0
     del e
+
+
+def test_tuning_knobs_round_trip_without_device():
+    """The CPU/GPU threshold, the schoolbook limit and the carry block size
+    answer without a GPU; setters round-trip and are restored."""
+    import ctypes
+    L = sba.lib()
+    n = L.sba_tune_sizes(None, 0)
+    sizes = (ctypes.c_uint64 * n)()
+    assert L.sba_tune_sizes(sizes, n) == n
+    ladder = list(sizes)
+    assert ladder == sorted(ladder) and ladder[0] >= 1 and len(set(ladder)) == n
+    old = sba.get_gpu_threshold()
+    assert old == 1024
+    try:
+        sba.set_gpu_threshold(12345)
+        assert sba.get_gpu_threshold() == 12345
+    finally:
+        sba.set_gpu_threshold(old)
+    olds = sba.get_school_max()
+    assert olds == 32768
+    try:
+        sba.set_school_max(0)
+        assert sba.get_school_max() == 0
+    finally:
+        sba.set_school_max(olds)
+    lpb = sba.carry_block_limbs()
+    assert lpb in (512, 1024, 2048) and lpb % 64 == 0
