@@ -573,6 +573,13 @@ def main():
                 "frac": round(ach / VALU_PEAK_T, 4),
                 "model": "%d VALU lane-ops per radix-2 butterfly, %d per pointwise product, "
                          "%d levels x 3 transforms" % (VALU_PER_BFLY, VALU_PER_POINT, last)}
+        # the same ceiling measured: butterflies on registers alone
+        try:
+            probe = sba.probe_butterflies() * VALU_PER_BFLY / 1e12
+            valu["probe_peak"] = round(probe, 2)
+            valu["frac_of_probe"] = round(ach / probe, 4)
+        except RuntimeError:
+            pass
 
     Z = 2 * n
     K_model = 6

@@ -285,6 +285,10 @@ int sba_gpu_profile_collect(void);
 void sba_gpu_profile_reset(void);
 int sba_gpu_profile_get(int idx, const char **name, double *ms, double *bytes,
                         uint64_t *count);
+/* radix-2 butterflies per second that the current device issues from
+ * registers alone (no memory traffic): the VALU ceiling the transform
+ * kernels are measured against. SBA_OK, or an error code. */
+int sba_gpu_probe_butterflies(double *per_s);
 /* the CPU tier alone (basecase + Karatsuba), either operand order */
 /* 1 when the reference's nt_mul would accept (an, bn) (its p50 band,
  * include/p50.h:1020-1046); nt_mul / nt_sqr / p50_* return 0 exactly

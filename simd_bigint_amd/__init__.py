@@ -159,6 +159,8 @@ def lib() -> ctypes.CDLL:
     L.sba_gpu_carry_blocks.restype = ctypes.c_int
     L.sba_gpu_carry_block_limbs.argtypes = []
     L.sba_gpu_carry_block_limbs.restype = ctypes.c_uint32
+    L.sba_gpu_probe_butterflies.argtypes = [ctypes.POINTER(ctypes.c_double)]
+    L.sba_gpu_probe_butterflies.restype = ctypes.c_int
     L.sba_set_gpu_threshold.argtypes = [u64]
     L.sba_set_gpu_threshold.restype = None
     L.sba_get_gpu_threshold.argtypes = []
@@ -291,6 +293,16 @@ def mul_dev(r, a, b, stream=None) -> int:
     _check_dev(r, a.numel() + b.numel())
     s = None if stream is None else ctypes.c_void_p(int(stream.cuda_stream))
     return lib().sba_gpu_mul_dev(_ptr(r), _ptr(a), a.numel(), _ptr(b), b.numel(), s)
+
+
+def probe_butterflies() -> float:
+    """Radix-2 butterflies per second the current GPU issues from registers
+    alone (the VALU ceiling of the transform kernels)."""
+    r = ctypes.c_double(0.0)
+    rc = lib().sba_gpu_probe_butterflies(ctypes.byref(r))
+    if rc != 1:
+        raise RuntimeError("simd_bigint_amd: butterfly probe failed (%d)" % rc)
+    return r.value
 
 
 def carry_block_limbs() -> int:

@@ -724,6 +724,63 @@ extern "C" int sba_gpu_profile_get(int idx, const char **name, double *ms, doubl
   return 1;
 }
 
+// ---------------------------------------------------------------------------
+// VALU ceiling of the butterfly: forward butterflies on registers alone
+// (eight independent pairs per thread, no memory access in the loop), so the
+// rate is what the butterfly's own instruction mix issues at -- the ceiling
+// the transform kernels' VALU roofline is read against.
+// ---------------------------------------------------------------------------
+#define PROBE_ITERS 2048
+#define PROBE_BLOCKS 2048
+#define PROBE_RUNS 5
+__global__ __launch_bounds__(256) void k_probe_bfly(uint32_t *out, uint32_t w, uint32_t p,
+                                                    uint32_t pinv) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  uint32_t x[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) x[j] = (t * 7u + (uint32_t)j * 977u) % p;
+  for (int i = 0; i < PROBE_ITERS; ++i) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) bfly_fwd(x[2 * j], x[2 * j + 1], w, p, pinv);
+  }
+  uint32_t s = 0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) s += x[j];
+  out[t] = s;
+}
+
+extern "C" int sba_gpu_probe_butterflies(double *per_s) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return SBA_ERR_NODEV;
+  const uint32_t p = kPrimes[SBA_K - 1];
+  uint32_t inv = 1;
+  for (int i = 0; i < 5; ++i) inv *= 2u - p * inv;
+  uint32_t *out = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float ms = 0.f;
+  int rc = SBA_ERR_HIP;
+  if (hipMalloc((void **)&out, (size_t)PROBE_BLOCKS * 256 * sizeof(uint32_t)) == hipSuccess &&
+      hipStreamCreate(&st) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+      hipEventCreate(&e1) == hipSuccess) {
+    for (int r = 0; r < 2 + PROBE_RUNS; ++r) {
+      if (r == 2) (void)hipEventRecord(e0, st);
+      k_probe_bfly<<<PROBE_BLOCKS, 256, 0, st>>>(out, 12345u, p, 0u - inv);
+    }
+    if (hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+        hipGetLastError() == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
+        ms > 0.f) {
+      *per_s = (double)PROBE_BLOCKS * 256 * PROBE_ITERS * 8 * PROBE_RUNS / (ms * 1e-3);
+      rc = SBA_OK;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (out) (void)hipFree(out);
+  return rc;
+}
+
 static int enqueue_garner(DevCtx &c, uint32_t Nt, uint32_t stride, const JobRef &jr,
                           uint32_t njobs, uint32_t nbmax, double zsum, hipStream_t st);
 static const AccArgs kNoAcc = {nullptr, 0, 0, 0, 0};

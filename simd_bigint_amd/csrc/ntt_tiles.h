@@ -62,6 +62,12 @@ struct GarnerArgs {
 #ifndef SBA_ZERO_LEVEL
 #define SBA_ZERO_LEVEL 1     // first level of a half-empty operand as a copy
 #endif
+#ifndef SBA_TW_EARLY
+#define SBA_TW_EARLY 1       // a chunk's twiddle loads issued before its barrier
+#endif
+#ifndef SBA_FUSED2_EARLY
+#define SBA_FUSED2_EARLY 1   // the same in k_fused2 (which holds a's spectrum in registers)
+#endif
 #ifndef SBA_FIRST_WAVES
 #define SBA_FIRST_WAVES SBA_WAVES
 #endif
@@ -120,6 +126,14 @@ __device__ __forceinline__ uint32_t limb_residue(uint64_t x, uint32_t p, uint32_
   return (uint32_t)(((uint64_t)m * p + t) >> 32);
 }
 
+// table entry i (i < 2^30) through a 32-bit byte offset from the table's
+// base: the load can take the uniform base in SGPRs and one VGPR offset
+__device__ __forceinline__ uint32_t tw_at(const uint32_t *tw, uint32_t i) {
+  typedef __attribute__((address_space(1))) const uint8_t gc_byte;
+  typedef __attribute__((address_space(1))) const uint32_t gc_word;
+  return *(gc_word *)((gc_byte *)tw + (i << 2));
+}
+
 // ---------------------------------------------------------------------------
 // one chunk of B levels (local levels T0 .. T0+B-1 of an ELL-level pass)
 //
@@ -149,12 +163,17 @@ template <int ELL, int LOGTC, int T0, int B> struct ChunkGeom {
 #define CHUNK_TO_REGS 1
 #define CHUNK_FROM_REGS 2
 #define CHUNK_REGS 3          // vv in, results left in vv (a pass of one chunk)
+// when a chunk loads its twiddles (SBA_TW_EARLY)
+#define TW_LATE 0             // at their first use
+#define TW_EARLY 1            // before the chunk's barrier
+#define TW_EARLY32 2          // the same, addressed by 32-bit offsets
 template <int ELL, int LOGTC, int T0, int B> struct ChunkRegs {
   static constexpr int GPT = (1 << (ELL + LOGTC)) / ((1 << B) * NT);
   typedef uint32_t type[GPT][1 << B];
 };
 
-template <int ELL, int LOGTC, int T0, int B, bool INV, int ZL = 0, int IO = CHUNK_LDS>
+template <int ELL, int LOGTC, int T0, int B, bool INV, int ZL = 0, int IO = CHUNK_LDS,
+          int TE = TW_EARLY>
 __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__restrict__ tw,
                                               uint32_t H, uint32_t p, uint32_t pinv,
                                               typename ChunkRegs<ELL, LOGTC, T0, B>::type &vv) {
@@ -165,6 +184,32 @@ __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__r
   static_assert(GPT >= 1, "chunk too wide for the tile");
   constexpr bool IN_REGS = IO == CHUNK_FROM_REGS || IO == CHUNK_REGS;
   constexpr bool OUT_REGS = IO == CHUNK_TO_REGS || IO == CHUNK_REGS;
+  // twiddles of group gi, level u, sub-node m2 at tv[gi][(1 << u) - 1 + m2].
+  // They depend on the tile position only, so with SBA_TW_EARLY they are
+  // loaded before the barrier: the previous chunk's values are in LDS by
+  // then (their registers are free) and the loads' latency overlaps the
+  // wait for the other waves. TE = TW_LATE keeps them at their first use;
+  // TW_EARLY32 addresses them by 32-bit offsets (fewer live address
+  // registers where a kernel holds other values across its chunks).
+  constexpr bool EARLY = SBA_TW_EARLY && TE != TW_LATE;
+  constexpr int TWN = EARLY ? (1 << B) - 1 : 1;
+  uint32_t tv[GPT][TWN];
+  if constexpr (EARLY) {
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int gi = 0; gi < GPT; ++gi) {
+      const uint32_t hj = (H << T0) | ((threadIdx.x + gi * NT) >> (LOGTC + LB));
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        if (!INV && T0 + u < ZL) continue;
+#pragma unroll
+        for (int m2 = 0; m2 < (1 << u); ++m2) {
+          const uint32_t i = (hj << u) | (uint32_t)m2;
+          tv[gi][(1 << u) - 1 + m2] = TE == TW_EARLY32 ? tw_at(tw, i) : tw[i];
+        }
+      }
+    }
+  }
   if (!IN_REGS) __syncthreads();
 #pragma unroll
   for (int gi = 0; gi < GPT; ++gi) {
@@ -195,7 +240,7 @@ __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__r
         }
 #pragma unroll
         for (int m2 = 0; m2 < (1 << u); ++m2) {
-          const uint32_t w = tw[(hj << u) | (uint32_t)m2];
+          const uint32_t w = EARLY ? tv[gi][(1 << u) - 1 + m2] : tw[(hj << u) | (uint32_t)m2];
 #pragma unroll
           for (int r = 0; r < half; ++r) {
             const int m = (m2 << (B - u)) | r;
@@ -209,7 +254,7 @@ __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__r
         const int half = 1 << (B - u - 1);
 #pragma unroll
         for (int m2 = 0; m2 < (1 << u); ++m2) {
-          const uint32_t w = tw[(hj << u) | (uint32_t)m2];
+          const uint32_t w = EARLY ? tv[gi][(1 << u) - 1 + m2] : tw[(hj << u) | (uint32_t)m2];
 #pragma unroll
           for (int r = 0; r < half; ++r) {
             const int m = (m2 << (B - u)) | r;
@@ -227,11 +272,11 @@ __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__r
   }
 }
 
-template <int ELL, int LOGTC, int T0, int B, bool INV, int ZL = 0>
+template <int ELL, int LOGTC, int T0, int B, bool INV, int ZL = 0, int TE = TW_EARLY>
 __device__ __forceinline__ void tile_chunk(uint32_t *lds, const uint32_t *__restrict__ tw,
                                            uint32_t H, uint32_t p, uint32_t pinv) {
   typename ChunkRegs<ELL, LOGTC, T0, B>::type unused;
-  tile_chunk_io<ELL, LOGTC, T0, B, INV, ZL, CHUNK_LDS>(lds, tw, H, p, pinv, unused);
+  tile_chunk_io<ELL, LOGTC, T0, B, INV, ZL, CHUNK_LDS, TE>(lds, tw, H, p, pinv, unused);
 }
 
 // chunk sizes: the largest B <= BM whose group addressing is linear,
@@ -252,13 +297,13 @@ template <int ELL, int LOGTC, int BMAX> struct ChunkPlan {
   }
 };
 
-template <int ELL, int LOGTC, int BMAX, int T0, int ZL = 0>
+template <int ELL, int LOGTC, int BMAX, int T0, int ZL = 0, int TE = TW_EARLY>
 __device__ __forceinline__ void tile_fwd(uint32_t *lds, const uint32_t *tw, uint32_t H,
                                          uint32_t p, uint32_t pinv) {
   if constexpr (T0 < ELL) {
     constexpr int B = ChunkPlan<ELL, LOGTC, BMAX>::pick(T0);
-    tile_chunk<ELL, LOGTC, T0, B, false, ZL>(lds, tw, H, p, pinv);
-    tile_fwd<ELL, LOGTC, BMAX, T0 + B, ZL>(lds, tw, H, p, pinv);
+    tile_chunk<ELL, LOGTC, T0, B, false, ZL, TE>(lds, tw, H, p, pinv);
+    tile_fwd<ELL, LOGTC, BMAX, T0 + B, ZL, TE>(lds, tw, H, p, pinv);
   }
 }
 
@@ -333,13 +378,13 @@ __device__ __forceinline__ void tile_fwd_first(uint32_t *lds, const uint32_t *tw
 
 
 
-template <int ELL, int LOGTC, int BMAX, int T0>
+template <int ELL, int LOGTC, int BMAX, int T0, int TE = TW_EARLY>
 __device__ __forceinline__ void tile_inv(uint32_t *lds, const uint32_t *tw, uint32_t H,
                                          uint32_t p, uint32_t pinv) {
   if constexpr (T0 < ELL) {
     constexpr int B = ChunkPlan<ELL, LOGTC, BMAX>::pick(T0);
-    tile_inv<ELL, LOGTC, BMAX, T0 + B>(lds, tw, H, p, pinv);
-    tile_chunk<ELL, LOGTC, T0, B, true>(lds, tw, H, p, pinv);
+    tile_inv<ELL, LOGTC, BMAX, T0 + B, TE>(lds, tw, H, p, pinv);
+    tile_chunk<ELL, LOGTC, T0, B, true, 0, TE>(lds, tw, H, p, pinv);
   }
 }
 
@@ -692,6 +737,9 @@ __global__ __launch_bounds__(NT, BMAX == 5 ? SBA_FUSED2_WAVES : SBA_WAVES) void 
   constexpr int NE = 1 << ELL;
   constexpr int E = NE / NT;
   __shared__ uint32_t lds[Pad<0>::words(NE)];
+  // a's spectrum stays in registers from its transform to the product, so
+  // early twiddles need the 32-bit-offset form to fit in 3 waves per SIMD
+  constexpr int F2TE = SBA_FUSED2_EARLY ? TW_EARLY32 : TW_LATE;
   const uint32_t z = blockIdx.z;
   const Job jb = get_job(jr, r3 ? z / 3 : z);
   const int q = blockIdx.y;
@@ -707,19 +755,19 @@ __global__ __launch_bounds__(NT, BMAX == 5 ? SBA_FUSED2_WAVES : SBA_WAVES) void 
   uint32_t av[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) lt[i * Pad<0>::words(NT)] = (as + i * NT)[tid];
-  tile_fwd<ELL, 0, BMAX, 0>(lds, pa.wn[q], H, p, pinv);
+  tile_fwd<ELL, 0, BMAX, 0, 0, F2TE>(lds, pa.wn[q], H, p, pinv);
   __syncthreads();
   // each thread takes back only its own elements, then refills them with b
 #pragma unroll
   for (int i = 0; i < E; ++i) av[i] = red1(lt[i * Pad<0>::words(NT)], p);
 #pragma unroll
   for (int i = 0; i < E; ++i) lt[i * Pad<0>::words(NT)] = (buf + i * NT)[tid];
-  tile_fwd<ELL, 0, BMAX, 0>(lds, pa.wn[q], H, p, pinv);
+  tile_fwd<ELL, 0, BMAX, 0, 0, F2TE>(lds, pa.wn[q], H, p, pinv);
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < E; ++i)
     lt[i * Pad<0>::words(NT)] = mont_mul(lt[i * Pad<0>::words(NT)], av[i], p, pinv);
-  tile_inv<ELL, 0, BMAX, 0>(lds, pa.wni[q], H, p, pinv);
+  tile_inv<ELL, 0, BMAX, 0, F2TE>(lds, pa.wni[q], H, p, pinv);
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < E; ++i) (buf + i * NT)[tid] = lt[i * Pad<0>::words(NT)];

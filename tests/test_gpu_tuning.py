@@ -287,3 +287,13 @@ def test_gpu_threshold_tuner():
     assert all(g < c for g, c in zip(gpu[i:], cpu[i:]))
     if i > 0:
         assert gpu[i - 1] >= cpu[i - 1]
+
+
+def test_butterfly_probe_near_issue_ceiling():
+    """The register-only butterfly loop runs close to the VALU issue ceiling
+    the roofline assumes: 10 operations per butterfly at 256 CUs x 4 SIMDs x
+    16 lanes x 2.4 GHz, i.e. 3.93e12 butterflies per second."""
+    rate = sba.probe_butterflies()
+    ceiling = 256 * 4 * 16 * 2.4e9 / 10
+    say("butterflies/s", rate, "of", ceiling)
+    assert 0.5 * ceiling < rate < 1.2 * ceiling
