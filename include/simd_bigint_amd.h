@@ -96,6 +96,9 @@ void divrem_u64_dc(uint64_t *qp, uint64_t *rp, const uint64_t *np,
 /* Device-resident entries (no counterpart in the reference, which has no  */
 /* device): device pointers, a HIP stream (hipStream_t passed as void*,    */
 /* NULL = default stream). Enqueue only. Return SBA_OK / SBA_BAND / < 0.   */
+/* Without a caller workspace, calls share the device's grow-only          */
+/* workspace: two of them on different streams must not overlap in time   */
+/* (the _ws variants, each with its own workspace, may).                   */
 /* ---------------------------------------------------------------------- */
 int sba_gpu_mul_dev(uint64_t *rp, const uint64_t *ap, int64_t an,
                     const uint64_t *bp, int64_t bn, void *stream);

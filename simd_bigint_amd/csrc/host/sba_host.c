@@ -24,7 +24,8 @@ typedef unsigned __int128 u128;
 static pthread_mutex_t g_stage_mu = PTHREAD_MUTEX_INITIALIZER;
 static uint64_t *g_da, *g_db, *g_dr;
 static size_t g_na, g_nb, g_nr;
-static int g_dev_checked = 0, g_dev_ok = 0;
+static pthread_once_t g_dev_once = PTHREAD_ONCE_INIT;
+static int g_dev_ok = 0;
 
 static void die(const char *what, int rc) {
   fprintf(stderr, "simd_bigint_amd: %s failed (code %d); the library has no CPU "
@@ -32,12 +33,14 @@ static void die(const char *what, int rc) {
   abort();
 }
 
+static void device_probe(void) {
+  int n = 0;
+  g_dev_ok = hipGetDeviceCount(&n) == hipSuccess && n > 0;
+}
+
+/* once per process, whichever thread gets here first */
 static int device_ok(void) {
-  if (!g_dev_checked) {
-    int n = 0;
-    g_dev_ok = hipGetDeviceCount(&n) == hipSuccess && n > 0;
-    g_dev_checked = 1;
-  }
+  pthread_once(&g_dev_once, device_probe);
   return g_dev_ok;
 }
 

@@ -153,6 +153,44 @@ def test_public_entry_default_dispatch():
         assert buf[n] == 0xABABABABABABABAB and buf[n + 1] == 0xABABABABABABABAB
 
 
+def test_host_entries_from_several_threads():
+    """The host entries are reentrant across threads (the reference's are,
+    include/scratch.h:339-356): four threads call nt_mul and simd_mpn_mul
+    at once -- GPU-sized and CPU-tier products mixed -- and every product
+    matches the oracle."""
+    import threading
+    L = sba.lib()
+    shapes = ((20000, 20000), (300, 500), (1 << 16, 3000), (4096, 4096))
+    out, errs = {}, []
+
+    def work(k):
+        try:
+            st = [1000 + k]
+            for rep in range(3):
+                for si, (an, bn) in enumerate(shapes):
+                    a = xorshift_fill(an, st)
+                    b = xorshift_fill(bn, st)
+                    r = np.zeros(an + bn, dtype=np.uint64)
+                    if (si + rep + k) % 2:
+                        rc = L.nt_mul(sba._arr(r), sba._arr(a), an, sba._arr(b), bn)
+                        assert rc == 1
+                    else:
+                        L.simd_mpn_mul(sba._arr(r), sba._arr(a), an, sba._arr(b), bn)
+                    out[(k, rep, si)] = (a, b, r)
+        except Exception as e:          # reported by the main thread
+            errs.append(repr(e))
+
+    ts = [threading.Thread(target=work, args=(k,)) for k in range(4)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errs, errs
+    assert len(out) == 4 * 3 * len(shapes)
+    for key, (a, b, r) in out.items():
+        assert (r == orc.nt_mul(a, b)).all(), key
+
+
 def test_divrem_reference():
     for e in GOLD["divrem"]:
         say("divrem", e["nn"], e["dn"], e["fn"])
