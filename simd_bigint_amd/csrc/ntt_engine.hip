@@ -44,18 +44,26 @@ __device__ __forceinline__ void garner_coef(const GarnerArgs &ga, gc_u32 *spec,
 #pragma unroll
   for (int q = 0; q < SBA_K; ++q)
     u[q] = red1(spec[q * N + i], ga.p[q]);           // already scaled by the inverse
+  // digit q: x_q = (u_q - x_0 - sum_{0<k<q} x_k c_qk) / (p_0 ... p_{q-1})
+  // mod p_q. The products x_k c_qk (each < 2^62, the c_qk in Montgomery
+  // form) are summed two at a time in 64 bits and reduced once per pair:
+  // (s + m p) / 2^32 < (2^63 + 2^63) / 2^32, and < 2p since s < 2^32 p.
+  // u_q + p - t lies in (0, 2p), a valid left operand of the last product.
   x[0] = u[0];
 #pragma unroll
   for (int q = 1; q < SBA_K; ++q) {
     const uint32_t p = ga.p[q], pinv = ga.pinv[q];
-    uint32_t t = x[0];
+    uint32_t t = 0;                                 // sum of the x_k c_qk, in [0, p)
 #pragma unroll
-    for (int k = 1; k < q; ++k) {
-      t += red1(mont_mul(x[k], ga.cm[q][k], p, pinv), p);
-      t = red1(t, p);
+    for (int k = 1; k < q; k += 2) {
+      uint64_t s = (uint64_t)x[k] * ga.cm[q][k];
+      if (k + 1 < q) s += (uint64_t)x[k + 1] * ga.cm[q][k + 1];
+      const uint32_t m = (uint32_t)s * pinv;
+      const uint32_t r = red1((uint32_t)(((uint64_t)m * p + s) >> 32), p);
+      t = red1(t + r, p);
     }
-    const uint32_t d = red1(u[q] + p - t, p);
-    x[q] = red1(mont_mul(d, ga.im[q], p, pinv), p);
+    t = red1(t + x[0], p);                          // x_0 < p_0 < p_q
+    x[q] = red1(mont_mul(u[q] + p - t, ga.im[q], p, pinv), p);
   }
   // Horner: c = (((x4 p3 + x3) p2 + x2) p1 + x1) p0 + x0  (< 2^155) on
   // 32-bit words, one 32 x 32 + 64-bit multiply-add per word and step
