@@ -264,7 +264,7 @@ def serve_leg(nreq, rate):
 
 
 def configs_leg(L, reps=10):
-    """The other BASELINE shapes on the same engine (operands resident in
+    """The other benchmark shapes on the same engine (operands resident in
     HBM): 3*2^20 balanced, 2^22 squaring, 2^23 x 2^18, and the 16384^2
     reference-CPU shape; each product checked against its golden SHA-256."""
     import numpy as np
@@ -274,8 +274,8 @@ def configs_leg(L, reps=10):
     gold = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
     gl = {e["name"]: e for e in gold["large_mul"]}
     out = {}
-    for name in ("cfg2_3x2^20", "cfg2_3x2^20_ones", "cfg2_sqr_2^22", "cfg3_2^23x2^18",
-                 "cfg1_16384"):
+    for name in ("mul_3x2^20", "mul_3x2^20_ones", "sqr_2^22", "mul_2^23x2^18",
+                 "mul_16384"):
         e = gl[name]
         st = [e["seed"]]
         a = pattern_fill(e["an"], e["pat"], st)
@@ -330,7 +330,7 @@ def configs_leg(L, reps=10):
         dt = (time.perf_counter() - t0) * 1e3
         best = dt if best is None else min(best, dt)
     q, r = q[:qn_.value], r[:rn_.value]
-    out["cfg3_divrem_2^21/2^20"] = {
+    out["divrem_2^21/2^20"] = {
         "nn": e["nn"], "dn": e["dn"], "ms": round(best, 3),
         "bit_exact": bool(sha(q) == e["q_sha256"] and sha(r) == e["r_sha256"]),
         "note": "host buffers (caller-allocated); includes the operand and result copies over PCIe"}
@@ -349,9 +349,9 @@ def configs_leg(L, reps=10):
         qn_, rn_ = ctypes.c_uint64(), ctypes.c_uint64()
         t0 = time.perf_counter()
         fr(qq, rr, nn_, e["nn"], dd_, e["dn"], ctypes.byref(qn_), ctypes.byref(rn_))
-        out["cfg3_divrem_2^21/2^20"]["reference_cpu_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
-        out["cfg3_divrem_2^21/2^20"]["reference_cpu"] = "divrem_u64_dc compiled from source, 1 core"
-    log("bench: divrem", out["cfg3_divrem_2^21/2^20"])
+        out["divrem_2^21/2^20"]["reference_cpu_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        out["divrem_2^21/2^20"]["reference_cpu"] = "divrem_u64_dc compiled from source, 1 core"
+    log("bench: divrem", out["divrem_2^21/2^20"])
     # the metric multiply through the host entry nt_mul (H2D + D2H inside)
     st = [42]
     a = xorshift_fill(1 << 22, st)
@@ -453,7 +453,7 @@ def main():
     ap.add_argument("--batch-jobs", type=int, default=4096,
                     help="jobs in the batched vs per-call comparison (0 = skip)")
     ap.add_argument("--configs", type=int, default=1,
-                    help="also time the other BASELINE shapes (0 = skip)")
+                    help="also time the other benchmark shapes (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=1000,
                     help="requests in the serving run (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=5000.0,
@@ -462,7 +462,7 @@ def main():
                     help="seconds of CPU work for the cpu_baseline sample (0 = skip)")
     ap.add_argument("--shape", default="metric_4Mi",
                     help="profiling aid: time another golden multiply shape (e.g. "
-                         "cfg2_3x2^20) instead of the metric; prints only that shape's time")
+                         "mul_3x2^20) instead of the metric; prints only that shape's time")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the 1 GiB elementwise kernel used to scale HBM counters")
     args = ap.parse_args()

@@ -96,13 +96,13 @@ def main():
         ("mid_513x512", 513, 512, 47, 0),
         ("mid_2048_ones", 2048, 2048, 0, 1),
         ("mid_2048_sqr", 2048, 0, 48, 0),
-        ("cfg1_16384", 16384, 16384, 42, 0),
-        ("cfg1_16384_sqr", 16384, 0, 42, 0),
+        ("mul_16384", 16384, 16384, 42, 0),
+        ("sqr_16384", 16384, 0, 42, 0),
         ("metric_4Mi", 1 << 22, 1 << 22, 42, 0),
-        ("cfg2_3x2^20", 3 << 20, 3 << 20, 43, 0),
-        ("cfg2_3x2^20_ones", 3 << 20, 3 << 20, 0, 1),
-        ("cfg2_sqr_2^22", 1 << 22, 0, 43, 0),
-        ("cfg3_2^23x2^18", 1 << 23, 1 << 18, 44, 0),
+        ("mul_3x2^20", 3 << 20, 3 << 20, 43, 0),
+        ("mul_3x2^20_ones", 3 << 20, 3 << 20, 0, 1),
+        ("sqr_2^22", 1 << 22, 0, 43, 0),
+        ("mul_2^23x2^18", 1 << 23, 1 << 18, 44, 0),
         ("p30cap_plus1_ones", 2344426, 2344426, 0, 1),
         ("half_2^19_ones", 1 << 19, 1 << 19, 0, 1),
         ("seam_lo", 180224 // 2 - 1, 180224 - 180224 // 2 + 2, 45, 0),

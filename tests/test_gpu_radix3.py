@@ -2,8 +2,8 @@
 length 3 * 2^L (Good-Thomas: a length-3 DFT down the columns of a 3 x 2^L
 array, riding the first forward and last inverse passes); every shape at
 the edges of each such class is bit-exact against the oracle, for
-multiplies and squares, and the 3 * 2^20 BASELINE pair (golden SHA-256 in
-test_gpu_parity) takes this path."""
+multiplies and squares, and the balanced 3 * 2^20-limb benchmark pair
+(golden SHA-256 in test_gpu_parity) takes this path."""
 import sys
 
 import numpy as np
