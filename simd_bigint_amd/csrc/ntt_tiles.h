@@ -198,7 +198,12 @@ __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__r
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int gi = 0; gi < GPT; ++gi) {
-      const uint32_t hj = (H << T0) | ((threadIdx.x + gi * NT) >> (LOGTC + LB));
+      // a chunk whose groups all lie in one node (NT threads x GPT groups
+      // under 2^(LOGTC + LB)) shares its twiddles across the block: a
+      // uniform index lets them come through the scalar cache
+      const uint32_t hj = (NT * GPT <= (1 << (LOGTC + LB)))
+                              ? (H << T0)
+                              : (H << T0) | ((threadIdx.x + gi * NT) >> (LOGTC + LB));
 #pragma unroll
       for (int u = 0; u < B; ++u) {
         if (!INV && T0 + u < ZL) continue;
