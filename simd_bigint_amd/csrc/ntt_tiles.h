@@ -195,6 +195,8 @@ __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__r
   constexpr int TWN = EARLY ? (1 << B) - 1 : 1;
   uint32_t tv[GPT][TWN];
   if constexpr (EARLY) {
+    // not hoisted above the caller's preceding work (the pointwise product
+    // before the first inverse chunk), where they would stretch live ranges
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int gi = 0; gi < GPT; ++gi) {
