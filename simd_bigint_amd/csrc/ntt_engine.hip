@@ -105,6 +105,9 @@ __device__ __forceinline__ void garner_coef(const GarnerArgs &ga, gc_u32 *spec,
 #ifndef SBA_GARNER_LPT
 #define SBA_GARNER_LPT 4                // limbs per thread of the Garner stage (measured: 8 and 2 slower)
 #endif
+#ifndef SBA_GARNER_UNROLL
+#define SBA_GARNER_UNROLL 2             // coefficients in flight per thread
+#endif
 #define CARRY_LPT SBA_GARNER_LPT
 #define CARRY_LPB (CARRY_LPT * NT)      // limbs per (carry) block
 #define SCAN_NT 1024                    // k_carry_blocks threads
@@ -189,7 +192,7 @@ __global__ __launch_bounds__(NT) void k_garner_limbs(GarnerArgs ga, JobRef jr, u
   gc_u32 *spec = (gc_u32 *)jb.bspec;
   const uint32_t zlen = jb.zlen, Z = jb.Z;
   const uint32_t B0 = blockIdx.x * CARRY_LPB;
-#pragma unroll 2
+#pragma unroll SBA_GARNER_UNROLL
   for (int k = 0; k < CARRY_LPT; ++k) {
     const uint32_t j = threadIdx.x + k * NT;
     const uint32_t i = B0 + j;
