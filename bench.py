@@ -214,52 +214,109 @@ def batch_leg(njobs, L):
                      "(shipped plans); 32 sampled products == Python integer products"}
 
 
-def serve_leg(nreq, rate):
-    """Serving: Poisson arrivals (80% multiplies / 20% squares), operand sizes
-    log-uniform in [256, 2^20] limbs drawn as slices of a device-resident
-    random pool, admission by a fixed HBM budget, continuous batching over 3
-    streams. Every result is checked on the CPU by fingerprints mod two
-    61-bit primes; 16 sampled results are compared with the same request run
-    alone."""
-    import numpy as np
+class DevFingerprint:
+    """x mod q of device-resident u64 limb tensors for primes q < 2^31,
+    computed by torch's own elementwise kernels on the device (independent
+    of this library's kernels): the 32-bit words times 2^(32 j) mod q,
+    summed. Powers are built once on the host for up to `maxlimbs` limbs."""
+
+    def __init__(self, q, maxlimbs):
+        import numpy as np
+        import torch
+        from simd_bigint_amd.workloads import fingerprint32
+        self.q = q
+        # 2^(32 j) mod q for j < 2 maxlimbs: the fingerprint of limb words e_j
+        n = 2 * maxlimbs
+        B = 4096
+        base = np.empty(B, dtype=np.int64)
+        x, w = 1, (1 << 32) % q
+        for i in range(B):
+            base[i] = x
+            x = x * w % q
+        heads = np.empty((n + B - 1) // B, dtype=np.int64)
+        y = 1
+        for k in range(len(heads)):
+            heads[k] = y
+            y = y * x % q
+        pw = (heads[:, None] * base[None, :] % q).reshape(-1)[:n]
+        self.pw = torch.from_numpy(pw).cuda()
+        probe = np.arange(1, 1001, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+        assert self.of(torch.from_numpy(probe.view(np.int64)).cuda()) == fingerprint32(probe, q)
+
+    def of(self, t):
+        import torch
+        lo = t & 0xFFFFFFFF
+        hi = (t >> 32) & 0xFFFFFFFF
+        h = torch.stack((lo, hi), dim=1).reshape(-1)
+        return int(((h % self.q) * self.pw[: h.numel()] % self.q).sum().item() % self.q)
+
+
+def serve_run(nreq, rate, pool, budget, max_batch, seed, fps, alone_checks=16):
+    """One serving run at `rate` requests/s; every result checked by device
+    fingerprints, a sample against the same request run alone."""
     import torch
     import simd_bigint_amd as sba
     from simd_bigint_amd import serving
-    from simd_bigint_amd.workloads import to_int
-
-    rng = np.random.default_rng(17)
-    pool_np = rng.integers(0, 2**64, 1 << 21, dtype=np.uint64)
-    pool = torch.from_numpy(pool_np.view(np.int64)).cuda()
-    budget = 2 << 30
-    reqs = serving.poisson_requests(nreq, rate, pool, seed=18)
-    srv = serving.Server(budget, nstreams=3, max_batch=128)
-    # warm the size classes once (plans, twiddles, allocator)
-    serving.Server(budget, nstreams=3, max_batch=128).run(
-        serving.poisson_requests(32, 1e6, pool, seed=19))
+    reqs = serving.poisson_requests(nreq, rate, pool, seed=seed)
+    srv = serving.Server(budget, nstreams=3, max_batch=max_batch)
     stats = srv.run(reqs)
     out = stats.summary()
-    qs = [(1 << 61) - 1, 2305843009213693921]
     ok = True
     for q in reqs:
-        r = to_int(q.r.cpu().numpy().view(np.uint64))
-        a = to_int(q.a.cpu().numpy().view(np.uint64))
-        b = a if q.b is None else to_int(q.b.cpu().numpy().view(np.uint64))
-        for m in qs:
-            ok &= (r % m) == (a % m) * (b % m) % m
+        n = q.limbs
+        for fp in fps:
+            fa = fp.of(q.a)
+            fb = fa if q.b is None else fp.of(q.b)
+            ok &= fp.of(q.r[:n]) == fa * fb % fp.q
     same = True
-    for q in reqs[:: max(nreq // 16, 1)]:
+    for q in reqs[:: max(nreq // alone_checks, 1)]:
         alone = torch.empty(q.limbs, dtype=torch.int64, device="cuda")
         if q.b is None:
             sba.sqr_dev(alone, q.a)
         else:
             sba.mul_dev(alone, q.a, q.b)
         torch.cuda.synchronize()
-        same &= torch.equal(alone, q.r)
-    out.update({"rate_per_s": rate, "hbm_budget_bytes": budget,
+        same &= torch.equal(alone, q.r[: q.limbs])
+    out.update({"rate_per_s": rate, "arrival_span_s": round(reqs[-1].arrival, 4),
+                "bit_exact": bool(ok and same)})
+    del srv
+    return out
+
+
+def serve_leg(nreq, rates):
+    """Serving: Poisson arrivals (80% multiplies / 20% squares), operand sizes
+    log-uniform in [256, 2^20] limbs drawn as slices of a device-resident
+    random pool, workspace from one preallocated 2 GiB arena, continuous
+    batching over 3 streams (at most 128 requests per launch), at several
+    arrival rates up to saturation (where muls/s stops following the
+    arrival rate). Every result is checked by fingerprints mod two 31-bit
+    primes computed on the device by torch; 16 results per rate are
+    compared with the same request run alone."""
+    import numpy as np
+    import torch
+    from simd_bigint_amd import serving
+
+    rng = np.random.default_rng(17)
+    pool_np = rng.integers(0, 2**64, 1 << 21, dtype=np.uint64)
+    pool = torch.from_numpy(pool_np.view(np.int64)).cuda()
+    budget = 2 << 30
+    fps = [DevFingerprint(q, 1 << 21) for q in (2147483647, 2147483629)]
+    # warm the size classes once (plans, root tables, torch's allocator)
+    serving.Server(budget, nstreams=3, max_batch=128).run(
+        serving.poisson_requests(64, 1e6, pool, seed=19))
+    sweep = []
+    for k, rate in enumerate(rates):
+        res = serve_run(nreq, rate, pool, budget, 128, 18 + k, fps)
+        log("bench: serving", res)
+        sweep.append(res)
+    out = dict(sweep[0])
+    out.update({"hbm_workspace_bytes": budget,
                 "sizes": "log-uniform [256, 2^20] limbs, 80% mul / 20% sqr",
-                "bit_exact": bool(ok and same),
-                "check": "all results: fingerprints mod 2 primes; 16 sampled: == alone"})
-    log("bench: serving", out)
+                "rate_sweep": sweep,
+                "saturation_muls_per_s": max(r["muls_per_s"] for r in sweep),
+                "bit_exact": all(r["bit_exact"] for r in sweep),
+                "check": "every result: fingerprints mod 2 primes (torch on the device); "
+                         "16 per rate: == the request run alone"})
     return out
 
 
@@ -456,8 +513,8 @@ def main():
                     help="also time the other benchmark shapes (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=1000,
                     help="requests in the serving run (0 = skip)")
-    ap.add_argument("--serve-rate", type=float, default=5000.0,
-                    help="Poisson arrival rate of the serving run, requests/s")
+    ap.add_argument("--serve-rates", default="2000,10000,50000,250000",
+                    help="Poisson arrival rates of the serving runs, requests/s")
     ap.add_argument("--cpu-budget", type=float, default=12.0,
                     help="seconds of CPU work for the cpu_baseline sample (0 = skip)")
     ap.add_argument("--shape", default="metric_4Mi",
@@ -592,7 +649,8 @@ def main():
         batched = batch_leg(args.batch_jobs, L)
     served = None
     if args.serve_requests > 0:
-        served = serve_leg(args.serve_requests, args.serve_rate)
+        served = serve_leg(args.serve_requests,
+                           [float(x) for x in args.serve_rates.split(",") if x])
 
     cpu = None
     if args.cpu_budget > 0:

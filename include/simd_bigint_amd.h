@@ -12,6 +12,23 @@
  * entries (sba_gpu_*) take device buffers and a HIP stream and only
  * enqueue work.
  *
+ * Errors. The int entries of the reference return 1 or 0 (0: outside the
+ * band, include/ntmul.h:19). A HIP failure inside a host entry (device or
+ * pinned allocation, copy, launch) is reported on stderr and never aborts:
+ * nt_* / p30_* / p50_* then return 0 with rp unspecified; the void entries
+ * simd_mpn_mul / divrem_u64* finish that product on the library's CPU tier
+ * and count it (sba_gpu_fallbacks). A process without any HIP device
+ * aborts in a host entry that needs the GPU (the library never runs the
+ * GPU path on the CPU unannounced), and host memory exhaustion aborts, as
+ * the reference's arena does (include/scratch.h:66-70). Device entries
+ * return the codes below.
+ *
+ * Device memory. The first GPU call on a device builds its root tables
+ * once: 2 x 5 primes x 2^24 x 4 B = 640 MiB, held until the process ends
+ * (one table serves every transform length up to 2^25). Entries without a
+ * caller workspace add a grow-only workspace of about 40 B per transform
+ * point of the largest product seen (2^23 points: 336 MiB).
+ *
  * Citations are files of the reference repository (iamstupid/simd_bigint).
  */
 #ifndef SIMD_BIGINT_AMD_H
@@ -39,9 +56,9 @@ struct scratch;
 
 /* ---------------------------------------------------------------------- */
 /* NTT dispatcher: replaces include/ntmul.h:47-79.                         */
-/* rp[0 .. an+bn) = {ap,an} * {bp,bn}. Returns 1, or 0 only beyond the     */
-/* band (an <= 0, bn <= 0, min(an,bn) > sba_gpu_min_operand_cap(), or      */
-/* an+bn-1 > 2^sba_gpu_max_lg()). rp must not overlap the inputs.          */
+/* rp[0 .. an+bn) = {ap,an} * {bp,bn}. Returns 1, or 0 exactly outside the */
+/* reference's band (sba_ref_nt_band: its p50 band, include/p50.h:1020-    */
+/* 1046) and on a HIP failure. rp must not overlap the inputs.             */
 /* ---------------------------------------------------------------------- */
 int nt_mul(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, const uint64_t *bp,
            ptrdiff_t bn);                                   /* ntmul.h:71-74 */
@@ -73,8 +90,9 @@ int p50_sqr_r(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, struct scratch *sc
 /* Public multiply with threshold dispatch: replaces include/u64cvt.h:94-  */
 /* 132. rp[0 .. an+bn) = product; operands in either order; never writes   */
 /* past an+bn; zero operands give zero. Sizes below the GPU crossover run  */
-/* the library's own CPU basecase / Karatsuba / Toom-3 (the reference's    */
-/* ladder, include/mul.h:764-795); above it, the GPU NTT.                  */
+/* the library's own CPU basecase / Karatsuba (in the roles of the         */
+/* reference's ladder, include/mul.h:764-795; no Toom level), with a       */
+/* per-thread grow-only scratch; above it, the GPU NTT.                    */
 /* ---------------------------------------------------------------------- */
 void simd_mpn_mul(uint64_t *rp, const uint64_t *ap, uint64_t an,
                   const uint64_t *bp, uint64_t bn);
@@ -292,13 +310,23 @@ int sba_gpu_profile_get(int idx, const char **name, double *ms, double *bytes,
  * registers alone (no memory traffic): the VALU ceiling the transform
  * kernels are measured against. SBA_OK, or an error code. */
 int sba_gpu_probe_butterflies(double *per_s);
-/* the CPU tier alone (basecase + Karatsuba), either operand order */
 /* 1 when the reference's nt_mul would accept (an, bn) (its p50 band,
  * include/p50.h:1020-1046); nt_mul / nt_sqr / p50_* return 0 exactly
- * outside it. */
+ * outside it (or on a HIP failure). */
 int sba_ref_nt_band(int64_t an, int64_t bn);
+/* the CPU tier alone (basecase + Karatsuba), either operand order */
 void sba_cpu_mul(uint64_t *rp, const uint64_t *ap, uint64_t an,
                  const uint64_t *bp, uint64_t bn);
+/* products that simd_mpn_mul or the division finished on the CPU tier
+ * after a HIP failure, since the process started */
+uint64_t sba_gpu_fallbacks(void);
+/* failure injection for tests of the error contract: the next `count`
+ * GPU multiplies of the host entries fail before touching the device
+ * (SBA_FAIL_ENTRY), or the next `count` device / pinned staging
+ * allocations of the host entries fail (SBA_FAIL_ALLOC); count 0 disarms */
+#define SBA_FAIL_ENTRY 0
+#define SBA_FAIL_ALLOC 1
+void sba_test_inject_failure(int where, int count);
 
 #ifdef __cplusplus
 }

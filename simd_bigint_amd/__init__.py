@@ -10,7 +10,9 @@ plumbing for device memory and streams only.
 
 The library has no CPU fallback for the GPU path: if the shared library is
 missing, importing the loader raises; if no HIP device is present, the GPU
-entries abort with a message.
+entries abort with a message. A HIP failure inside a host entry is reported
+and passed up: nt_* return 0, simd_mpn_mul / divrem finish that product on
+the library's CPU tier and count it (``gpu_fallbacks``).
 """
 from __future__ import annotations
 
@@ -25,6 +27,9 @@ LIB_PATH = os.path.join(_HERE, "lib", "libsimd_bigint_amd.so")
 
 SBA_OK = 1
 SBA_BAND = 0
+SBA_ERR_HIP = -1
+SBA_ERR_WORKSPACE = -2
+SBA_ERR_NODEV = -3
 
 _lib: Optional[ctypes.CDLL] = None
 
@@ -63,6 +68,14 @@ def lib() -> ctypes.CDLL:
     for name in ("nt_sqr", "p30_sqr", "p50_sqr"):
         f = getattr(L, name)
         f.argtypes = [_u64p, _u64p, sp]
+        f.restype = ctypes.c_int
+    for name in ("nt_mul_r", "p30_mul_r", "p50_mul_r"):
+        f = getattr(L, name)
+        f.argtypes = [_u64p, _u64p, sp, _u64p, sp, vp]
+        f.restype = ctypes.c_int
+    for name in ("nt_sqr_r", "p30_sqr_r", "p50_sqr_r"):
+        f = getattr(L, name)
+        f.argtypes = [_u64p, _u64p, sp, vp]
         f.restype = ctypes.c_int
     L.simd_mpn_mul.argtypes = [_u64p, _u64p, u64, _u64p, u64]
     L.simd_mpn_mul.restype = None
@@ -174,8 +187,29 @@ def lib() -> ctypes.CDLL:
     L.sba_tune_gpu_threshold.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_double)]
     L.sba_tune_gpu_threshold.restype = u64
+    L.sba_gpu_fallbacks.argtypes = []
+    L.sba_gpu_fallbacks.restype = u64
+    L.sba_test_inject_failure.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.sba_test_inject_failure.restype = None
     _lib = L
     return L
+
+
+FAIL_ENTRY = 0
+FAIL_ALLOC = 1
+
+
+def gpu_fallbacks() -> int:
+    """Products simd_mpn_mul or the division finished on the CPU tier after
+    a HIP failure, since the process started."""
+    return int(lib().sba_gpu_fallbacks())
+
+
+def inject_failure(where: int, count: int) -> None:
+    """Test hook: the next `count` GPU multiplies of the host entries fail
+    before touching the device (FAIL_ENTRY), or the next `count` staging
+    allocations fail (FAIL_ALLOC)."""
+    lib().sba_test_inject_failure(int(where), int(count))
 
 
 def _as_limbs(a) -> np.ndarray:
@@ -203,15 +237,46 @@ def nt_sqr(a) -> Tuple[int, np.ndarray]:
 
 
 def p30_mul(a, b) -> Tuple[int, np.ndarray]:
+    """include/p30.h:1940 (p30's band, include/p30.h:1851-1858)."""
     a, b = _as_limbs(a), _as_limbs(b)
     r = np.zeros(len(a) + len(b), dtype=np.uint64)
     return lib().p30_mul(_arr(r), _arr(a), len(a), _arr(b), len(b)), r
 
 
 def p50_mul(a, b) -> Tuple[int, np.ndarray]:
+    """include/p50.h:1096."""
     a, b = _as_limbs(a), _as_limbs(b)
     r = np.zeros(len(a) + len(b), dtype=np.uint64)
     return lib().p50_mul(_arr(r), _arr(a), len(a), _arr(b), len(b)), r
+
+
+def p30_sqr(a) -> Tuple[int, np.ndarray]:
+    """include/p30.h:2018."""
+    a = _as_limbs(a)
+    r = np.zeros(2 * len(a), dtype=np.uint64)
+    return lib().p30_sqr(_arr(r), _arr(a), len(a)), r
+
+
+def p50_sqr(a) -> Tuple[int, np.ndarray]:
+    """include/p50.h:1176."""
+    a = _as_limbs(a)
+    r = np.zeros(2 * len(a), dtype=np.uint64)
+    return lib().p50_sqr(_arr(r), _arr(a), len(a)), r
+
+
+def mul_r(name: str, a, b=None) -> Tuple[int, np.ndarray]:
+    """The reference's `_r` entries (nt_mul_r, nt_sqr_r, p30_mul_r,
+    p30_sqr_r, p50_mul_r, p50_sqr_r: include/ntmul.h:47-69,
+    include/p30.h:1847,1950, include/p50.h:1020,1106) with a NULL arena;
+    b None for the squares."""
+    a = _as_limbs(a)
+    f = getattr(lib(), name)
+    if b is None:
+        r = np.zeros(2 * len(a), dtype=np.uint64)
+        return f(_arr(r), _arr(a), len(a), None), r
+    b = _as_limbs(b)
+    r = np.zeros(len(a) + len(b), dtype=np.uint64)
+    return f(_arr(r), _arr(a), len(a), _arr(b), len(b), None), r
 
 
 def simd_mpn_mul(a, b) -> np.ndarray:

@@ -245,18 +245,18 @@ static void divrem_core(u64 *qp, u64 *rp, const u64 *np, u64 nn64, const u64 *dp
       fprintf(stderr, "simd_bigint_amd: quotient estimate out of bounds\n");
       abort();
     }
-    if (rc != SBA_OK) {
-      fprintf(stderr, "simd_bigint_amd: GPU division failed (code %d); the library has no "
-                      "CPU fallback for the GPU path\n", rc);
-      abort();
+    if (rc == SBA_OK) {
+      for (u64 i = qn; i < qm; ++i) qp[i] = 0;
+      u64 qt = trim(qp, qm);
+      *qn_out = qt ? qt : 1;
+      for (u64 i = dn; i < dn64; ++i) rp[i] = 0;
+      u64 rt = trim(rp, dn64);
+      *rn_out = rt ? rt : 1;
+      return;
     }
-    for (u64 i = qn; i < qm; ++i) qp[i] = 0;
-    u64 qt = trim(qp, qm);
-    *qn_out = qt ? qt : 1;
-    for (u64 i = dn; i < dn64; ++i) rp[i] = 0;
-    u64 rt = trim(rp, dn64);
-    *rn_out = rt ? rt : 1;
-    return;
+    /* a HIP failure (reported): the host Newton branch below finishes the
+     * division, its multiplies through simd_mpn_mul */
+    sba_note_fallback("division on device buffers", rc);
   }
   u64 *Dn = (u64 *)xmalloc(dn * 8);
   lshift(Dn, dp, dn, s);

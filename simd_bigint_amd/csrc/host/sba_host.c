@@ -27,10 +27,40 @@ static size_t g_na, g_nb, g_nr;
 static pthread_once_t g_dev_once = PTHREAD_ONCE_INIT;
 static int g_dev_ok = 0;
 
+/* Error contract. A HIP failure inside a host entry (allocation, copy,
+ * launch) is reported on stderr and passed up: nt_* / p30_* / p50_* return
+ * 0, simd_mpn_mul and the division finish that product on the CPU tier
+ * (counted by sba_gpu_fallbacks). Only a process without any HIP device
+ * aborts: the library never computes the GPU path on the CPU silently. */
 static void die(const char *what, int rc) {
   fprintf(stderr, "simd_bigint_amd: %s failed (code %d); the library has no CPU "
                   "fallback for the GPU path\n", what, rc);
   abort();
+}
+static void report(const char *what, int rc) {
+  fprintf(stderr, "simd_bigint_amd: %s failed (code %d)\n", what, rc);
+}
+
+static _Atomic uint64_t g_fallbacks;
+uint64_t sba_gpu_fallbacks(void) { return atomic_load(&g_fallbacks); }
+void sba_note_fallback(const char *what, int rc) {
+  report(what, rc);
+  fprintf(stderr, "simd_bigint_amd: that product is finished on the CPU tier\n");
+  atomic_fetch_add(&g_fallbacks, 1);
+}
+
+/* failure injection for the tests of this contract */
+static _Atomic int g_fail_entry, g_fail_alloc;
+void sba_test_inject_failure(int where, int count) {
+  if (where == SBA_FAIL_ENTRY) atomic_store(&g_fail_entry, count);
+  else if (where == SBA_FAIL_ALLOC) atomic_store(&g_fail_alloc, count);
+}
+/* 1 when an armed injection point fires (and disarms one count) */
+static int inject(_Atomic int *ctr) {
+  int c = atomic_load(ctr);
+  while (c > 0)
+    if (atomic_compare_exchange_weak(ctr, &c, c - 1)) return 1;
+  return 0;
 }
 
 static void device_probe(void) {
@@ -44,14 +74,22 @@ static int device_ok(void) {
   return g_dev_ok;
 }
 
-static void grow(uint64_t **p, size_t *cap, size_t n) {
-  if (*cap >= n) return;
-  if (*p) hipFree(*p);
+/* grow-only device buffer of >= n limbs; SBA_ERR_HIP leaves it empty (an
+ * armed SBA_FAIL_ALLOC injection acts as a failed reallocation even when
+ * the buffer is large enough) */
+static int grow(uint64_t **p, size_t *cap, size_t n) {
+  if (*cap >= n && atomic_load(&g_fail_alloc) <= 0) return 0;
+  if (*p) (void)hipFree(*p);
   *p = NULL;
   *cap = 0;
   size_t want = n + n / 8 + 64;
-  if (hipMalloc((void **)p, want * 8) != hipSuccess) die("hipMalloc", SBA_ERR_HIP);
+  if (inject(&g_fail_alloc) || hipMalloc((void **)p, want * 8) != hipSuccess) {
+    (void)hipGetLastError();             /* do not leave the error for a later check */
+    *p = NULL;
+    return SBA_ERR_HIP;
+  }
   *cap = want;
+  return 0;
 }
 
 /* pinned host staging (grow-only): the caller's pageable buffers are copied
@@ -60,15 +98,20 @@ static void grow(uint64_t **p, size_t *cap, size_t n) {
 static uint64_t *g_ha, *g_hb, *g_hr;
 static size_t g_hna, g_hnb, g_hnr;
 
-static void grow_pinned(uint64_t **p, size_t *cap, size_t n) {
-  if (*cap >= n) return;
-  if (*p) hipHostFree(*p);
+static int grow_pinned(uint64_t **p, size_t *cap, size_t n) {
+  if (*cap >= n && atomic_load(&g_fail_alloc) <= 0) return 0;
+  if (*p) (void)hipHostFree(*p);
   *p = NULL;
   *cap = 0;
   size_t want = n + n / 8 + 64;
-  if (hipHostMalloc((void **)p, want * 8, hipHostMallocDefault) != hipSuccess)
-    die("hipHostMalloc", SBA_ERR_HIP);
+  if (inject(&g_fail_alloc) ||
+      hipHostMalloc((void **)p, want * 8, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = NULL;
+    return SBA_ERR_HIP;
+  }
   *cap = want;
+  return 0;
 }
 
 #define COPY_THREADS 8
@@ -123,64 +166,89 @@ static size_t stage_piece(size_t bytes) {
 }
 
 /* dev <- host through the pinned buffer pin, on the default stream */
-static void stage_up(void *dev, const void *host, void *pin, size_t bytes) {
+static int stage_up(void *dev, const void *host, void *pin, size_t bytes) {
   const size_t piece = stage_piece(bytes);
   for (size_t off = 0; off < bytes; off += piece) {
     const size_t n = bytes - off < piece ? bytes - off : piece;
     par_copy((char *)pin + off, (const char *)host + off, n);
     if (hipMemcpyAsync((char *)dev + off, (char *)pin + off, n, hipMemcpyHostToDevice, NULL) !=
         hipSuccess)
-      die("hipMemcpyAsync H2D", SBA_ERR_HIP);
+      return SBA_ERR_HIP;
   }
+  return 0;
 }
 
 /* host <- dev through the pinned buffer pin: every piece's DMA is queued with
  * an event behind it, and each piece is copied out as soon as it lands */
-static void stage_down(void *host, const void *dev, void *pin, size_t bytes) {
+static int stage_down(void *host, const void *dev, void *pin, size_t bytes) {
   const size_t piece = stage_piece(bytes);
   int k = 0;
   for (size_t off = 0; off < bytes; off += piece, ++k) {
     const size_t n = bytes - off < piece ? bytes - off : piece;
     if (!g_stage_ev[k] && hipEventCreateWithFlags(&g_stage_ev[k], hipEventDisableTiming) !=
-                              hipSuccess)
-      die("hipEventCreate", SBA_ERR_HIP);
+                              hipSuccess) {
+      g_stage_ev[k] = NULL;
+      return SBA_ERR_HIP;
+    }
     if (hipMemcpyAsync((char *)pin + off, (const char *)dev + off, n, hipMemcpyDeviceToHost,
                        NULL) != hipSuccess ||
         hipEventRecord(g_stage_ev[k], NULL) != hipSuccess)
-      die("hipMemcpyAsync D2H", SBA_ERR_HIP);
+      return SBA_ERR_HIP;
   }
   k = 0;
   for (size_t off = 0; off < bytes; off += piece, ++k) {
     const size_t n = bytes - off < piece ? bytes - off : piece;
-    if (hipEventSynchronize(g_stage_ev[k]) != hipSuccess) die("hipEventSynchronize", SBA_ERR_HIP);
+    if (hipEventSynchronize(g_stage_ev[k]) != hipSuccess) return SBA_ERR_HIP;
     par_copy((char *)host + off, (const char *)pin + off, n);
   }
+  return 0;
 }
 
-/* H2D, multiply on the default stream, D2H. Returns the engine's code. */
 static int gpu_in_band(size_t an, size_t bn) {
   return sba_gpu_in_band((int64_t)an, (int64_t)bn);
 }
 
+/* H2D, multiply on the default stream, D2H. SBA_OK; SBA_BAND outside the
+ * device band (nothing touched); < 0 on a HIP failure (reported on stderr;
+ * rp unspecified). Aborts only when the process has no HIP device. */
 static int gpu_mul_host(uint64_t *rp, const uint64_t *ap, size_t an,
                         const uint64_t *bp, size_t bn, int sqr) {
   if (!gpu_in_band(an, bn)) return SBA_BAND;   /* before touching any buffer */
+  if (inject(&g_fail_entry)) {
+    report("GPU multiply (injected failure)", SBA_ERR_HIP);
+    return SBA_ERR_HIP;
+  }
   if (!device_ok()) die("device lookup (no HIP device)", SBA_ERR_NODEV);
   pthread_mutex_lock(&g_stage_mu);
-  grow(&g_da, &g_na, an);
-  if (!sqr) grow(&g_db, &g_nb, bn);
-  grow(&g_dr, &g_nr, an + bn);
-  grow_pinned(&g_ha, &g_hna, an);
-  if (!sqr) grow_pinned(&g_hb, &g_hnb, bn);
-  grow_pinned(&g_hr, &g_hnr, an + bn);
-  stage_up(g_da, ap, g_ha, an * 8);
-  if (!sqr) stage_up(g_db, bp, g_hb, bn * 8);    /* overlaps a's last DMA */
-  int rc = sqr ? sba_gpu_sqr_dev(g_dr, g_da, (int64_t)an, NULL)
-               : sba_gpu_mul_dev(g_dr, g_da, (int64_t)an, g_db, (int64_t)bn, NULL);
-  if (rc < 0) die("GPU multiply", rc);
-  if (rc == SBA_OK) stage_down(rp, g_dr, g_hr, (an + bn) * 8);
-  else if (hipStreamSynchronize(NULL) != hipSuccess)   /* staging idle before reuse */
-    die("hipStreamSynchronize", SBA_ERR_HIP);
+  int rc = 0;
+  if (!rc) rc = grow(&g_da, &g_na, an);
+  if (!rc && !sqr) rc = grow(&g_db, &g_nb, bn);
+  if (!rc) rc = grow(&g_dr, &g_nr, an + bn);
+  if (!rc) rc = grow_pinned(&g_ha, &g_hna, an);
+  if (!rc && !sqr) rc = grow_pinned(&g_hb, &g_hnb, bn);
+  if (!rc) rc = grow_pinned(&g_hr, &g_hnr, an + bn);
+  if (rc) {
+    report("device or pinned staging allocation", rc);
+  } else {
+    rc = stage_up(g_da, ap, g_ha, an * 8);
+    if (!rc && !sqr) rc = stage_up(g_db, bp, g_hb, bn * 8);    /* overlaps a's last DMA */
+    if (rc) report("hipMemcpyAsync H2D", rc);
+  }
+  if (!rc) {
+    rc = sqr ? sba_gpu_sqr_dev(g_dr, g_da, (int64_t)an, NULL)
+             : sba_gpu_mul_dev(g_dr, g_da, (int64_t)an, g_db, (int64_t)bn, NULL);
+    if (rc == SBA_OK) {
+      const int c = stage_down(rp, g_dr, g_hr, (an + bn) * 8);
+      if (c) {
+        report("hipMemcpyAsync D2H", c);
+        rc = c;
+      }
+    } else if (rc < 0) {
+      report("GPU multiply", rc);
+    }
+  }
+  /* staging idle before the next caller reuses it */
+  if (rc != SBA_OK && hipStreamSynchronize(NULL) != hipSuccess) (void)hipGetLastError();
   pthread_mutex_unlock(&g_stage_mu);
   return rc;
 }
@@ -220,16 +288,14 @@ int nt_mul_r(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, const uint64_t *bp,
              ptrdiff_t bn, struct scratch *sc) {
   (void)sc;
   if (!p50_band(an, bn)) return 0;
-  if (gpu_mul_host(rp, ap, (size_t)an, bp, (size_t)bn, 0) != SBA_OK)
-    die("GPU multiply inside the nt_mul band", SBA_BAND);
-  return 1;
+  /* inside the reference band the device band always holds; a HIP failure
+   * returns 0 (reported on stderr): the product is not written */
+  return gpu_mul_host(rp, ap, (size_t)an, bp, (size_t)bn, 0) == SBA_OK;
 }
 int nt_sqr_r(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, struct scratch *sc) {
   (void)sc;
   if (!p50_band(an, an)) return 0;
-  if (gpu_mul_host(rp, ap, (size_t)an, ap, (size_t)an, 1) != SBA_OK)
-    die("GPU square inside the nt_sqr band", SBA_BAND);
-  return 1;
+  return gpu_mul_host(rp, ap, (size_t)an, ap, (size_t)an, 1) == SBA_OK;
 }
 int nt_mul(uint64_t *rp, const uint64_t *ap, ptrdiff_t an, const uint64_t *bp,
            ptrdiff_t bn) {
@@ -383,32 +449,60 @@ static size_t kara_tmp(size_t n) {
   return t + 64;
 }
 
-/* an >= bn >= 1 */
-static void cpu_mul(uint64_t *rp, const uint64_t *ap, size_t an, const uint64_t *bp,
-                    size_t bn) {
+/* scratch limbs cpu_mul_ws needs for an >= bn: the Karatsuba levels, one
+ * 2bn-limb block product, and the strip-mined last block's own call */
+static size_t cpu_need(size_t an, size_t bn) {
+  if (bn < SBA_KARA_MIN) return 0;
+  size_t t = kara_tmp(bn);
+  if (an == bn) return t;
+  t += 2 * bn + 8;
+  const size_t w = an % bn;
+  if (w) t += cpu_need(bn, w);
+  return t;
+}
+
+/* an >= bn >= 1; tmp holds cpu_need(an, bn) limbs */
+static void cpu_mul_ws(uint64_t *rp, const uint64_t *ap, size_t an, const uint64_t *bp,
+                       size_t bn, uint64_t *tmp) {
   if (bn < SBA_KARA_MIN) {
     cpu_basecase(rp, ap, an, bp, bn);
     return;
   }
-  uint64_t *tmp = (uint64_t *)malloc((kara_tmp(bn) + 4 * bn + 8) * 8);
-  if (!tmp) die("malloc", 0);
   if (an == bn) {
     cpu_mul_bal(rp, ap, bp, bn, tmp);
-    free(tmp);
     return;
   }
   /* strip-mine a in bn-limb blocks, overlap-adding each 2bn product */
-  uint64_t *blk = tmp + kara_tmp(bn);
+  uint64_t *blk = tmp + kara_tmp(bn), *inner = blk + 2 * bn + 8;
   memset(rp, 0, (an + bn) * 8);
   size_t done = 0;
   while (done < an) {
     size_t w = an - done < bn ? an - done : bn;
     if (w == bn) cpu_mul_bal(blk, ap + done, bp, bn, tmp);
-    else cpu_mul(blk, bp, bn, ap + done, w);
+    else cpu_mul_ws(blk, bp, bn, ap + done, w, inner);
     add_into(rp + done, an + bn - done, blk, w + bn);
     done += w;
   }
-  free(tmp);
+}
+
+/* per-thread grow-only scratch of the CPU tier (the role of the reference's
+ * thread arena, include/scratch.h:72-77,339-356: no heap traffic once the
+ * largest size has been seen). Host memory exhaustion aborts, as the
+ * reference's arena does (include/scratch.h:66-70). */
+static _Thread_local uint64_t *t_cpu_tmp;
+static _Thread_local size_t t_cpu_cap;
+
+/* an >= bn >= 1 */
+static void cpu_mul(uint64_t *rp, const uint64_t *ap, size_t an, const uint64_t *bp,
+                    size_t bn) {
+  const size_t need = cpu_need(an, bn);
+  if (need > t_cpu_cap) {
+    uint64_t *t = (uint64_t *)realloc(t_cpu_tmp, (need + need / 4) * 8);
+    if (!t) die("host scratch allocation", 0);
+    t_cpu_tmp = t;
+    t_cpu_cap = need + need / 4;
+  }
+  cpu_mul_ws(rp, ap, an, bp, bn, t_cpu_tmp);
 }
 
 /* ------------------------------------------------------------------ */
@@ -448,7 +542,7 @@ int sba_tune_sizes(uint64_t *totals, int max) {
 uint64_t sba_tune_gpu_threshold(int apply, double *cpu_ms, double *gpu_ms) {
   const uint64_t top = kTuneTotals[TUNE_N - 1];
   uint64_t *a = (uint64_t *)malloc(top * 8), *r = (uint64_t *)malloc(top * 8);
-  if (!a || !r) die("malloc", 0);
+  if (!a || !r) die("host allocation in the tuner", 0);
   uint64_t x = 0x9E3779B97F4A7C15ull;
   for (uint64_t i = 0; i < top; ++i) {
     x ^= x << 13; x ^= x >> 7; x ^= x << 17;
@@ -459,15 +553,20 @@ uint64_t sba_tune_gpu_threshold(int apply, double *cpu_ms, double *gpu_ms) {
     const uint64_t T = kTuneTotals[k], an = T / 2, bn = T - an;
     const uint64_t *b = a + an;
     double bc = 1e30, bg = 1e30;
-    if (gpu_mul_host(r, a, an, b, bn, 0) != SBA_OK) die("GPU multiply in the tuner", SBA_BAND);
-    for (int rep = 0; rep < 5; ++rep) {
+    int ok = gpu_mul_host(r, a, an, b, bn, 0) == SBA_OK;
+    for (int rep = 0; rep < 5 && ok; ++rep) {
       double t0 = now_ms();
       cpu_mul(r, a, an, b, bn);
       double t1 = now_ms();
-      if (gpu_mul_host(r, a, an, b, bn, 0) != SBA_OK) die("GPU multiply in the tuner", SBA_BAND);
+      ok = gpu_mul_host(r, a, an, b, bn, 0) == SBA_OK;
       double t2 = now_ms();
       if (t1 - t0 < bc) bc = t1 - t0;
       if (t2 - t1 < bg) bg = t2 - t1;
+    }
+    if (!ok) {                       /* a GPU failure (reported): no threshold */
+      free(a);
+      free(r);
+      return 0;
     }
     if (cpu_ms) cpu_ms[k] = bc;
     if (gpu_ms) gpu_ms[k] = bg;
@@ -497,10 +596,16 @@ void simd_mpn_mul(uint64_t *rp, const uint64_t *ap, uint64_t an, const uint64_t 
   if (an + bn >= sba_get_gpu_threshold()) {
     int rc = gpu_mul_host(rp, ap, an, bp, bn, ap == bp && an == bn);
     if (rc == SBA_OK) return;
+    if (rc < 0) {
+      /* a HIP failure (reported): the void entry still delivers the product */
+      sba_note_fallback("simd_mpn_mul on the GPU", rc);
+      cpu_mul(rp, ap, an, bp, bn);
+      return;
+    }
     /* beyond the GPU band (operands of > 43M limbs): split a in halves */
     size_t h = an / 2;
     uint64_t *t = (uint64_t *)malloc((an - h + bn) * 8);
-    if (!t) die("malloc", 0);
+    if (!t) die("host allocation", 0);
     simd_mpn_mul(rp, ap, h, bp, bn);
     memset(rp + h + bn, 0, (an - h) * 8);
     simd_mpn_mul(t, ap + h, an - h, bp, bn);

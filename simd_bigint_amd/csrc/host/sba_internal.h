@@ -24,6 +24,10 @@ void sba_mul(uint64_t *rp, const uint64_t *ap, size_t an, const uint64_t *bp, si
  * A*X < B^2n <= A*(X+2), B = 2^64 (host Newton, sba_div.c) */
 void sba_approx_recip_host(uint64_t *X, const uint64_t *A, size_t n);
 
+/* a GPU failure whose product the CPU tier finishes: reported on stderr
+ * and counted (sba_gpu_fallbacks) */
+void sba_note_fallback(const char *what, int rc);
+
 /* memcpy split over several host threads (sba_host.c) */
 void sba_par_copy(void *dst, const void *src, size_t bytes);
 

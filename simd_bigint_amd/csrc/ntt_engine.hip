@@ -492,27 +492,34 @@ static int ctx_init(DevCtx &c) {
   return 0;
 }
 
+// the root-table kernel's arguments for prime q
+static TwArgs tw_args(const DevCtx &c, int q) {
+  const uint64_t p = kPrimes[q];
+  const uint64_t g = h_primroot(p);
+  TwArgs ta;
+  ta.p = (uint32_t)p;
+  ta.pinv = c.pa.pinv[q];
+  ta.one = h_mont(1, p);
+  for (int t = 0; t < SBA_MAX_LG; ++t) {
+    // bit t of a node index carries zeta_{t+2}, a root of order 2^(t+2)
+    const int ord = t + 2;
+    uint64_t z = ord <= SBA_MAX_LG ? h_powm(g, (p - 1) >> ord, p) : 1;
+    ta.z[t] = h_mont(z, p);
+    ta.zi[t] = h_mont(h_invm(z, p), p);
+  }
+  return ta;
+}
+
 static int ctx_twiddles(DevCtx &c, int L, hipStream_t st) {
   if (L <= c.tw_lg) return 0;
   const int Lt = SBA_MAX_LG;            // build once for the largest transform
   const uint32_t count = 1u << (Lt - 1);
   for (int q = 0; q < SBA_K; ++q) {
     if (c.wn[q]) { HIPCHK(hipFree(c.wn[q])); HIPCHK(hipFree(c.wni[q])); }
+    c.wn[q] = c.wni[q] = nullptr;
     HIPCHK(hipMalloc(&c.wn[q], (size_t)count * 4));
     HIPCHK(hipMalloc(&c.wni[q], (size_t)count * 4));
-    const uint64_t p = kPrimes[q];
-    const uint64_t g = h_primroot(p);
-    TwArgs ta;
-    ta.p = (uint32_t)p;
-    ta.pinv = c.pa.pinv[q];
-    ta.one = h_mont(1, p);
-    for (int t = 0; t < SBA_MAX_LG; ++t) {
-      // bit t of a node index carries zeta_{t+2}, a root of order 2^(t+2)
-      const int ord = t + 2;
-      uint64_t z = ord <= SBA_MAX_LG ? h_powm(g, (p - 1) >> ord, p) : 1;
-      ta.z[t] = h_mont(z, p);
-      ta.zi[t] = h_mont(h_invm(z, p), p);
-    }
+    const TwArgs ta = tw_args(c, q);
     hipLaunchKernelGGL(k_twiddles, dim3((count + NT - 1) / NT), dim3(NT), 0, st, ta, c.wn[q],
                        c.wni[q], count);
     HIPCHK(hipGetLastError());

@@ -150,3 +150,33 @@ def batch_workload(n: int = 4096, seed: int = 4, lo: int = 64, hi: int = 65536,
                 b[rng.random(bn) < 0.9] = 0
         out.append((a, b))
     return out
+
+
+_FP_BASE = {}
+
+
+def fingerprint32(limbs, q: int) -> int:
+    """(the integer of the u64 limbs) mod q for a prime q < 2^31, in O(n)
+    numpy work: the 32-bit words h_j times 2^(32 j) mod q, summed mod q."""
+    assert 2 < q < 2**31
+    h = np.ascontiguousarray(limbs, dtype=np.uint64).view(np.uint32).astype(np.int64)
+    n = len(h)
+    if n == 0:
+        return 0
+    B = 4096
+    if q not in _FP_BASE:
+        base = np.empty(B, dtype=np.int64)
+        x, w = 1, (1 << 32) % q
+        for i in range(B):
+            base[i] = x
+            x = x * w % q
+        _FP_BASE[q] = (base, x)                  # x = 2^(32 B) mod q
+    base, stepB = _FP_BASE[q]
+    nb = (n + B - 1) // B
+    heads = np.empty(nb, dtype=np.int64)
+    y = 1
+    for k in range(nb):
+        heads[k] = y
+        y = y * stepB % q
+    pw = (heads[:, None] * base[None, :] % q).reshape(-1)[:n]
+    return int(((h % q) * pw % q).sum() % q)

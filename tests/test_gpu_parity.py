@@ -314,7 +314,7 @@ def test_serving_results_same_alone_and_in_mix():
     stats = srv.run(reqs)
     s = stats.summary()
     say("serve", s)
-    assert s["requests"] == len(reqs) and stats.peak_hbm_bytes <= budget
+    assert s["requests"] == len(reqs) and stats.peak_ws_bytes <= budget
     assert stats.batches < len(reqs)          # requests were batched together
     for q in reqs:
         alone = torch.empty(q.limbs, dtype=torch.int64, device="cuda")
