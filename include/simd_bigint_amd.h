@@ -47,6 +47,7 @@ extern "C" {
 #define SBA_ERR_HIP (-1)     /* a HIP runtime call failed (message on stderr) */
 #define SBA_ERR_WORKSPACE (-2) /* caller workspace missing or too small */
 #define SBA_ERR_NODEV (-3)   /* no HIP device: the library never computes on the CPU */
+#define SBA_ERR_TOO_LARGE (-4) /* beyond the allocation-free entry's static scratch */
 
 /* The reference's scratch arena type (include/scratch.h:248-256). The _r
  * entries accept the caller's arena for signature compatibility; device
@@ -129,6 +130,22 @@ int sba_gpu_mul_dev_ws(uint64_t *rp, const uint64_t *ap, int64_t an,
 int sba_gpu_sqr_dev_ws(uint64_t *rp, const uint64_t *ap, int64_t an, void *ws,
                        size_t ws_bytes, void *stream);
 size_t sba_gpu_workspace_bytes(int64_t an, int64_t bn);
+/* Allocation-free multiply. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same contract as
+ * sba_gpu_mul_dev (bp == ap with bn == an runs the square), but the call
+ * makes no hipMalloc / hipFree / allocating library call: its spectra,
+ * carry maps and root tables live in a static device array of
+ * SBA_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends. Products
+ * of up to 2^SBA_NOALLOC_MAX_LG coefficients (an + bn - 1), always one
+ * transform; larger ones return SBA_ERR_TOO_LARGE. The first call on a
+ * device builds that device's tables and waits for them. All calls on one
+ * GPU share its array: calls that overlap in time on one GPU (different
+ * streams, threads) race and must be serialised by the caller. */
+#define SBA_NOALLOC_MAX_LG 20
+#define SBA_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int sba_gpu_mul_dev_noalloc(uint64_t *rp, const uint64_t *ap, int64_t an,
+                            const uint64_t *bp, int64_t bn, void *stream);
 /* 1 when the device entries accept (an, bn): min(an, bn) within the CRT
  * cap and either an + bn - 1 <= 2^25 (one transform) or a chunked product
  * (the longer operand cut into chunks against the shorter one's spectrum,

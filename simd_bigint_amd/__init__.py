@@ -30,6 +30,8 @@ SBA_BAND = 0
 SBA_ERR_HIP = -1
 SBA_ERR_WORKSPACE = -2
 SBA_ERR_NODEV = -3
+SBA_ERR_TOO_LARGE = -4
+NOALLOC_MAX_LG = 20
 
 _lib: Optional[ctypes.CDLL] = None
 
@@ -94,6 +96,8 @@ def lib() -> ctypes.CDLL:
     L.sba_gpu_mul_dev_ws.restype = ctypes.c_int
     L.sba_gpu_sqr_dev_ws.argtypes = [vp, vp, ctypes.c_int64, vp, ctypes.c_size_t, vp]
     L.sba_gpu_sqr_dev_ws.restype = ctypes.c_int
+    L.sba_gpu_mul_dev_noalloc.argtypes = [vp, vp, ctypes.c_int64, vp, ctypes.c_int64, vp]
+    L.sba_gpu_mul_dev_noalloc.restype = ctypes.c_int
     L.sba_gpu_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int64]
     L.sba_gpu_workspace_bytes.restype = ctypes.c_size_t
     L.sba_gpu_min_operand_cap.argtypes = []
@@ -358,6 +362,15 @@ def mul_dev(r, a, b, stream=None) -> int:
     _check_dev(r, a.numel() + b.numel())
     s = None if stream is None else ctypes.c_void_p(int(stream.cuda_stream))
     return lib().sba_gpu_mul_dev(_ptr(r), _ptr(a), a.numel(), _ptr(b), b.numel(), s)
+
+
+def mul_dev_noalloc(r, a, b, stream=None) -> int:
+    """Enqueue r = a * b through the allocation-free entry (products of up
+    to 2^NOALLOC_MAX_LG coefficients; the device's static scratch is shared
+    by every such call: overlapping calls race)."""
+    _check_dev(r, a.numel() + b.numel())
+    s = None if stream is None else ctypes.c_void_p(int(stream.cuda_stream))
+    return lib().sba_gpu_mul_dev_noalloc(_ptr(r), _ptr(a), a.numel(), _ptr(b), b.numel(), s)
 
 
 def probe_butterflies() -> float:

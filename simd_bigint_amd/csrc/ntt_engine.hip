@@ -441,6 +441,11 @@ struct DevCtx {
   // grow-only workspace
   void *ws = nullptr;
   size_t ws_bytes = 0;
+  // the allocation-free entry: this device's copy of the static scratch
+  // array (its address from hipGetSymbolAddress) and whether its root
+  // tables are built
+  char *na_scratch = nullptr;
+  int na_tables = 0;
 };
 
 #define SBA_MAX_DEV 16
@@ -1273,6 +1278,129 @@ extern "C" int sba_gpu_sqr_dev_ws(uint64_t *rp, const uint64_t *ap, int64_t an, 
                                   size_t ws_bytes, void *stream) {
   if (!ws) return SBA_ERR_WORKSPACE;
   return mul_common(rp, ap, an, ap, an, 1, ws, ws_bytes, stream);
+}
+
+// ---------------------------------------------------------------------------
+// allocation-free multiply (for callers that cannot allocate device memory:
+// code under graph capture, tight serving loops). Everything the kernels use
+// beyond the operands and the product lives in one file-scope device array,
+// `scratch`, of SBA_NOALLOC_SCRATCH_BYTES per device:
+//   [0, NA_TW_BYTES)     root tables of the 5 primes for transforms up to
+//                        2^SBA_NOALLOC_MAX_LG points (wn, wni per prime),
+//                        built once per device by the first call
+//   [NA_TW_BYTES, ...)   a's and b's spectra and the carry maps of one
+//                        product (ws_need(2^SBA_NOALLOC_MAX_LG) bytes)
+// Every buffer the kernels read is written earlier in the same call (the
+// first pass writes all residues, Garner every carry map), so nothing needs
+// clearing between calls. In host code `scratch` names a host-side stand-in:
+// the device address comes from hipGetSymbolAddress, once per device.
+// ---------------------------------------------------------------------------
+#define NA_TW_COUNT ((size_t)1 << (SBA_NOALLOC_MAX_LG - 1))   // node twiddles per table
+#define NA_TW_BYTES (2 * SBA_K * NA_TW_COUNT * 4)
+static_assert(NA_TW_BYTES + (2 * SBA_K * ((size_t)4 << SBA_NOALLOC_MAX_LG) +
+                             (((size_t)1 << SBA_NOALLOC_MAX_LG) / CARRY_LPB + 2) * 4 + 4096) <=
+                  SBA_NOALLOC_SCRATCH_BYTES,
+              "static scratch too small for its largest product");
+static __device__ __align__(256) uint8_t scratch[SBA_NOALLOC_SCRATCH_BYTES];
+
+// this device's scratch address and its root tables (first call on the
+// device: hipGetSymbolAddress, then the table kernel on the caller's stream,
+// waited for so that later calls on any stream find the tables built)
+static int na_setup(DevCtx &c, hipStream_t st) {
+  if (!c.na_scratch) {
+    void *ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return SBA_ERR_HIP;
+    c.na_scratch = (char *)ptr;
+  }
+  if (!c.na_tables) {
+    uint32_t *tw = (uint32_t *)c.na_scratch;
+    for (int q = 0; q < SBA_K; ++q) {
+      const TwArgs ta = tw_args(c, q);
+      hipLaunchKernelGGL(k_twiddles, dim3((uint32_t)((NA_TW_COUNT + NT - 1) / NT)), dim3(NT), 0,
+                         st, ta, tw + (2 * q) * NA_TW_COUNT, tw + (2 * q + 1) * NA_TW_COUNT,
+                         (uint32_t)NA_TW_COUNT);
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    c.na_tables = 1;
+  }
+  return 0;
+}
+
+extern "C" int sba_gpu_mul_dev_noalloc(uint64_t *rp, const uint64_t *ap, int64_t an,
+                                       const uint64_t *bp, int64_t bn, void *stream) {
+  if (!rp || !ap || !bp || !in_band_direct(an, bn)) return SBA_BAND;
+  const int sqr = ap == bp && an == bn;
+  if (an < bn) {
+    const uint64_t *t = ap; ap = bp; bp = t;
+    int64_t tn = an; an = bn; bn = tn;
+  }
+  const uint64_t zlen = (uint64_t)an + (uint64_t)bn - 1;
+  if (zlen > ((uint64_t)1 << SBA_NOALLOC_MAX_LG)) {
+    fprintf(stderr,
+            "simd_bigint_amd: sba_gpu_mul_dev_noalloc: %llu coefficients exceed the static "
+            "scratch's largest transform (2^%d)\n",
+            (unsigned long long)zlen, SBA_NOALLOC_MAX_LG);
+    return SBA_ERR_TOO_LARGE;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lk(g_mu);
+  const int saved_prof = g_prof_on;
+  g_prof_on = 0;                                  // the profiler creates events: off here
+  struct Restore { int v; ~Restore() { g_prof_on = v; } } restore{saved_prof};
+  if (school_job((uint64_t)an, (uint64_t)bn, true)) {
+    JobRef jr;
+    jr.arr = nullptr;
+    Job &j = jr.one;
+    memset(&j, 0, sizeof j);
+    j.a = ap;
+    j.b = sqr ? ap : bp;
+    j.r = rp;
+    j.an = (uint32_t)an;
+    j.bn = (uint32_t)bn;
+    j.zlen = (uint32_t)zlen;
+    j.Z = (uint32_t)(zlen + 1);
+    j.nb = 1;
+    hipLaunchKernelGGL(k_school, dim3(1, 1, 1), dim3(NT), 0, st, jr);
+    HIPCHK(hipGetLastError());
+    return SBA_OK;
+  }
+  DevCtx *c;
+  int rc = get_ctx(&c);
+  if (rc) return rc;
+  if ((rc = na_setup(*c, st)) != 0) return rc;
+  // the context with the static tables in place of the allocated ones
+  DevCtx nc;
+  nc.pa = c->pa;
+  nc.ga_base = c->ga_base;
+  const uint32_t *tw = (const uint32_t *)c->na_scratch;
+  for (int q = 0; q < SBA_K; ++q) {
+    nc.pa.wn[q] = tw + (2 * q) * NA_TW_COUNT;
+    nc.pa.wni[q] = tw + (2 * q + 1) * NA_TW_COUNT;
+  }
+  // always one transform (power of two or 3 * 2^L), never chunked
+  const int L3 = r3_lg(zlen);
+  const int L = L3 ? L3 : lg_for(zlen);
+  const size_t Ntot = L3 ? (size_t)3 << L3 : (size_t)1 << L;
+  JobRef jr;
+  jr.arr = nullptr;
+  Job &j = jr.one;
+  memset(&j, 0, sizeof j);
+  j.a = ap;
+  j.b = sqr ? ap : bp;
+  j.r = rp;
+  j.an = (uint32_t)an;
+  j.bn = (uint32_t)bn;
+  j.zlen = (uint32_t)zlen;
+  j.Z = (uint32_t)(zlen + 1);
+  j.nb = carry_blocks(j.Z);
+  job_layout(j, c->na_scratch + NA_TW_BYTES, (uint32_t)Ntot);
+  if (L3)
+    return enqueue_r3(nc, L3, sqr, r3_plan(L3, sqr, 1), jr, 1, j.nb, (double)an, (double)bn,
+                      (double)j.Z, st);
+  return enqueue_class(nc, L, sqr, plan_for(L, sqr, 1), jr, 1, j.nb, (double)an, (double)bn,
+                       (double)j.Z, st);
 }
 
 // ---------------------------------------------------------------------------
