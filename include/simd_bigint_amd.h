@@ -157,8 +157,18 @@ int sba_gpu_in_band(int64_t an, int64_t bn);
 int sba_gpu_chunking(int64_t an, int64_t bn, int sqr, int *Lc, uint64_t *chunk,
                      uint64_t *nchunks);
 /* L when a single product of (an, bn) runs at the radix-3 length 3 * 2^L
- * (2^(L+1) < an+bn-1 <= 3 * 2^L, 13 <= L <= 22, not chunked), else 0 */
+ * (2^(L+1) < an+bn-1 <= 3 * 2^L, sba_get_r3_min_lg() <= L <= 22, not
+ * chunked), else 0. Batch jobs take radix-3 classes from L = 13. */
 int sba_gpu_radix3(int64_t an, int64_t bn, int sqr);
+/* smallest radix-3 row length 2^L for single products (default 16, measured
+ * on MI355X; batches always from 13) */
+void sba_set_r3_min_lg(int L);
+int sba_get_r3_min_lg(void);
+/* 1 when a single product of (an, bn) runs truncated (a power-of-two length
+ * 2^*L whose merged last pass transforms only the first *rows tiles of
+ * 2^*last residues, rows = ceil((an+bn-1) / 2^last), and whose inverse
+ * column pass is the truncated one), else 0 */
+int sba_gpu_truncated(int64_t an, int64_t bn, int sqr, int *L, int *last, int *rows);
 /* The carry stage between blocks of sba_gpu_carry_block_limbs() limbs on
  * its own (no counterpart in the reference; it lets a test drive the stage
  * with chosen carry chains): rp[0..Z) holds limbs each finished with block
@@ -307,6 +317,14 @@ uint64_t sba_get_gpu_threshold(void);
  * default, 32768, is the MI355X-measured crossover. */
 void sba_set_school_max(uint64_t max_product);
 uint64_t sba_get_school_max(void);
+/* truncation: a two-pass product (or batch class) of length >= 2^min_lg
+ * whose shortest product needs at most max_fill_pct percent of its
+ * last-pass tiles runs truncated; products of such lengths that fill at
+ * most r3_min_pct percent of their power-of-two length take that truncated
+ * length instead of the radix-3 length. max_fill_pct 0 turns truncation off.
+ * Defaults, measured on MI355X: 85, 0, 21. */
+void sba_set_trunc(int max_fill_pct, int r3_min_pct, int min_lg);
+void sba_get_trunc(int *max_fill_pct, int *r3_min_pct, int *min_lg);
 /* crossover tuner: times balanced products of each total in the ladder
  * that sba_tune_sizes lists (host buffers; the CPU multiply against the GPU
  * path with its copies, best of 5) and returns the smallest total from

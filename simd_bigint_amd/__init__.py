@@ -191,6 +191,16 @@ def lib() -> ctypes.CDLL:
     L.sba_tune_gpu_threshold.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_double)]
     L.sba_tune_gpu_threshold.restype = u64
+    L.sba_set_r3_min_lg.argtypes = [ctypes.c_int]
+    L.sba_set_r3_min_lg.restype = None
+    L.sba_get_r3_min_lg.argtypes = []
+    L.sba_get_r3_min_lg.restype = ctypes.c_int
+    L.sba_gpu_truncated.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ip, ip, ip]
+    L.sba_gpu_truncated.restype = ctypes.c_int
+    L.sba_set_trunc.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.sba_set_trunc.restype = None
+    L.sba_get_trunc.argtypes = [ip, ip, ip]
+    L.sba_get_trunc.restype = None
     L.sba_gpu_fallbacks.argtypes = []
     L.sba_gpu_fallbacks.restype = u64
     L.sba_test_inject_failure.argtypes = [ctypes.c_int, ctypes.c_int]
@@ -201,6 +211,37 @@ def lib() -> ctypes.CDLL:
 
 FAIL_ENTRY = 0
 FAIL_ALLOC = 1
+
+
+def truncated(an: int, bn: int, sqr: bool = False):
+    """(L, last, rows) when a single product runs truncated, else None."""
+    Lg, last, rows = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    if lib().sba_gpu_truncated(an, bn, 1 if sqr else 0, ctypes.byref(Lg), ctypes.byref(last),
+                               ctypes.byref(rows)):
+        return Lg.value, last.value, rows.value
+    return None
+
+
+def set_trunc(max_fill_pct: int, r3_min_pct: int, min_lg: int = 21) -> None:
+    """Truncation thresholds (percent of last-pass tiles, percent of the
+    power-of-two length, smallest log2 length); max_fill_pct 0 turns
+    truncation off."""
+    lib().sba_set_trunc(int(max_fill_pct), int(r3_min_pct), int(min_lg))
+
+
+def get_trunc():
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    lib().sba_get_trunc(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    return a.value, b.value, c.value
+
+
+def set_r3_min_lg(L: int) -> None:
+    """Smallest radix-3 row length 2^L for single products."""
+    lib().sba_set_r3_min_lg(int(L))
+
+
+def get_r3_min_lg() -> int:
+    return int(lib().sba_get_r3_min_lg())
 
 
 def gpu_fallbacks() -> int:

@@ -31,6 +31,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <algorithm>
 #include <atomic>
 #include <mutex>
 
@@ -480,6 +481,7 @@ static int ctx_init(DevCtx &c) {
     const uint64_t w = h_powm(h_primroot(p), (p - 1) / 3, p);
     const uint64_t inv2 = (p + 1) / 2;
     const uint64_t h2 = (w + p - w * w % p) % p * inv2 % p;
+    c.pa.inv2[q] = h_mont(inv2, p);
     c.pa.h1[q] = h_mont(p - inv2, p);
     c.pa.h2[q] = h_mont(h2, p);
     c.pa.h2i[q] = h_mont((p - h2) % p, p);
@@ -823,12 +825,57 @@ static PrimeArgs prime_args(const DevCtx &c, uint64_t Nt) {
 // carry blocks of a product of Z limbs
 static uint32_t carry_blocks(uint64_t Z) { return (uint32_t)((Z + CARRY_LPB - 1) / CARRY_LPB); }
 
+// Truncated products (two-pass plans): the merged last pass transforms only
+// the tiles H < ceil(zlen / 2^last) of a job and the truncated inverse column
+// pass recovers the coefficients from them. Used for a class whose shortest
+// product leaves at least a fraction 1 - SBA_TRUNC_FILL_PCT / 100 of the
+// tiles out; fuller products keep the full inverse (its register-chunk pass
+// is faster than the truncated one's radix-2 sweeps).
+// Defaults measured on MI355X (single products, 2^16 .. 2^23 at fills
+// 0.51 .. 0.95 of the power-of-two length): truncation beats the full length
+// by 4-6% at fills in (3/4, 0.85] from 2^21 on, breaks even near 0.85 and
+// loses below 2^21 (its extra barriers and spine steps); below 3/4 the
+// radix-3 length 3/4 wins, so truncation never replaces it by default.
+#ifndef SBA_TRUNC_FILL_PCT
+#define SBA_TRUNC_FILL_PCT 85
+#endif
+// products that fill at most this share (percent) of their power-of-two
+// length run truncated at that length rather than at the radix-3 length
+#ifndef SBA_TRUNC_R3_MIN_PCT
+#define SBA_TRUNC_R3_MIN_PCT 0
+#endif
+#ifndef SBA_TRUNC_MIN_LG
+#define SBA_TRUNC_MIN_LG 21
+#endif
+static std::atomic<int> g_trunc_fill{SBA_TRUNC_FILL_PCT};
+static std::atomic<int> g_trunc_r3{SBA_TRUNC_R3_MIN_PCT};
+static std::atomic<int> g_trunc_minlg{SBA_TRUNC_MIN_LG};
+static bool use_trunc(int L, const Plan &pl, uint32_t zmin) {
+  const int fill = g_trunc_fill.load(std::memory_order_relaxed);
+  if (fill <= 0 || pl.last >= L || zmin == 0 || L < g_trunc_minlg.load(std::memory_order_relaxed))
+    return false;
+  const uint64_t R = (uint64_t)1 << (L - pl.last);
+  const uint64_t M = ((uint64_t)zmin + ((uint64_t)1 << pl.last) - 1) >> pl.last;
+  return M * 100 <= R * (uint64_t)fill;
+}
+extern "C" void sba_set_trunc(int max_fill_pct, int r3_min_pct, int min_lg) {
+  g_trunc_fill.store(max_fill_pct, std::memory_order_relaxed);
+  g_trunc_r3.store(r3_min_pct, std::memory_order_relaxed);
+  g_trunc_minlg.store(min_lg, std::memory_order_relaxed);
+}
+extern "C" void sba_get_trunc(int *max_fill_pct, int *r3_min_pct, int *min_lg) {
+  if (max_fill_pct) *max_fill_pct = g_trunc_fill.load(std::memory_order_relaxed);
+  if (r3_min_pct) *r3_min_pct = g_trunc_r3.load(std::memory_order_relaxed);
+  if (min_lg) *min_lg = g_trunc_minlg.load(std::memory_order_relaxed);
+}
+
 // Enqueue the whole multiply for `njobs` jobs sharing N = 2^L (jr.arr ==
 // nullptr: the single job jr.one) under plan pl. nbmax = the largest
-// carry-block count; zsum / asum / bsum feed the profiler's byte counts.
+// carry-block count; zmin = the smallest product (coefficients) of the
+// class; zsum / asum / bsum feed the profiler's byte counts.
 static int enqueue_class(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef &jr,
-                         uint32_t njobs, uint32_t nbmax, double asum, double bsum, double zsum,
-                         hipStream_t st) {
+                         uint32_t njobs, uint32_t nbmax, uint32_t zmin, double asum, double bsum,
+                         double zsum, hipStream_t st) {
   if (!plan_ok(L, pl)) return SBA_BAND;
   const Launchers k = launchers(L, sqr, pl);
   const uint32_t N = 1u << L;
@@ -856,19 +903,27 @@ static int enqueue_class(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef
       PROF("first_fwd", 8.0 * bsum + KN4);
       hipLaunchKernelGGL(k.first, dim3(ntile1, gy1, njobs), dim3(NT), 0, st, pa, jr, 1, L);
     }
+    const bool trunc = use_trunc(L, pl, zmin);
+    const int mode = trunc ? MODE_TRUNC : 0;
+    // bytes of a truncated pass: the transformed tiles of the average job
+    const double tf = trunc ? std::min(1.0, (zsum / njobs + (1u << pl.last)) / N) : 1.0;
     if (!sqr && SBA_MERGE_LAST) {
-      PROF("fused_last", 3 * KN4);
-      hipLaunchKernelGGL(k.fused2, dim3(ntile2, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L, 0);
+      PROF("fused_last", 3 * KN4 * tf);
+      hipLaunchKernelGGL(k.fused2, dim3(ntile2, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L, mode);
     } else {
       if (!sqr) {
         PROF("fwd_last", 2 * KN4);
         hipLaunchKernelGGL(k.fwd_last, dim3(ntile2, SBA_K, njobs), dim3(NT), 0, st, pa, jr, 0, L,
                            ell1, kNoAcc);
       }
-      PROF("fused_last", (sqr ? 2 : 3) * KN4);
-      hipLaunchKernelGGL(k.fused, dim3(ntile2, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L, 0);
+      PROF("fused_last", (sqr ? 2 : 3) * KN4 * tf);
+      hipLaunchKernelGGL(k.fused, dim3(ntile2, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L, mode);
     }
-    {
+    if (trunc) {
+      PROF("inv_trunc", 2 * KN4 * tf);
+      hipLaunchKernelGGL(sba_trunc_inv(ell1), dim3(ntile1, SBA_K, njobs), dim3(NT), 0, st, pa, jr,
+                         L);
+    } else {
       PROF("inv_first", 2 * KN4);
       hipLaunchKernelGGL(k.inv_first, dim3(ntile1, SBA_K, njobs), dim3(NT), 0, st, pa, jr, 1, L,
                          0, kNoAcc);
@@ -1061,15 +1116,34 @@ static int enqueue_chunked(DevCtx &c, const ChunkChoice &cp, uint64_t *rp, const
 // 3 * 2^L instead of 2^(L+2) (a quarter fewer residues in every pass); the
 // column DFT-3 rides the first forward and the last inverse pass.
 // ---------------------------------------------------------------------------
+// row lengths of the radix-3 classes: 2^13 .. 2^22 (first-pass tiles keep
+// >= 4 columns of 2^11 per row) in batches; single products from
+// SBA_R3_SINGLE_MIN_L (measured on MI355X, one product at a time: at row
+// 2^14 the radix-3 product is 37% slower than the power-of-two one of twice
+// the row length -- too few blocks to hide its extra launch -- at 2^16 they
+// are even, at 2^18 radix-3 wins by 15%)
 #define R3_MIN_L 13
-#define R3_MAX_L 22          // first-pass tiles keep >= 4 columns (2^11 per row)
+#define R3_MAX_L 22
+#ifndef SBA_R3_SINGLE_MIN_L
+#define SBA_R3_SINGLE_MIN_L 16
+#endif
+static std::atomic<int> g_r3_single_min{SBA_R3_SINGLE_MIN_L};
+extern "C" void sba_set_r3_min_lg(int L) { g_r3_single_min.store(L, std::memory_order_relaxed); }
+extern "C" int sba_get_r3_min_lg(void) { return g_r3_single_min.load(std::memory_order_relaxed); }
 
-// L of the radix-3 class of zlen coefficients, 0 if none
-static int r3_lg(uint64_t zlen) {
+// L of the radix-3 class of zlen coefficients, 0 if none (single: a product
+// on its own, else a batch job)
+static int r3_lg(uint64_t zlen, bool single) {
   if (!SBA_MERGE_LAST) return 0;
   const int L2 = lg_for(zlen);
   const int L = L2 - 2;
-  if (L < R3_MIN_L || L > R3_MAX_L) return 0;
+  const int lmin = single ? std::max(R3_MIN_L, g_r3_single_min.load(std::memory_order_relaxed))
+                          : R3_MIN_L;
+  if (L < lmin || L > R3_MAX_L) return 0;
+  if (g_trunc_fill.load(std::memory_order_relaxed) > 0 &&
+      L2 >= g_trunc_minlg.load(std::memory_order_relaxed) &&
+      zlen * 100 <= ((uint64_t)g_trunc_r3.load(std::memory_order_relaxed) << L2))
+    return 0;
   return zlen <= (uint64_t)3 << L ? L : 0;
 }
 
@@ -1172,7 +1246,7 @@ extern "C" size_t sba_gpu_workspace_bytes(int64_t an, int64_t bn) {
   const uint64_t mx = (uint64_t)(an > bn ? an : bn), mn = (uint64_t)(an < bn ? an : bn);
   const ChunkChoice cp = chunk_choice(mx, mn, 0);
   if (cp.n) return ws_chunked(cp.Lc, mx + mn);
-  const int L3 = r3_lg(mx + mn - 1);
+  const int L3 = r3_lg(mx + mn - 1, true);
   if (L3) return ws_need((size_t)3 << L3);
   return ws_need((size_t)1 << lg_for(mx + mn - 1));
 }
@@ -1181,7 +1255,22 @@ extern "C" int sba_gpu_radix3(int64_t an, int64_t bn, int sqr) {
   if (!in_band(an, bn)) return 0;
   const uint64_t mx = (uint64_t)(an > bn ? an : bn), mn = (uint64_t)(an < bn ? an : bn);
   if (chunk_choice(mx, mn, sqr).n) return 0;
-  return r3_lg(mx + mn - 1);
+  return r3_lg(mx + mn - 1, true);
+}
+
+extern "C" int sba_gpu_truncated(int64_t an, int64_t bn, int sqr, int *L, int *last, int *rows) {
+  if (!in_band(an, bn)) return 0;
+  const uint64_t mx = (uint64_t)(an > bn ? an : bn), mn = (uint64_t)(an < bn ? an : bn);
+  if (school_job(mx, mn, true) || chunk_choice(mx, mn, sqr).n || r3_lg(mx + mn - 1, true))
+    return 0;
+  const uint64_t zlen = mx + mn - 1;
+  const int Lg = lg_for(zlen);
+  const Plan pl = plan_for(Lg, sqr, 1);
+  if (!use_trunc(Lg, pl, (uint32_t)zlen)) return 0;
+  if (L) *L = Lg;
+  if (last) *last = pl.last;
+  if (rows) *rows = (int)((zlen + ((uint64_t)1 << pl.last) - 1) >> pl.last);
+  return 1;
 }
 
 static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64_t *bp,
@@ -1214,7 +1303,7 @@ static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64
     return SBA_OK;
   }
   const ChunkChoice cp = chunk_choice((uint64_t)an, (uint64_t)bn, sqr);
-  const int L3 = cp.n ? 0 : r3_lg((uint64_t)an + (uint64_t)bn - 1);
+  const int L3 = cp.n ? 0 : r3_lg((uint64_t)an + (uint64_t)bn - 1, true);
   const int L = cp.n ? cp.Lc : (L3 ? L3 : lg_for((uint64_t)an + (uint64_t)bn - 1));
   const size_t Ntot = L3 ? (size_t)3 << L3 : (size_t)1 << L;
   hipStream_t st = (hipStream_t)stream;
@@ -1254,8 +1343,8 @@ static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64
   if (L3)
     return enqueue_r3(*c, L3, sqr, r3_plan(L3, sqr, 1), jr, 1, j.nb, (double)an, (double)bn,
                       (double)j.Z, st);
-  return enqueue_class(*c, L, sqr, plan_for(L, sqr, 1), jr, 1, j.nb, (double)an, (double)bn,
-                       (double)j.Z, st);
+  return enqueue_class(*c, L, sqr, plan_for(L, sqr, 1), jr, 1, j.nb, j.zlen, (double)an,
+                       (double)bn, (double)j.Z, st);
 }
 
 extern "C" int sba_gpu_mul_dev(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64_t *bp,
@@ -1380,7 +1469,7 @@ extern "C" int sba_gpu_mul_dev_noalloc(uint64_t *rp, const uint64_t *ap, int64_t
     nc.pa.wni[q] = tw + (2 * q + 1) * NA_TW_COUNT;
   }
   // always one transform (power of two or 3 * 2^L), never chunked
-  const int L3 = r3_lg(zlen);
+  const int L3 = r3_lg(zlen, true);
   const int L = L3 ? L3 : lg_for(zlen);
   const size_t Ntot = L3 ? (size_t)3 << L3 : (size_t)1 << L;
   JobRef jr;
@@ -1399,8 +1488,8 @@ extern "C" int sba_gpu_mul_dev_noalloc(uint64_t *rp, const uint64_t *ap, int64_t
   if (L3)
     return enqueue_r3(nc, L3, sqr, r3_plan(L3, sqr, 1), jr, 1, j.nb, (double)an, (double)bn,
                       (double)j.Z, st);
-  return enqueue_class(nc, L, sqr, plan_for(L, sqr, 1), jr, 1, j.nb, (double)an, (double)bn,
-                       (double)j.Z, st);
+  return enqueue_class(nc, L, sqr, plan_for(L, sqr, 1), jr, 1, j.nb, j.zlen, (double)an,
+                       (double)bn, (double)j.Z, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1410,7 +1499,7 @@ extern "C" int sba_gpu_mul_dev_noalloc(uint64_t *rp, const uint64_t *ap, int64_t
 // ---------------------------------------------------------------------------
 struct ClassDesc {
   int L, sqr, start;
-  uint32_t n, nbmax;
+  uint32_t n, nbmax, zmin;
   double as, bs, zs;
   Plan plan;
   int r3;                          // radix-3 class: length 3 * 2^L
@@ -1421,7 +1510,7 @@ struct ClassDesc {
 // class of a batch job: (L, radix-3?) of its transform (never chunked)
 static void job_key(const sba_job &j, int *L, int *r3) {
   const uint64_t an = (uint64_t)j.an, bn = j.bp ? (uint64_t)j.bn : an;
-  const int L3 = r3_lg(an + bn - 1);
+  const int L3 = r3_lg(an + bn - 1, false);
   *r3 = L3 != 0;
   *L = L3 ? L3 : lg_for(an + bn - 1);
 }
@@ -1447,7 +1536,7 @@ static int classify_jobs(const sba_job *jobs, int njobs, char *ws, Job *hj, Clas
   size_t off = 0;
   int count[2][2][SBA_MAX_LG + 1] = {};
   if (school) {
-    ClassDesc k = {0, 0, pos, 0, 0, 0, 0, 0, Plan{0, 0, 0}, 0, 1};
+    ClassDesc k = {0, 0, pos, 0, 0, 0, 0, 0, 0, Plan{0, 0, 0}, 0, 1};
     for (int i = 0; i < njobs; ++i) {
       if (!job_school(jobs[i])) continue;
       const int isq = jobs[i].bp == nullptr;
@@ -1478,7 +1567,7 @@ static int classify_jobs(const sba_job *jobs, int njobs, char *ws, Job *hj, Clas
     for (int r3 = 0; r3 < 2; ++r3)
     for (int L = 0; L <= SBA_MAX_LG; ++L) {
       if (!count[sq][r3][L]) continue;
-      ClassDesc k = {L, sq, pos, 0, 0, 0, 0, 0, heuristic_plan(L), r3, 0};
+      ClassDesc k = {L, sq, pos, 0, 0, ~0u, 0, 0, 0, heuristic_plan(L), r3, 0};
       for (int i = 0; i < njobs; ++i) {
         if (school && job_school(jobs[i])) continue;
         const int isq = jobs[i].bp == nullptr;
@@ -1501,6 +1590,7 @@ static int classify_jobs(const sba_job *jobs, int njobs, char *ws, Job *hj, Clas
         off += job_ws(jobs[i]);
         k.n++;
         if (j.nb > k.nbmax) k.nbmax = j.nb;
+        if (j.zlen < k.zmin) k.zmin = j.zlen;
         k.as += j.an;
         k.bs += j.bn;
         k.zs += j.Z;
@@ -1558,7 +1648,7 @@ static int enqueue_classes(DevCtx &c, Job *djobs, const ClassDesc *cls, int ncls
         cls[k].r3 ? enqueue_r3(c, cls[k].L, cls[k].sqr, cls[k].plan, jr, cls[k].n, cls[k].nbmax,
                                cls[k].as, cls[k].bs, cls[k].zs, st)
                   : enqueue_class(c, cls[k].L, cls[k].sqr, cls[k].plan, jr, cls[k].n,
-                                  cls[k].nbmax, cls[k].as, cls[k].bs, cls[k].zs, st);
+                                  cls[k].nbmax, cls[k].zmin, cls[k].as, cls[k].bs, cls[k].zs, st);
     if (rc != SBA_OK) return rc;
   }
   return SBA_OK;

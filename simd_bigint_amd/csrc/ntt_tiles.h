@@ -18,6 +18,7 @@ struct PrimeArgs {
   // applied by the pass that finishes the inverse transform: Nt^-1 * R^4
   // (Nt the transform length; see the host's prime_args)
   uint32_t scale[SBA_K];
+  uint32_t inv2[SBA_K];                          // 1/2 (Montgomery, canonical)
   const uint32_t *wn[SBA_K];
   const uint32_t *wni[SBA_K];
 };
@@ -172,11 +173,15 @@ template <int ELL, int LOGTC, int T0, int B> struct ChunkRegs {
   typedef uint32_t type[GPT][1 << B];
 };
 
+// GUARD (inverse only, column passes: H = 0): a butterfly runs only when its
+// block of the column transform lies inside rows [0, gM) -- the complete
+// blocks of a truncated inverse; the others leave their values as they are.
 template <int ELL, int LOGTC, int T0, int B, bool INV, int ZL = 0, int IO = CHUNK_LDS,
-          int TE = TW_EARLY>
+          int TE = TW_EARLY, bool GUARD = false>
 __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__restrict__ tw,
                                               uint32_t H, uint32_t p, uint32_t pinv,
-                                              typename ChunkRegs<ELL, LOGTC, T0, B>::type &vv) {
+                                              typename ChunkRegs<ELL, LOGTC, T0, B>::type &vv,
+                                              uint32_t gM = 0) {
   constexpr int TC = 1 << LOGTC;
   constexpr int GPT = ChunkRegs<ELL, LOGTC, T0, B>::GPT;
   using G = ChunkGeom<ELL, LOGTC, T0, B>;
@@ -262,10 +267,12 @@ __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__r
 #pragma unroll
         for (int m2 = 0; m2 < (1 << u); ++m2) {
           const uint32_t w = EARLY ? tv[gi][(1 << u) - 1 + m2] : tw[(hj << u) | (uint32_t)m2];
+          // the block of this sub-node: rows [b, b + 1) << (ELL - T0 - u)
+          const bool run = !GUARD || (((((hb << u) | (uint32_t)m2) + 1u) << (ELL - T0 - u)) <= gM);
 #pragma unroll
           for (int r = 0; r < half; ++r) {
             const int m = (m2 << (B - u)) | r;
-            bfly_inv(v[m], v[m + half], w, p, pinv);
+            if (run) bfly_inv(v[m], v[m + half], w, p, pinv);
           }
         }
       }
@@ -384,6 +391,19 @@ __device__ __forceinline__ void tile_fwd_first(uint32_t *lds, const uint32_t *tw
 }
 
 
+
+// the inverse of a column tile's complete blocks below row gM (GUARD above)
+template <int ELL, int LOGTC, int BMAX, int T0>
+__device__ __forceinline__ void tile_inv_guard(uint32_t *lds, const uint32_t *tw, uint32_t p,
+                                               uint32_t pinv, uint32_t gM) {
+  if constexpr (T0 < ELL) {
+    constexpr int B = ChunkPlan<ELL, LOGTC, BMAX>::pick(T0);
+    tile_inv_guard<ELL, LOGTC, BMAX, T0 + B>(lds, tw, p, pinv, gM);
+    typename ChunkRegs<ELL, LOGTC, T0, B>::type unused;
+    tile_chunk_io<ELL, LOGTC, T0, B, true, 0, CHUNK_LDS, TW_EARLY, true>(lds, tw, 0, p, pinv,
+                                                                       unused, gM);
+  }
+}
 
 template <int ELL, int LOGTC, int BMAX, int T0, int TE = TW_EARLY>
 __device__ __forceinline__ void tile_inv(uint32_t *lds, const uint32_t *tw, uint32_t H,
@@ -686,13 +706,25 @@ __global__ __launch_bounds__(NT, INV ? SBA_INV_WAVES : SBA_WAVES) void k_pass(Pr
 // r3: the class has a radix-3 outer level (length 3 * 2^L): grid z runs over
 // (job, row) and the row's length-2^L transform sits at row * 2^L inside
 // each prime's 3 * 2^L residues (LIMB_IN is never launched with r3).
+// mode: bit 0 (MODE_R3) radix-3 class; bit 1 (MODE_TRUNC) truncated product:
+// only the tiles H < ceil(zlen / 2^ELL) are transformed (the others hold no
+// coefficient of the product; the truncated inverse column pass treats them
+// as zero coefficients)
+#define MODE_R3 1
+#define MODE_TRUNC 2
+__device__ __forceinline__ bool tile_skipped(const Job &jb, int ell, int mode) {
+  return (mode & MODE_TRUNC) && blockIdx.x >= ((jb.zlen + (1u << ell) - 1) >> ell);
+}
+
 template <int ELL, bool LIMB_IN, bool SQR, int BMAX>
-__global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr, int L, int r3) {
+__global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr, int L, int mode) {
   constexpr int NE = 1 << ELL;
   constexpr int E = NE / NT;
   __shared__ uint32_t lds[Pad<0>::words(NE)];
+  const int r3 = mode & MODE_R3;
   const uint32_t z = blockIdx.z;
   const Job jb = get_job(jr, r3 ? z / 3 : z);
+  if (tile_skipped(jb, ELL, mode)) return;            // uniform per block
   const int q = blockIdx.y;
   const uint32_t N = 1u << L;
   const uint32_t pstride = r3 ? 3 * N : N;
@@ -740,15 +772,17 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
 // levels. a's transformed tile is kept in registers (canonical, the
 // Montgomery operand) while b's is transformed in the same LDS.
 template <int ELL, int BMAX>
-__global__ __launch_bounds__(NT, BMAX == 5 ? SBA_FUSED2_WAVES : SBA_WAVES) void k_fused2(PrimeArgs pa, JobRef jr, int L, int r3) {
+__global__ __launch_bounds__(NT, BMAX == 5 ? SBA_FUSED2_WAVES : SBA_WAVES) void k_fused2(PrimeArgs pa, JobRef jr, int L, int mode) {
   constexpr int NE = 1 << ELL;
   constexpr int E = NE / NT;
   __shared__ uint32_t lds[Pad<0>::words(NE)];
   // a's spectrum stays in registers from its transform to the product, so
   // early twiddles need the 32-bit-offset form to fit in 3 waves per SIMD
   constexpr int F2TE = SBA_FUSED2_EARLY ? TW_EARLY32 : TW_LATE;
+  const int r3 = mode & MODE_R3;
   const uint32_t z = blockIdx.z;
   const Job jb = get_job(jr, r3 ? z / 3 : z);
+  if (tile_skipped(jb, ELL, mode)) return;            // uniform per block
   const int q = blockIdx.y;
   const uint32_t N = 1u << L;
   const uint32_t pstride = r3 ? 3 * N : N;
@@ -1006,6 +1040,146 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_inv_first3(PrimeArgs pa, JobR
 }
 
 // ---------------------------------------------------------------------------
+// truncated inverse column pass: the last ELL inverse levels (global levels
+// [0, ELL), the column transform of length R = 2^ELL down each of the tile's
+// TC columns) when only the rows k < M = ceil(zlen / S) were transformed
+// (S = 2^(L - ELL) residues per row; MODE_TRUNC in the merged last pass).
+// Per column, the values of rows k < M are spectral (the column transform's
+// first M outputs) and the coefficients of rows k >= M are known to be zero
+// (they lie past the product). The inverse truncated transform of van der
+// Hoeven, as the oracle's tft_inv states it (tests/p30_tft_model_test.c:
+// 125-181, include/p30_ref.h:556-611), recovers the coefficients of rows
+// k < M, scaled by R like the full inverse. Here it runs in three phases on
+// the LDS tile, each step a radix-2 sweep followed by a barrier:
+//   1. the full inverse of every complete dyadic block [b*bs, (b+1)*bs) with
+//      (b+1)*bs <= M, level by level from the smallest (these are the full
+//      subtrees the recursion inverts before it uses them);
+//   2. down the spine (the nodes that contain row M): a node [o, o+m) with
+//      z = M - o rows known, h = m/2, forward root w of the node, either
+//        z < h  (A) rows i in [o+z, o+h): x_i = (x_i + w x_(i+h)) / 2, go left
+//        z >= h (C) rows i in [o+z-h, o+h): t = x_i - w x_(i+h), x_i += t,
+//                   x_(i+h) = t, go right
+//      until a node is complete or empty;
+//   3. back up the spine: (B) rows i in [o, o+z): x_i = 2 x_i - w x_(i+h);
+//      (D) rows i in [o, o+z-h): the plain inverse butterfly.
+// Values stay lazy in [0, 2p) throughout.
+// ---------------------------------------------------------------------------
+template <int ELL, int LOGTC>
+__global__ __launch_bounds__(NT) void k_inv_trunc(PrimeArgs pa, JobRef jr, int L) {
+  constexpr int TC = 1 << LOGTC;
+  constexpr int R = 1 << ELL;
+  constexpr int NE = R * TC;
+  constexpr int E = NE / NT;
+  static_assert(E >= 1, "tile shape");
+  __shared__ uint32_t lds[Pad<LOGTC>::words(NE)];
+  const Job jb = get_job(jr);
+  const int q = blockIdx.y;
+  const uint32_t N = 1u << L;
+  const uint32_t S = 1u << (L - ELL);
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t col0 = tile << LOGTC;
+  g_u32 *buf = (g_u32 *)jb.bspec + q * N + col0;
+  const uint32_t M0 = (jb.zlen + S - 1) >> (L - ELL);
+  const uint32_t M = M0 < (uint32_t)R ? M0 : (uint32_t)R;
+  const uint32_t p = pa.p[q], pinv = pa.pinv[q];
+  const uint32_t *wn = pa.wn[q], *wni = pa.wni[q];
+  const uint32_t tid = threadIdx.x;
+  // tile element (row k, column c) in LDS; a sweep index u over rows
+  // [r0, r0 + n) of all columns is row r0 + (u >> LOGTC), column u & (TC - 1)
+  auto at = [&](uint32_t k, uint32_t c) -> uint32_t & {
+    return lds[lpadt<LOGTC>((k << LOGTC) | c)];
+  };
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const uint32_t e = tid + (uint32_t)i * NT;
+    const uint32_t k = e >> LOGTC, c = e & (TC - 1);
+    at(k, c) = k < M ? (buf + k * S)[c] : 0u;
+  }
+  __syncthreads();
+  // phase 1: the complete blocks, every inverse level (register chunks as
+  // in the full inverse column pass, each butterfly guarded)
+  tile_inv_guard<ELL, LOGTC, 5, 0>(lds, wni, p, pinv, M);
+  __syncthreads();
+  if (M < (uint32_t)R) {
+    // phase 2: down the spine (dec bit d: the node at depth d went right)
+    uint32_t dec = 0;
+    int nd = 0;
+    uint32_t o = 0, m = R;
+    const uint32_t inv2 = pa.inv2[q];
+#pragma unroll 1
+    for (int d = 0; d < ELL; ++d) {
+      const uint32_t zz = M - o;
+      if (zz == 0 || zz >= m) break;
+      const uint32_t h = m >> 1;
+      const uint32_t w = tw_at(wn, o >> (ELL - d));
+      if (zz < h) {
+        const uint32_t r0 = o + zz, cnt = (h - zz) << LOGTC;
+        for (uint32_t u = tid; u < cnt; u += NT) {
+          const uint32_t r = r0 + (u >> LOGTC), c = u & (TC - 1);
+          const uint32_t x = red1(at(r, c), p), y = red1(mont_mul(at(r + h, c), w, p, pinv), p);
+          at(r, c) = mont_mul(x + y, inv2, p, pinv);
+        }
+      } else {
+        const uint32_t zr = zz - h;
+        const uint32_t r0 = o + zr, cnt = (h - zr) << LOGTC;
+        for (uint32_t u = tid; u < cnt; u += NT) {
+          const uint32_t r = r0 + (u >> LOGTC), c = u & (TC - 1);
+          const uint32_t x = red1(at(r, c), p), y = red1(mont_mul(at(r + h, c), w, p, pinv), p);
+          const uint32_t tt = x + p - y;            // (0, 2p)
+          at(r, c) = x + red1(tt, p);
+          at(r + h, c) = tt;
+        }
+        dec |= 1u << d;
+        o += h;
+      }
+      m = h;
+      nd = d + 1;
+      __syncthreads();
+    }
+    // phase 3: back up the spine
+#pragma unroll 1
+    for (int d = nd - 1; d >= 0; --d) {
+      uint32_t oo = 0;                             // the node's first row: the right turns above it
+      for (int e = 0; e < d; ++e)
+        if ((dec >> e) & 1u) oo += (uint32_t)R >> (e + 1);
+      const uint32_t h = (uint32_t)R >> (d + 1);
+      const uint32_t zz = M - oo;
+      const uint32_t j = oo >> (ELL - d);
+      if (!((dec >> d) & 1u)) {
+        const uint32_t w = tw_at(wn, j);
+        const uint32_t cnt = zz << LOGTC;
+        for (uint32_t u = tid; u < cnt; u += NT) {
+          const uint32_t r = oo + (u >> LOGTC), c = u & (TC - 1);
+          const uint32_t x = red1(at(r, c), p);
+          const uint32_t x2 = red1(x + x, p);
+          const uint32_t y = red1(mont_mul(at(r + h, c), w, p, pinv), p);
+          at(r, c) = x2 + p - y;
+        }
+      } else {
+        const uint32_t wi = tw_at(wni, j);
+        const uint32_t cnt = (zz - h) << LOGTC;
+        for (uint32_t u = tid; u < cnt; u += NT) {
+          const uint32_t r = oo + (u >> LOGTC), c = u & (TC - 1);
+          uint32_t x = at(r, c), y = at(r + h, c);
+          bfly_inv(x, y, wi, p, pinv);
+          at(r, c) = x;
+          at(r + h, c) = y;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // the column pass finishes the inverse: scale and store the rows k < M
+  const uint32_t sc = pa.scale[q];
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const uint32_t e = tid + (uint32_t)i * NT;
+    const uint32_t k = e >> LOGTC, c = e & (TC - 1);
+    if (k < M) (buf + k * S)[c] = mont_mul(at(k, c), sc, p, pinv);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // launch tables
 // ---------------------------------------------------------------------------
 typedef void (*first_fn)(PrimeArgs, JobRef, int, int);
@@ -1100,6 +1274,9 @@ template <int BM> static Launchers launchers_bm(int L, int sqr, int last) {
 typedef void (*inv3_fn)(PrimeArgs, JobRef, int);
 first_fn sba_r3_first(int ell);
 inv3_fn sba_r3_inv(int ell);
+// the truncated inverse column pass of ELL = L - last levels (column tiles of
+// 2^SBA_FIRST_TILE_LG residues; defined in the radix-32 object)
+inv3_fn sba_trunc_inv(int ell);
 // defined in ntt_pass_bm.hip, one object per radix
 Launchers sba_launchers_bm3(int L, int sqr, int last);
 Launchers sba_launchers_bm4(int L, int sqr, int last);

@@ -36,7 +36,7 @@ import numpy as np
 from . import SbaJob, SBA_OK, lib
 
 _ALIGN = 256
-POLL_S = 50e-6                      # sleep quantum while launches are in flight
+POLL_S = 20e-6                      # sleep quantum while launches are in flight
 
 
 @dataclass

@@ -1,5 +1,5 @@
 """GPU: radix-3 lengths. A product with 2^(L+1) < an+bn-1 <= 3 * 2^L runs at
-length 3 * 2^L (Good-Thomas: a length-3 DFT down the columns of a 3 x 2^L
+length 3 * 2^L (single products from L = 16 by default, every class here) (Good-Thomas: a length-3 DFT down the columns of a 3 x 2^L
 array, riding the first forward and last inverse passes); every shape at
 the edges of each such class is bit-exact against the oracle, for
 multiplies and squares, and the balanced 3 * 2^20-limb benchmark pair
@@ -14,6 +14,16 @@ from simd_bigint_amd.workloads import all_ones_product, xorshift_fill
 from tests import _oracle as orc
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def all_radix3_classes():
+    """single products take radix-3 classes from row length 2^16 by default;
+    these tests reach every class from 2^13"""
+    old = sba.get_r3_min_lg()
+    sba.set_r3_min_lg(13)
+    yield
+    sba.set_r3_min_lg(old)
 
 
 def say(*a):
