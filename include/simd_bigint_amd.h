@@ -160,6 +160,14 @@ int sba_gpu_chunking(int64_t an, int64_t bn, int sqr, int *Lc, uint64_t *chunk,
  * (2^(L+1) < an+bn-1 <= 3 * 2^L, sba_get_r3_min_lg() <= L <= 22, not
  * chunked), else 0. Batch jobs take radix-3 classes from L = 13. */
 int sba_gpu_radix3(int64_t an, int64_t bn, int sqr);
+/* the schoolbook route of a product: 1 the one-workgroup schoolbook (an +
+ * bn <= 2048 limbs, see sba_set_school_max), 2 the short x long schoolbook
+ * (the shorter operand <= 512 limbs and an * bn <= ratio / 16 * N log2 N,
+ * N the NTT length the product would get; sba_set_school_long_ratio, 0 turns
+ * it off, default measured on MI355X), 0 the NTT */
+int sba_gpu_school(int64_t an, int64_t bn, int sqr);
+void sba_set_school_long_ratio(uint32_t sixteenths);
+uint32_t sba_get_school_long_ratio(void);
 /* smallest radix-3 row length 2^L for single products (default 16, measured
  * on MI355X; batches always from 13) */
 void sba_set_r3_min_lg(int L);

@@ -191,6 +191,12 @@ def lib() -> ctypes.CDLL:
     L.sba_tune_gpu_threshold.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_double)]
     L.sba_tune_gpu_threshold.restype = u64
+    L.sba_gpu_school.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int]
+    L.sba_gpu_school.restype = ctypes.c_int
+    L.sba_set_school_long_ratio.argtypes = [ctypes.c_uint32]
+    L.sba_set_school_long_ratio.restype = None
+    L.sba_get_school_long_ratio.argtypes = []
+    L.sba_get_school_long_ratio.restype = ctypes.c_uint32
     L.sba_set_r3_min_lg.argtypes = [ctypes.c_int]
     L.sba_set_r3_min_lg.restype = None
     L.sba_get_r3_min_lg.argtypes = []
@@ -233,6 +239,21 @@ def get_trunc():
     a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     lib().sba_get_trunc(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
     return a.value, b.value, c.value
+
+
+def school_route(an: int, bn: int, sqr: bool = False) -> int:
+    """1: one-workgroup schoolbook, 2: short x long schoolbook, 0: NTT."""
+    return int(lib().sba_gpu_school(an, bn, 1 if sqr else 0))
+
+
+def set_school_long_ratio(sixteenths: int) -> None:
+    """Short x long schoolbook when an * bn <= sixteenths / 16 * N log2 N;
+    0 turns it off."""
+    lib().sba_set_school_long_ratio(int(sixteenths))
+
+
+def get_school_long_ratio() -> int:
+    return int(lib().sba_get_school_long_ratio())
 
 
 def set_r3_min_lg(L: int) -> None:

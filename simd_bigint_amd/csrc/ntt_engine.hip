@@ -289,6 +289,110 @@ static bool school_job(uint64_t an, uint64_t bn, bool single) {
   return an * bn <= lim;
 }
 
+// ---------------------------------------------------------------------------
+// short x long products: schoolbook with the short operand b (bn <=
+// SCHOOL_SHORT_MAX limbs) against a long a, one workgroup per block of
+// CARRY_LPB product limbs (grid x = block, z = job). A workgroup keeps b and
+// the window of a its columns read in LDS; each thread sums 4 consecutive
+// columns, register-blocked: per b limb one new a limb enters a 4-limb
+// sliding window and 4 multiply-adds follow (about 2 LDS reads per 4
+// products). Column sums are the Garner stage's 3-word coefficients (<
+// 2^128 bn), so the same limb finishing (block carry-in 0, 2-state block
+// map) and the carry scan between blocks complete the product. Used where
+// an * bn is small against the NTT's N log N (very uneven shapes).
+// ---------------------------------------------------------------------------
+#define SCHOOL_SHORT_MAX 512
+#define SCHOOL_LONG_W (CARRY_LPB + 3 + SCHOOL_SHORT_MAX)
+
+__device__ __forceinline__ void mac128(uint64_t &lo, uint64_t &mid, uint64_t &hi, uint64_t x,
+                                       uint64_t y) {
+  const uint64_t pl = x * y, ph = __umul64hi(x, y);
+  lo += pl;
+  const uint64_t c1 = lo < pl ? 1u : 0u;
+  const uint64_t m1 = mid + ph;
+  hi += m1 < mid ? 1u : 0u;
+  mid = m1 + c1;
+  hi += mid < c1 ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(NT) void k_school_long(JobRef jr) {
+  __shared__ uint64_t sb[SCHOOL_SHORT_MAX];
+  __shared__ uint64_t sa[SCHOOL_LONG_W];       // a[abase + x], zero outside [0, an)
+  __shared__ uint64_t ws0[CARRY_LPB + 3], ws1[CARRY_LPB + 3];
+  __shared__ uint32_t ws2[CARRY_LPB + 3];
+  const Job jb = get_job(jr);
+  if (blockIdx.x >= jb.nb) return;               // this job has fewer blocks
+  const uint32_t an = jb.an, bn = jb.bn, Z = jb.Z;
+  gc_u64 *ga = (gc_u64 *)jb.a;
+  gc_u64 *gb = (gc_u64 *)jb.b;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t B0 = blockIdx.x * CARRY_LPB;
+  // column c = B0 - 3 + x' reads a[c - j], j < bn: window from B0 - 3 - (bn - 1)
+  const int64_t abase = (int64_t)B0 - 3 - (int64_t)(bn - 1);
+  const uint32_t W = CARRY_LPB + 3 + bn - 1;
+  for (uint32_t x = tid; x < W; x += NT) {
+    const int64_t i = abase + (int64_t)x;
+    sa[x] = (i >= 0 && i < (int64_t)an) ? ga[i] : 0;
+  }
+  for (uint32_t x = tid; x < bn; x += NT) sb[x] = gb[x];
+  __syncthreads();
+  {
+    // columns B0 + 4 tid + u (u < 4); a[B0 + 4 tid + u - j] sits at window
+    // index x0 + u - j
+    const uint32_t k0 = 4 * tid;
+    const uint32_t x0 = k0 + 3 + bn - 1;
+    uint64_t lo[4] = {0, 0, 0, 0}, mid[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
+    uint64_t w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) w[u] = sa[x0 + u];
+    for (uint32_t j = 0; j < bn; ++j) {
+      const uint64_t y = sb[j];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) mac128(lo[u], mid[u], hi[u], w[u], y);
+      w[3] = w[2];
+      w[2] = w[1];
+      w[1] = w[0];
+      w[0] = sa[x0 - j - 1];                      // >= index k0 + 2 while j < bn
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      ws0[k0 + u + 3] = lo[u];
+      ws1[k0 + u + 3] = mid[u];
+      ws2[k0 + u + 3] = (uint32_t)hi[u];
+    }
+  }
+  if (tid < 3) {
+    // columns B0 - 3 + tid (zero below column 0: the window holds zeros there)
+    uint64_t lo = 0, mid = 0, hi = 0;
+    for (uint32_t j = 0; j < bn; ++j) mac128(lo, mid, hi, sa[tid + bn - 1 - j], sb[j]);
+    ws0[tid] = lo;
+    ws1[tid] = mid;
+    ws2[tid] = (uint32_t)hi;
+  }
+  __syncthreads();
+  limbs_from_words<CARRY_LPT, uint32_t>(ws0, ws1, ws2, B0, Z, (g_u64 *)jb.r,
+                                        (g_u32 *)jb.bmaps + blockIdx.x);
+}
+
+// short x long products take k_school_long when b is short and its
+// multiply-adds are cheap against the NTT the product would get:
+// an * bn <= g_long_ratio / 16 * N * log2 N (default measured on MI355X).
+#ifndef SBA_SCHOOL_LONG_RATIO
+#define SBA_SCHOOL_LONG_RATIO 48               // sixteenths
+#endif
+static std::atomic<uint32_t> g_long_ratio{SBA_SCHOOL_LONG_RATIO};
+static bool school_long_job(uint64_t an, uint64_t bn) {
+  if (an < bn) { const uint64_t t = an; an = bn; bn = t; }
+  if (bn > SCHOOL_SHORT_MAX || an + bn <= SCHOOL_ZMAX) return false;
+  const uint64_t zlen = an + bn - 1;
+  int L = 10;
+  while (((uint64_t)1 << L) < zlen) ++L;
+  const uint64_t ratio = g_long_ratio.load(std::memory_order_relaxed);
+  return an * bn * 16 <= ratio * ((uint64_t)L << L);
+}
+// the workspace of a short x long product: its carry maps
+static size_t ws_school_long(uint64_t Z) { return ((Z / CARRY_LPB + 2) * 4 + 255) & ~(size_t)255; }
+
 // workgroup-wide exclusive scan of 2-state maps in thread order; *total =
 // the composition of all of them
 __device__ uint32_t c2_block_scan(uint32_t f, uint32_t *total) {
@@ -1234,7 +1338,8 @@ extern "C" int sba_gpu_chunking(int64_t an, int64_t bn, int sqr, int *Lc, uint64
                                 uint64_t *nchunks) {
   if (!in_band(an, bn)) return SBA_BAND;
   const uint64_t mx = (uint64_t)(an > bn ? an : bn), mn = (uint64_t)(an < bn ? an : bn);
-  const ChunkChoice cp = chunk_choice(mx, mn, sqr);
+  const ChunkChoice cp = (!sqr && school_long_job(mx, mn)) ? ChunkChoice{0, 0, 0}
+                                                          : chunk_choice(mx, mn, sqr);
   *Lc = cp.Lc;
   *chunk = cp.C;
   *nchunks = cp.n;
@@ -1244,6 +1349,7 @@ extern "C" int sba_gpu_chunking(int64_t an, int64_t bn, int sqr, int *Lc, uint64
 extern "C" size_t sba_gpu_workspace_bytes(int64_t an, int64_t bn) {
   if (!in_band(an, bn)) return 0;
   const uint64_t mx = (uint64_t)(an > bn ? an : bn), mn = (uint64_t)(an < bn ? an : bn);
+  if (school_long_job(mx, mn)) return ws_school_long(mx + mn);
   const ChunkChoice cp = chunk_choice(mx, mn, 0);
   if (cp.n) return ws_chunked(cp.Lc, mx + mn);
   const int L3 = r3_lg(mx + mn - 1, true);
@@ -1254,14 +1360,28 @@ extern "C" size_t sba_gpu_workspace_bytes(int64_t an, int64_t bn) {
 extern "C" int sba_gpu_radix3(int64_t an, int64_t bn, int sqr) {
   if (!in_band(an, bn)) return 0;
   const uint64_t mx = (uint64_t)(an > bn ? an : bn), mn = (uint64_t)(an < bn ? an : bn);
-  if (chunk_choice(mx, mn, sqr).n) return 0;
+  if ((!sqr && school_long_job(mx, mn)) || chunk_choice(mx, mn, sqr).n) return 0;
   return r3_lg(mx + mn - 1, true);
+}
+
+extern "C" int sba_gpu_school(int64_t an, int64_t bn, int sqr) {
+  if (!in_band(an, bn)) return 0;
+  const uint64_t mx = (uint64_t)(an > bn ? an : bn), mn = (uint64_t)(an < bn ? an : bn);
+  if (school_job(mx, mn, true)) return 1;
+  return (!sqr && school_long_job(mx, mn)) ? 2 : 0;
+}
+extern "C" void sba_set_school_long_ratio(uint32_t sixteenths) {
+  g_long_ratio.store(sixteenths, std::memory_order_relaxed);
+}
+extern "C" uint32_t sba_get_school_long_ratio(void) {
+  return g_long_ratio.load(std::memory_order_relaxed);
 }
 
 extern "C" int sba_gpu_truncated(int64_t an, int64_t bn, int sqr, int *L, int *last, int *rows) {
   if (!in_band(an, bn)) return 0;
   const uint64_t mx = (uint64_t)(an > bn ? an : bn), mn = (uint64_t)(an < bn ? an : bn);
-  if (school_job(mx, mn, true) || chunk_choice(mx, mn, sqr).n || r3_lg(mx + mn - 1, true))
+  if (school_job(mx, mn, true) || (!sqr && school_long_job(mx, mn)) ||
+      chunk_choice(mx, mn, sqr).n || r3_lg(mx + mn - 1, true))
     return 0;
   const uint64_t zlen = mx + mn - 1;
   const int Lg = lg_for(zlen);
@@ -1271,6 +1391,36 @@ extern "C" int sba_gpu_truncated(int64_t an, int64_t bn, int sqr, int *L, int *l
   if (last) *last = pl.last;
   if (rows) *rows = (int)((zlen + ((uint64_t)1 << pl.last) - 1) >> pl.last);
   return 1;
+}
+
+// one short x long product (an >= bn), carry maps in bmaps
+static int enqueue_school_long_one(uint64_t *rp, const uint64_t *ap, uint64_t an,
+                                   const uint64_t *bp, uint64_t bn, uint32_t *bmaps,
+                                   hipStream_t st) {
+  JobRef jr;
+  jr.arr = nullptr;
+  Job &j = jr.one;
+  memset(&j, 0, sizeof j);
+  j.a = ap;
+  j.b = bp;
+  j.r = rp;
+  j.an = (uint32_t)an;
+  j.bn = (uint32_t)bn;
+  j.zlen = (uint32_t)(an + bn - 1);
+  j.Z = (uint32_t)(an + bn);
+  j.nb = carry_blocks(j.Z);
+  j.bmaps = bmaps;
+  {
+    PROF("school_long", 8.0 * (an + bn) + 8.0 * j.Z);
+    hipLaunchKernelGGL(k_school_long, dim3(j.nb, 1, 1), dim3(NT), 0, st, jr);
+  }
+  {
+    PROF("carry_blocks", 8.0 * j.Z / CARRY_LPB);
+    const uint32_t g = j.nb < CARRY_WGS ? j.nb : CARRY_WGS;
+    hipLaunchKernelGGL(k_carry_blocks, dim3(g, 1, 1), dim3(SCAN_NT), 0, st, jr);
+  }
+  HIPCHK(hipGetLastError());
+  return SBA_OK;
 }
 
 static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64_t *bp,
@@ -1301,6 +1451,28 @@ static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64
     }
     HIPCHK(hipGetLastError());
     return SBA_OK;
+  }
+  if (!sqr && school_long_job((uint64_t)an, (uint64_t)bn)) {
+    hipStream_t st = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lk(g_mu);
+    const uint64_t Z = (uint64_t)an + (uint64_t)bn;
+    const size_t need = ws_school_long(Z);
+    if (!ws) {
+      DevCtx *c;
+      int rc = get_ctx(&c);
+      if (rc) return rc;
+      if (c->ws_bytes < need) {
+        if (c->ws) HIPCHK(hipFree(c->ws));
+        c->ws = nullptr;
+        c->ws_bytes = 0;
+        HIPCHK(hipMalloc(&c->ws, need));
+        c->ws_bytes = need;
+      }
+      ws = c->ws;
+    } else if (ws_bytes < need) {
+      return SBA_ERR_WORKSPACE;
+    }
+    return enqueue_school_long_one(rp, ap, (uint64_t)an, bp, (uint64_t)bn, (uint32_t *)ws, st);
   }
   const ChunkChoice cp = chunk_choice((uint64_t)an, (uint64_t)bn, sqr);
   const int L3 = cp.n ? 0 : r3_lg((uint64_t)an + (uint64_t)bn - 1, true);
@@ -1503,7 +1675,8 @@ struct ClassDesc {
   double as, bs, zs;
   Plan plan;
   int r3;                          // radix-3 class: length 3 * 2^L
-  int school;                      // the schoolbook class (every small product)
+  int school;                      // 1: the schoolbook class (every small product),
+                                   // 2: the short x long schoolbook class
 };
 #define SBA_MAX_CLASSES (4 * (SBA_MAX_LG + 1) + 1)
 
@@ -1525,6 +1698,11 @@ static size_t table_bytes(int njobs) { return ((size_t)njobs * sizeof(Job) + 255
 static bool job_school(const sba_job &j) {
   return school_job((uint64_t)j.an, j.bp ? (uint64_t)j.bn : (uint64_t)j.an, false);
 }
+// a short x long job (multiplies only; never a square, whose operands are equal)
+static bool job_school_long(const sba_job &j) {
+  return j.bp && !job_school(j) && school_long_job((uint64_t)j.an, (uint64_t)j.bn);
+}
+static bool job_off_ntt(const sba_job &j) { return job_school(j) || job_school_long(j); }
 
 // Orders the jobs by class into hj[], carving each job's buffers from ws in
 // that order (job_ws bytes each), and fills cls[] (at most SBA_MAX_CLASSES
@@ -1556,9 +1734,32 @@ static int classify_jobs(const sba_job *jobs, int njobs, char *ws, Job *hj, Clas
       k.zs += j.Z;
     }
     if (k.n) cls[ncls++] = k;
+    // the short x long class: carry maps in the job's workspace slot (job_ws
+    // reserves the NTT's, which is larger)
+    ClassDesc kl = {0, 0, pos, 0, 0, 0, 0, 0, 0, Plan{0, 0, 0}, 0, 2};
+    for (int i = 0; i < njobs; ++i) {
+      if (!job_school_long(jobs[i])) continue;
+      const bool swap = jobs[i].an < jobs[i].bn;
+      Job &j = hj[pos++];
+      memset(&j, 0, sizeof j);
+      j.a = swap ? jobs[i].bp : jobs[i].ap;
+      j.b = swap ? jobs[i].ap : jobs[i].bp;
+      j.an = (uint32_t)(swap ? jobs[i].bn : jobs[i].an);
+      j.bn = (uint32_t)(swap ? jobs[i].an : jobs[i].bn);
+      j.r = jobs[i].rp;
+      j.zlen = j.an + j.bn - 1;
+      j.Z = j.an + j.bn;
+      j.nb = carry_blocks(j.Z);
+      j.bmaps = (uint32_t *)(ws + off);
+      off += job_ws(jobs[i]);
+      kl.n++;
+      if (j.nb > kl.nbmax) kl.nbmax = j.nb;
+      kl.zs += j.Z;
+    }
+    if (kl.n) cls[ncls++] = kl;
   }
   for (int i = 0; i < njobs; ++i) {
-    if (school && job_school(jobs[i])) continue;
+    if (school && job_off_ntt(jobs[i])) continue;
     int L, r3;
     job_key(jobs[i], &L, &r3);
     count[jobs[i].bp == nullptr][r3][L]++;
@@ -1569,7 +1770,7 @@ static int classify_jobs(const sba_job *jobs, int njobs, char *ws, Job *hj, Clas
       if (!count[sq][r3][L]) continue;
       ClassDesc k = {L, sq, pos, 0, 0, ~0u, 0, 0, 0, heuristic_plan(L), r3, 0};
       for (int i = 0; i < njobs; ++i) {
-        if (school && job_school(jobs[i])) continue;
+        if (school && job_off_ntt(jobs[i])) continue;
         const int isq = jobs[i].bp == nullptr;
         const uint64_t an = (uint64_t)jobs[i].an, bn = isq ? an : (uint64_t)jobs[i].bn;
         int jL, jr3;
@@ -1636,10 +1837,23 @@ static int enqueue_classes(DevCtx &c, Job *djobs, const ClassDesc *cls, int ncls
     JobRef jr;
     jr.arr = djobs + cls[k].start;
     memset(&jr.one, 0, sizeof jr.one);
-    if (cls[k].school) {
+    if (cls[k].school == 1) {
       {
         PROF("school", 16.0 * cls[k].zs);
         hipLaunchKernelGGL(k_school, dim3(cls[k].n, 1, 1), dim3(NT), 0, st, jr);
+      }
+      HIPCHK(hipGetLastError());
+      continue;
+    }
+    if (cls[k].school == 2) {
+      {
+        PROF("school_long", 16.0 * cls[k].zs);
+        hipLaunchKernelGGL(k_school_long, dim3(cls[k].nbmax, 1, cls[k].n), dim3(NT), 0, st, jr);
+      }
+      {
+        PROF("carry_blocks", 8.0 * cls[k].zs / CARRY_LPB);
+        const uint32_t g = cls[k].nbmax < CARRY_WGS ? cls[k].nbmax : CARRY_WGS;
+        hipLaunchKernelGGL(k_carry_blocks, dim3(g, 1, cls[k].n), dim3(SCAN_NT), 0, st, jr);
       }
       HIPCHK(hipGetLastError());
       continue;

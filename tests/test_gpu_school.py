@@ -1,5 +1,7 @@
-"""The schoolbook tier for small products (an + bn <= 2048 limbs): single
-calls and batches, checked against Python integers and against the NTT path
+"""The schoolbook tiers: small products (an + bn <= 2048 limbs, one
+workgroup) and short x long products (the short operand <= 512 limbs, one
+workgroup per 1024 product limbs, carries between blocks by the carry scan):
+single calls and batches, checked against Python integers and against the NTT path
 on the same operands (the tier switched off), at the tier's edges: one-limb
 operands, the 2048-limb column limit, all-ones operands (the longest carry
 chains), zeros and sparse limbs, squares."""
@@ -9,7 +11,8 @@ import numpy as np
 import pytest
 
 import simd_bigint_amd as sba
-from simd_bigint_amd.workloads import from_int, to_int, xorshift_fill
+from simd_bigint_amd.workloads import all_ones_product, from_int, to_int, xorshift_fill
+from tests import _oracle as orc
 
 pytestmark = pytest.mark.gpu
 ONES = np.uint64(0xFFFFFFFFFFFFFFFF)
@@ -91,3 +94,80 @@ def test_school_batch_matches_ntt_path(wide_school):
             for k, (tr, _, _) in enumerate(jobs):
                 assert (tr.cpu().numpy().view(np.uint64) == want[k]).all(), (school, rep, k)
         bt.close()
+
+
+@pytest.fixture
+def school_long_everywhere():
+    """route every short x long shape to k_school_long (the default routes
+    only those measured faster than the NTT)"""
+    old = sba.get_school_long_ratio()
+    sba.set_school_long_ratio(1 << 30)
+    yield
+    sba.set_school_long_ratio(old)
+
+
+def test_short_x_long_vs_oracle(school_long_everywhere):
+    """The short x long schoolbook: short operands of 1 .. 512 limbs against
+    long ones over many 1024-limb blocks, block-boundary lengths, both
+    operand orders, random / all-ones / sparse limbs."""
+    rng = np.random.default_rng(77)
+    shapes = [(2049, 1), (2048, 3), (5000, 512), (4096, 511), (1023 * 7 + 2, 129),
+              (65536, 64), (100003, 300), (1 << 18, 17), (3 * 1024 - 2, 2), (20000, 512)]
+    for an, bn in shapes:
+        assert sba.school_route(an, bn) == 2, (an, bn)
+        for pat in range(3):
+            a = rng.integers(0, 2**64, an, dtype=np.uint64)
+            b = rng.integers(0, 2**64, bn, dtype=np.uint64)
+            if pat == 1:
+                a[:] = 2**64 - 1
+                b[:] = 2**64 - 1
+            elif pat == 2:
+                a[rng.random(an) < 0.9] = 0
+                b[rng.random(bn) < 0.5] = 0
+            ok, r = sba.nt_mul(a, b)
+            assert ok == 1 and (r == orc.nt_mul(a, b)).all(), (an, bn, pat)
+            ok, r = sba.nt_mul(b, a)
+            assert ok == 1 and (r == orc.nt_mul(a, b)).all(), (bn, an, pat)
+    a = np.full(70000, 2**64 - 1, np.uint64)
+    b = np.full(500, 2**64 - 1, np.uint64)
+    ok, r = sba.nt_mul(a, b)
+    assert ok == 1 and (r == all_ones_product(70000, 500)).all()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_short_x_long_batch_class(school_long_everywhere, use_graph):
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(78)
+    jobs, want = [], []
+    for k in range(60):
+        an = int(rng.integers(2100, 70000))
+        bn = int(rng.integers(1, 513))
+        if k % 2:
+            an, bn = bn, an
+        a = rng.integers(0, 2**64, an, dtype=np.uint64)
+        b = rng.integers(0, 2**64, bn, dtype=np.uint64)
+        if k % 5 == 0:
+            a[:] = 2**64 - 1
+            b[:] = 2**64 - 1
+        jobs.append((torch.zeros(an + bn, dtype=torch.int64, device="cuda"),
+                     torch.from_numpy(a.view(np.int64)).cuda(),
+                     torch.from_numpy(b.view(np.int64)).cuda()))
+        want.append(orc.nt_mul(a, b))
+    # plus a few NTT-sized and small jobs in the same batch
+    for an, bn in ((3000, 3000), (300, 200), (40000, 30000)):
+        a = rng.integers(0, 2**64, an, dtype=np.uint64)
+        b = rng.integers(0, 2**64, bn, dtype=np.uint64)
+        jobs.append((torch.zeros(an + bn, dtype=torch.int64, device="cuda"),
+                     torch.from_numpy(a.view(np.int64)).cuda(),
+                     torch.from_numpy(b.view(np.int64)).cuda()))
+        want.append(orc.nt_mul(a, b))
+    bt = sba.Batch(jobs, use_graph=use_graph)
+    assert bt.classes >= 3
+    for rep in range(2):
+        for r, _, _ in jobs:
+            r.zero_()
+        bt.run()
+        torch.cuda.synchronize()
+        for k, (r, _, _) in enumerate(jobs):
+            assert (r.cpu().numpy().view(np.uint64) == want[k]).all(), (rep, k)
+    bt.close()
