@@ -18,7 +18,10 @@ Also reported, from the same process:
                 issue peak (the transforms are VALU-bound, not HBM-bound);
   cpu_baseline  the reference's nt_mul compiled from source (oracle/_ref,
                 "reference") when present and the host CPU has its ISA, else
-                the oracle's C restatement ("port"), on the same operands.
+                the oracle's C restatement ("port", with a warning), on the
+                same operands; beside it the reference's simd_mpn_mul over
+                the batched workload on every host core, GMP's mpn_mul as a
+                yardstick, and the host's core count and CPU model.
 """
 import argparse
 import ctypes
@@ -55,24 +58,67 @@ def cpu_flags():
     return set()
 
 
-def cpu_baseline(a, b, budget_s):
+def host_info():
+    """The host the CPU legs ran on: nproc, the cores Python sees, lscpu's
+    model name."""
+    info = {"nproc": None, "os_cpu_count": os.cpu_count(), "model": None}
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True,
+                                           timeout=10).stdout.strip())
+    except Exception:
+        pass
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.lower().startswith("model name"):
+                info["model"] = ln.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return info
+
+
+REF_ISA = {"avx512f", "avx512bw", "avx512vl", "avx512dq", "avx512ifma", "avx512vbmi",
+           "avx512_vbmi2", "avx512_vpopcntdq", "avx512_bitalg", "bmi2", "adx"}
+
+
+def reference_lib():
+    """oracle/_ref/libsbref.so (the reference compiled from its own sources)
+    when it was built and this CPU has its ISA; else None (reported)."""
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libsbref.so")
+    if not os.path.exists(ref_so):
+        log("bench: WARNING: %s is absent (the reference build travels only from a build "
+            "container); the CPU baseline falls back to the oracle's C restatement" % ref_so)
+        return None
+    missing = REF_ISA - cpu_flags()
+    if missing:
+        log("bench: WARNING: this CPU lacks %s; the reference build cannot run, the CPU "
+            "baseline falls back to the oracle's C restatement" % sorted(missing))
+        return None
+    return ctypes.CDLL(ref_so)
+
+
+def cpu_baseline(a, b, budget_s, batch_pairs, gpu_batch_muls_per_s):
+    """The reference's own nt_mul on one host core (best of a bounded number
+    of full 4Mi x 4Mi multiplies); its simd_mpn_mul over a sample of the
+    batched workload on every host core (independent multiplies, one per
+    thread at a time); and GMP's mpn_mul (libgmp.so.10 by dlopen, one core)
+    on the same 4Mi x 4Mi operands, as a yardstick. Without the reference
+    build the first two run the oracle's restatement and say so."""
+    import threading
     import numpy as np
     n = len(a)
-    need = {"avx512f", "avx512bw", "avx512vl", "avx512dq", "avx512ifma", "avx512vbmi",
-            "avx512_vbmi2", "avx512_vpopcntdq", "avx512_bitalg", "bmi2", "adx"}
-    ref_so = os.path.join(ROOT, "oracle", "_ref", "libsbref.so")
     u64p = np.ctypeslib.ndpointer(np.uint64, flags="C")
     r = np.zeros(2 * n, np.uint64)
-    if os.path.exists(ref_so) and need <= cpu_flags():
-        L = ctypes.CDLL(ref_so)
-        fn = L.sbref_nt_mul
+    Lr = reference_lib()
+    if Lr is not None:
+        fn = Lr.sbref_nt_mul
         kind, cores = "reference", 1
         what = "reference nt_mul (include/ntmul.h, p50 engine at this size) compiled from source"
     else:
-        L = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
-        fn = L.orc_nt_mul
+        fn = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so")).orc_nt_mul
         kind, cores = "port", os.cpu_count() or 1
-        what = "oracle C restatement of the p30 math (six primes, OpenMP over primes)"
+        what = ("oracle C restatement of the p30 math (six primes, OpenMP over primes); the "
+                "reference build oracle/_ref was not available")
     fn.argtypes = [u64p, u64p, ctypes.c_ssize_t, u64p, ctypes.c_ssize_t]
     fn.restype = ctypes.c_int
     times = []
@@ -87,9 +133,66 @@ def cpu_baseline(a, b, budget_s):
             break
     best = min(times)
     mean = sum(times) / len(times)
-    return {"value": round(best * 1e9 / n, 3), "unit": "ns/limb", "cores": cores, "kind": kind,
-            "sample": "%s; %d full 4Mi x 4Mi multiplies on the same operands, best of %d "
-                      "(mean %.3f ns/limb)" % (what, len(times), len(times), mean * 1e9 / n)}, r
+    out = {"value": round(best * 1e9 / n, 3), "unit": "ns/limb", "cores": cores, "kind": kind,
+           "sample": "%s; %d full 4Mi x 4Mi multiplies on the same operands, best of %d "
+                     "(mean %.3f ns/limb)" % (what, len(times), len(times), mean * 1e9 / n),
+           "host": host_info()}
+    # all host cores on independent multiplies of the batched workload
+    if Lr is not None and batch_pairs:
+        mf = Lr.sbref_simd_mpn_mul
+        mf.argtypes = [u64p, u64p, ctypes.c_uint64, u64p, ctypes.c_uint64]
+        mf.restype = None
+        nth = os.cpu_count() or 1
+        res = [np.zeros(len(x) + len(y), np.uint64) for x, y in batch_pairs]
+        x0, y0 = batch_pairs[0]
+        mf(res[0], x0, len(x0), y0, len(y0))           # lazily built tables, one thread
+        nxt = [0]
+        lock = threading.Lock()
+
+        def worker():
+            while True:
+                with lock:
+                    k = nxt[0]
+                    nxt[0] += 1
+                if k >= len(batch_pairs):
+                    return
+                x, y = batch_pairs[k]
+                mf(res[k], x, len(x), y, len(y))      # ctypes releases the GIL
+
+        t0 = time.perf_counter()
+        th = [threading.Thread(target=worker) for _ in range(nth)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        dt = time.perf_counter() - t0
+        out["all_cores_batch"] = {
+            "muls_per_s": round(len(batch_pairs) / dt, 1), "threads": nth,
+            "pairs": len(batch_pairs), "seconds": round(dt, 3),
+            "gpu_batched_muls_per_s": gpu_batch_muls_per_s,
+            "sample": "the reference's simd_mpn_mul (include/u64cvt.h:94-132) on the first "
+                      "%d pairs of the batched workload (same seed), one multiply per thread at "
+                      "a time on %d threads" % (len(batch_pairs), nth)}
+    # GMP as a yardstick (not the reference)
+    try:
+        g = ctypes.CDLL("libgmp.so.10")
+        gm = g.__gmpn_mul
+        gm.argtypes = [u64p, u64p, ctypes.c_long, u64p, ctypes.c_long]
+        gm.restype = ctypes.c_uint64
+        gt = []
+        rg = np.zeros(2 * n, np.uint64)
+        for _ in range(3):
+            t0 = time.perf_counter()
+            gm(rg, a, n, b, n)
+            gt.append(time.perf_counter() - t0)
+        out["gmp"] = {"value": round(min(gt) * 1e9 / n, 3), "unit": "ns/limb", "cores": 1,
+                      "version": ctypes.c_char_p.in_dll(g, "__gmp_version").value.decode(),
+                      "bit_exact": bool((rg == r).all()),
+                      "sample": "GMP mpn_mul (libgmp.so.10 by dlopen) on the same 4Mi x 4Mi "
+                                "operands, best of 3"}
+    except OSError:
+        out["gmp"] = None
+    return out, r
 
 
 def _time_batch(bt, stream, reps=5):
@@ -570,8 +673,14 @@ def main():
     bit_exact = hashlib.sha256(prod.tobytes()).hexdigest() == want
     log("bench: warmup done, bit_exact =", bit_exact)
 
+    # the timed region: K back-to-back multiplies; every kernel is launched
+    # with start / stop events of its own (hipExtLaunchKernelGGL on the launch
+    # stream, no extra packets between kernels), so the step time and the
+    # per-kernel durations come from the same launches
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
+    L.sba_gpu_profile_reset()
+    L.sba_gpu_profile_enable(1)
     torch.cuda.synchronize()
     e0.record(stream)
     for _ in range(args.steps):
@@ -579,17 +688,9 @@ def main():
     e1.record(stream)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / args.steps
-    log("bench: %.4f ms per multiply" % ms)
-
-
-    # per-kernel timing pass (events around each launch on the same stream)
-    L.sba_gpu_profile_reset()
-    L.sba_gpu_profile_enable(1)
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
     L.sba_gpu_profile_collect()
     L.sba_gpu_profile_enable(0)
+    log("bench: %.4f ms per multiply" % ms)
     kernels = {}
     i = 0
     while True:
@@ -655,7 +756,11 @@ def main():
     cpu = None
     if args.cpu_budget > 0:
         log("bench: cpu baseline")
-        cpu, rcpu = cpu_baseline(a, b, args.cpu_budget)
+        from simd_bigint_amd.workloads import batch_workload
+        pairs = batch_workload(args.batch_jobs or 4096, seed=4)
+        cpu, rcpu = cpu_baseline(a, b, args.cpu_budget, pairs,
+                                 batched["batched_muls_per_s"] if batched else None)
+        del pairs
         if not (rcpu == prod).all():
             raise RuntimeError("cpu baseline product differs from the GPU product")
 
@@ -678,6 +783,10 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": dom["achieved_GBs"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dom["achieved_GBs"] / HBM_PEAK_GBS, 4),
+                     "avg_us": dom["avg_us"],
+                     "timing": "the kernel's own start/stop events (hipExtLaunchKernelGGL) on "
+                               "its launch stream, averaged over the %d timed multiplies"
+                               % args.steps,
                      "traffic": profiled_traffic(dom_name),
                      "traffic_source": "profiles/ntt_mul_4Mi_x_4Mi_hbm_traffic.md "
                                        "(2 x FETCH_SIZE + 1 x WRITE_SIZE, calibrated)"},

@@ -28,6 +28,8 @@
 #include "ntt_tiles.h"
 #include "plan_table.h"
 
+#include <hip/hip_ext.h>
+
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -797,24 +799,36 @@ static hipEvent_t prof_event(void) {
   if (!g_prof_pool[i]) (void)hipEventCreate(&g_prof_pool[i]);
   return g_prof_pool[i];
 }
+// A PROF scope names the kernel launched next in it; with profiling on,
+// that launch goes through hipExtLaunchKernelGGL with the record's start and
+// stop events, which time the kernel itself without extra event packets
+// between kernels (back-to-back launches keep their spacing). Launches
+// outside a PROF scope, or with profiling off, are plain launches.
+static int g_prof_pending = -1;                 // record awaiting its launch (under g_mu)
 struct ProfScope {
-  int rec = -1;
-  hipStream_t st;
-  ProfScope(const char *name, double bytes, hipStream_t s) : st(s) {
+  ProfScope(const char *name, double bytes) {
     if (!g_prof_on || g_prof_n >= PROF_MAX_REC) return;
-    rec = g_prof_n++;
+    const int rec = g_prof_n++;
     ProfRec &r = g_prof_rec[rec];
     r.name = prof_name(name);
     r.bytes = bytes;
     r.e0 = prof_event();
     r.e1 = prof_event();
-    (void)hipEventRecord(r.e0, st);
+    g_prof_pending = rec;
   }
-  ~ProfScope() {
-    if (rec >= 0) (void)hipEventRecord(g_prof_rec[rec].e1, st);
-  }
+  ~ProfScope() { g_prof_pending = -1; }
 };
-#define PROF(name, bytes) ProfScope prof_scope_##__LINE__(name, (double)(bytes), st)
+#define PROF(name, bytes) ProfScope prof_scope_##__LINE__(name, (double)(bytes))
+#define KLAUNCH(K, G, B, SH, ST, ...)                                                        \
+  do {                                                                                      \
+    if (g_prof_pending >= 0) {                                                              \
+      const ProfRec &r_ = g_prof_rec[g_prof_pending];                                       \
+      g_prof_pending = -1;                                                                  \
+      hipExtLaunchKernelGGL(K, G, B, SH, ST, r_.e0, r_.e1, 0, __VA_ARGS__);                 \
+    } else {                                                                                \
+      hipLaunchKernelGGL(K, G, B, SH, ST, __VA_ARGS__);                                     \
+    }                                                                                       \
+  } while (0)
 
 extern "C" void sba_gpu_profile_enable(int on) {
   std::lock_guard<std::mutex> lk(g_mu);
@@ -989,11 +1003,11 @@ static int enqueue_class(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef
   if (pl.last == L) {
     if (!sqr) {
       PROF("first_fwd", 8.0 * asum + KN4);
-      hipLaunchKernelGGL(k.first, dim3(1, gy1, njobs), dim3(NT), 0, st, pa, jr, 0, L);
+      KLAUNCH(k.first, dim3(1, gy1, njobs), dim3(NT), 0, st, pa, jr, 0, L);
     }
     {
       PROF("fused_last", 8.0 * bsum + (sqr ? 1 : 2) * KN4);
-      hipLaunchKernelGGL(k.fused, dim3(1, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L, 0);
+      KLAUNCH(k.fused, dim3(1, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L, 0);
     }
   } else {
     const int ell1 = L - pl.last;
@@ -1002,10 +1016,10 @@ static int enqueue_class(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef
     // both operands' first passes in one launch (one kernel boundary fewer)
     if (!sqr) {
       PROF("first_fwd", 8.0 * (asum + bsum) + 2 * KN4);
-      hipLaunchKernelGGL(k.first, dim3(ntile1, 2 * gy1, njobs), dim3(NT), 0, st, pa, jr, 2, L);
+      KLAUNCH(k.first, dim3(ntile1, 2 * gy1, njobs), dim3(NT), 0, st, pa, jr, 2, L);
     } else {
       PROF("first_fwd", 8.0 * bsum + KN4);
-      hipLaunchKernelGGL(k.first, dim3(ntile1, gy1, njobs), dim3(NT), 0, st, pa, jr, 1, L);
+      KLAUNCH(k.first, dim3(ntile1, gy1, njobs), dim3(NT), 0, st, pa, jr, 1, L);
     }
     const bool trunc = use_trunc(L, pl, zmin);
     const int mode = trunc ? MODE_TRUNC : 0;
@@ -1013,23 +1027,23 @@ static int enqueue_class(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef
     const double tf = trunc ? std::min(1.0, (zsum / njobs + (1u << pl.last)) / N) : 1.0;
     if (!sqr && SBA_MERGE_LAST) {
       PROF("fused_last", 3 * KN4 * tf);
-      hipLaunchKernelGGL(k.fused2, dim3(ntile2, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L, mode);
+      KLAUNCH(k.fused2, dim3(ntile2, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L, mode);
     } else {
       if (!sqr) {
         PROF("fwd_last", 2 * KN4);
-        hipLaunchKernelGGL(k.fwd_last, dim3(ntile2, SBA_K, njobs), dim3(NT), 0, st, pa, jr, 0, L,
+        KLAUNCH(k.fwd_last, dim3(ntile2, SBA_K, njobs), dim3(NT), 0, st, pa, jr, 0, L,
                            ell1, kNoAcc);
       }
       PROF("fused_last", (sqr ? 2 : 3) * KN4 * tf);
-      hipLaunchKernelGGL(k.fused, dim3(ntile2, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L, mode);
+      KLAUNCH(k.fused, dim3(ntile2, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L, mode);
     }
     if (trunc) {
       PROF("inv_trunc", 2 * KN4 * tf);
-      hipLaunchKernelGGL(sba_trunc_inv(ell1), dim3(ntile1, SBA_K, njobs), dim3(NT), 0, st, pa, jr,
+      KLAUNCH(sba_trunc_inv(ell1), dim3(ntile1, SBA_K, njobs), dim3(NT), 0, st, pa, jr,
                          L);
     } else {
       PROF("inv_first", 2 * KN4);
-      hipLaunchKernelGGL(k.inv_first, dim3(ntile1, SBA_K, njobs), dim3(NT), 0, st, pa, jr, 1, L,
+      KLAUNCH(k.inv_first, dim3(ntile1, SBA_K, njobs), dim3(NT), 0, st, pa, jr, 1, L,
                          0, kNoAcc);
     }
   }
@@ -1045,12 +1059,12 @@ static int enqueue_garner(DevCtx &c, uint32_t Nt, uint32_t stride, const JobRef 
   GarnerArgs ga = c.ga_base;
   {
     PROF("garner", 4.0 * SBA_K * zsum + 8.0 * zsum);
-    hipLaunchKernelGGL(k_garner_limbs, dim3(nbmax, 1, njobs), dim3(NT), 0, st, ga, jr, stride);
+    KLAUNCH(k_garner_limbs, dim3(nbmax, 1, njobs), dim3(NT), 0, st, ga, jr, stride);
   }
   {
     PROF("carry_blocks", 8.0 * zsum / CARRY_LPB);
     const uint32_t g = nbmax < CARRY_WGS ? nbmax : CARRY_WGS;
-    hipLaunchKernelGGL(k_carry_blocks, dim3(g, 1, njobs), dim3(SCAN_NT), 0, st, jr);
+    KLAUNCH(k_carry_blocks, dim3(g, 1, njobs), dim3(SCAN_NT), 0, st, jr);
   }
   HIPCHK(hipGetLastError());
   return SBA_OK;
@@ -1099,7 +1113,7 @@ extern "C" int sba_gpu_carry_blocks(uint64_t *rp, const uint32_t *bmaps, uint64_
   j.zlen = (uint32_t)(Z - 1);
   j.nb = carry_blocks(Z);
   const uint32_t g = j.nb < CARRY_WGS ? j.nb : CARRY_WGS;
-  hipLaunchKernelGGL(k_carry_blocks, dim3(g, 1, 1), dim3(SCAN_NT), 0, (hipStream_t)stream, jr);
+  KLAUNCH(k_carry_blocks, dim3(g, 1, 1), dim3(SCAN_NT), 0, (hipStream_t)stream, jr);
   HIPCHK(hipGetLastError());
   return SBA_OK;
 }
@@ -1170,11 +1184,11 @@ static int enqueue_chunked(DevCtx &c, const ChunkChoice &cp, uint64_t *rp, const
   j.an = (uint32_t)bn;
   {
     PROF("first_fwd", 8.0 * bn + KN4);
-    hipLaunchKernelGGL(k.first, dim3(ntile1, gy1, 1), dim3(NT), 0, st, pa, jr, 0, Lc);
+    KLAUNCH(k.first, dim3(ntile1, gy1, 1), dim3(NT), 0, st, pa, jr, 0, Lc);
   }
   {
     PROF("fwd_last", 2 * KN4);
-    hipLaunchKernelGGL(k.fwd_last, dim3(ntile2, SBA_K, 1), dim3(NT), 0, st, pa, jr, 0, Lc, ell1,
+    KLAUNCH(k.fwd_last, dim3(ntile2, SBA_K, 1), dim3(NT), 0, st, pa, jr, 0, Lc, ell1,
                        kNoAcc);
   }
   for (uint64_t ci = 0; ci < cp.n; ++ci) {
@@ -1184,11 +1198,11 @@ static int enqueue_chunked(DevCtx &c, const ChunkChoice &cp, uint64_t *rp, const
     j.bn = (uint32_t)len;
     {
       PROF("first_fwd", 8.0 * len + KN4);
-      hipLaunchKernelGGL(k.first, dim3(ntile1, gy1, 1), dim3(NT), 0, st, pa, jr, 1, Lc);
+      KLAUNCH(k.first, dim3(ntile1, gy1, 1), dim3(NT), 0, st, pa, jr, 1, Lc);
     }
     {
       PROF("fused_last", 3 * KN4);
-      hipLaunchKernelGGL(k.fused, dim3(ntile2, SBA_K, 1), dim3(NT), 0, st, pa, jr, Lc, 0);
+      KLAUNCH(k.fused, dim3(ntile2, SBA_K, 1), dim3(NT), 0, st, pa, jr, Lc, 0);
     }
     {
       AccArgs aa;
@@ -1198,7 +1212,7 @@ static int enqueue_chunked(DevCtx &c, const ChunkChoice &cp, uint64_t *rp, const
       aa.ovl = ci ? (uint32_t)(bn - 1) : 0u;     // the previous chunk's tail
       aa.lim = (uint32_t)(len + bn - 1);          // this chunk's product
       PROF("inv_first", KN4 + 8.0 * SBA_K * (len + bn));
-      hipLaunchKernelGGL(k.inv_first, dim3(ntile1, SBA_K, 1), dim3(NT), 0, st, pa, jr, 1, Lc, 0,
+      KLAUNCH(k.inv_first, dim3(ntile1, SBA_K, 1), dim3(NT), 0, st, pa, jr, 1, Lc, 0,
                          aa);
     }
   }
@@ -1295,22 +1309,22 @@ static int enqueue_r3(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef &j
   const uint32_t gy1 = pl.split ? SBA_K : 1;
   if (!sqr) {
     PROF("first_fwd", 8.0 * (asum + bsum) + 2 * KN4);
-    hipLaunchKernelGGL(f1, dim3(ntile1, 2 * gy1, njobs), dim3(NT), 0, st, pa, jr, 2, L);
+    KLAUNCH(f1, dim3(ntile1, 2 * gy1, njobs), dim3(NT), 0, st, pa, jr, 2, L);
   } else {
     PROF("first_fwd", 8.0 * bsum + KN4);
-    hipLaunchKernelGGL(f1, dim3(ntile1, gy1, njobs), dim3(NT), 0, st, pa, jr, 1, L);
+    KLAUNCH(f1, dim3(ntile1, gy1, njobs), dim3(NT), 0, st, pa, jr, 1, L);
   }
   {
     PROF("fused_last", (sqr ? 2 : 3) * KN4);
     if (sqr)
-      hipLaunchKernelGGL(k.fused, dim3(ntile2, SBA_K, 3 * njobs), dim3(NT), 0, st, pa, jr, L, 1);
+      KLAUNCH(k.fused, dim3(ntile2, SBA_K, 3 * njobs), dim3(NT), 0, st, pa, jr, L, 1);
     else
-      hipLaunchKernelGGL(k.fused2, dim3(ntile2, SBA_K, 3 * njobs), dim3(NT), 0, st, pa, jr, L,
+      KLAUNCH(k.fused2, dim3(ntile2, SBA_K, 3 * njobs), dim3(NT), 0, st, pa, jr, L,
                          1);
   }
   {
     PROF("inv_first", 2 * KN4);
-    hipLaunchKernelGGL(i1, dim3(ntile1, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L);
+    KLAUNCH(i1, dim3(ntile1, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L);
   }
   return enqueue_garner(c, 3 * M, 3 * M, jr, njobs, nbmax, zsum, st);
 }
@@ -1412,12 +1426,12 @@ static int enqueue_school_long_one(uint64_t *rp, const uint64_t *ap, uint64_t an
   j.bmaps = bmaps;
   {
     PROF("school_long", 8.0 * (an + bn) + 8.0 * j.Z);
-    hipLaunchKernelGGL(k_school_long, dim3(j.nb, 1, 1), dim3(NT), 0, st, jr);
+    KLAUNCH(k_school_long, dim3(j.nb, 1, 1), dim3(NT), 0, st, jr);
   }
   {
     PROF("carry_blocks", 8.0 * j.Z / CARRY_LPB);
     const uint32_t g = j.nb < CARRY_WGS ? j.nb : CARRY_WGS;
-    hipLaunchKernelGGL(k_carry_blocks, dim3(g, 1, 1), dim3(SCAN_NT), 0, st, jr);
+    KLAUNCH(k_carry_blocks, dim3(g, 1, 1), dim3(SCAN_NT), 0, st, jr);
   }
   HIPCHK(hipGetLastError());
   return SBA_OK;
@@ -1447,7 +1461,7 @@ static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64
     j.nb = 1;
     {
       PROF("school", 16.0 * (double)j.Z);
-      hipLaunchKernelGGL(k_school, dim3(1, 1, 1), dim3(NT), 0, st, jr);
+      KLAUNCH(k_school, dim3(1, 1, 1), dim3(NT), 0, st, jr);
     }
     HIPCHK(hipGetLastError());
     return SBA_OK;
@@ -1578,7 +1592,7 @@ static int na_setup(DevCtx &c, hipStream_t st) {
     uint32_t *tw = (uint32_t *)c.na_scratch;
     for (int q = 0; q < SBA_K; ++q) {
       const TwArgs ta = tw_args(c, q);
-      hipLaunchKernelGGL(k_twiddles, dim3((uint32_t)((NA_TW_COUNT + NT - 1) / NT)), dim3(NT), 0,
+      KLAUNCH(k_twiddles, dim3((uint32_t)((NA_TW_COUNT + NT - 1) / NT)), dim3(NT), 0,
                          st, ta, tw + (2 * q) * NA_TW_COUNT, tw + (2 * q + 1) * NA_TW_COUNT,
                          (uint32_t)NA_TW_COUNT);
       HIPCHK(hipGetLastError());
@@ -1623,7 +1637,7 @@ extern "C" int sba_gpu_mul_dev_noalloc(uint64_t *rp, const uint64_t *ap, int64_t
     j.zlen = (uint32_t)zlen;
     j.Z = (uint32_t)(zlen + 1);
     j.nb = 1;
-    hipLaunchKernelGGL(k_school, dim3(1, 1, 1), dim3(NT), 0, st, jr);
+    KLAUNCH(k_school, dim3(1, 1, 1), dim3(NT), 0, st, jr);
     HIPCHK(hipGetLastError());
     return SBA_OK;
   }
@@ -1840,7 +1854,7 @@ static int enqueue_classes(DevCtx &c, Job *djobs, const ClassDesc *cls, int ncls
     if (cls[k].school == 1) {
       {
         PROF("school", 16.0 * cls[k].zs);
-        hipLaunchKernelGGL(k_school, dim3(cls[k].n, 1, 1), dim3(NT), 0, st, jr);
+        KLAUNCH(k_school, dim3(cls[k].n, 1, 1), dim3(NT), 0, st, jr);
       }
       HIPCHK(hipGetLastError());
       continue;
@@ -1848,12 +1862,12 @@ static int enqueue_classes(DevCtx &c, Job *djobs, const ClassDesc *cls, int ncls
     if (cls[k].school == 2) {
       {
         PROF("school_long", 16.0 * cls[k].zs);
-        hipLaunchKernelGGL(k_school_long, dim3(cls[k].nbmax, 1, cls[k].n), dim3(NT), 0, st, jr);
+        KLAUNCH(k_school_long, dim3(cls[k].nbmax, 1, cls[k].n), dim3(NT), 0, st, jr);
       }
       {
         PROF("carry_blocks", 8.0 * cls[k].zs / CARRY_LPB);
         const uint32_t g = cls[k].nbmax < CARRY_WGS ? cls[k].nbmax : CARRY_WGS;
-        hipLaunchKernelGGL(k_carry_blocks, dim3(g, 1, cls[k].n), dim3(SCAN_NT), 0, st, jr);
+        KLAUNCH(k_carry_blocks, dim3(g, 1, cls[k].n), dim3(SCAN_NT), 0, st, jr);
       }
       HIPCHK(hipGetLastError());
       continue;

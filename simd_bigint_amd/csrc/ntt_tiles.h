@@ -491,6 +491,22 @@ __device__ __forceinline__ OpSel op_sel(int which) {
   return OpSel{op, per > 1, per > 1 ? y : 0u};
 }
 
+// residues of elements [I0, I1) of a first-pass thread (element i at tile
+// offset col0 + (i NT / TC) S + (i NT) mod TC + toff); CHECK: limbs past nl
+// read as zero
+template <int E, int LOGTC, int I0, int I1, bool CHECK>
+__device__ __forceinline__ void first_loads(uint32_t (&v)[E], gc_u64 *limbs, uint32_t col0,
+                                            uint32_t S, uint32_t toff, uint32_t nl, uint32_t p,
+                                            uint32_t pinv, uint32_t r1) {
+  constexpr int TC = 1 << LOGTC;
+#pragma unroll SBA_FIRST_LOAD_UNROLL
+  for (int i = I0; i < I1; ++i) {
+    const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+    gc_u64 *lp = limbs + uoff;
+    v[i] = limb_residue(!CHECK || uoff + toff < nl ? lp[toff] : 0, p, pinv, r1);
+  }
+}
+
 // First forward pass (global levels [0, ELL)): limbs -> residues of all K
 // primes -> ELL levels -> spec[q][.]. Tiles = column tiles (H = 0).
 // which = 0: operand a into aspec; 1: operand b into bspec; 2: both (op_sel).
@@ -533,13 +549,29 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
     __syncthreads();
     // the tile's limbs are re-read per prime (cache-resident after the first)
     if constexpr (DIRECT) {
-      // the level-0 chunk starts from the residues in registers
+      // the level-0 chunk starts from the residues in registers. Element i
+      // lies in rows i NT / TC .. (i + 1) NT / TC - 1 of the tile; with a
+      // half-empty operand the rows of the upper half (i >= E/2) are zero and
+      // the level-0 copy overwrites them, so they are neither loaded nor
+      // reduced; rows wholly inside the operand load without a bounds check
       typename ChunkRegs<ELL, LOGTC, 0, B0>::type vv;
-#pragma unroll SBA_FIRST_LOAD_UNROLL
-      for (int i = 0; i < E; ++i) {
-        const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
-        gc_u64 *lp = limbs + uoff;
-        vv[0][i] = limb_residue(uoff + toff < nl ? lp[toff] : 0, p, pinv, r1);
+#ifndef SBA_FIRST_HZ_SKIP
+#define SBA_FIRST_HZ_SKIP 1
+#endif
+#ifndef SBA_FIRST_NOCHECK
+#define SBA_FIRST_NOCHECK 0   // measured: unchecked loads with the zero-half skip were slower
+#endif
+      const bool hz = SBA_FIRST_HZ_SKIP && SBA_ZERO_LEVEL && half_zero;
+      const uint32_t rows = hz ? (1u << (ELL - 1)) : (1u << ELL);
+      const bool inside = SBA_FIRST_NOCHECK && col0 + (rows - 1) * S + (TC - 1) < nl;   // uniform
+      if (hz) {
+        if (inside) first_loads<E, LOGTC, 0, E / 2, false>(vv[0], limbs, col0, S, toff, nl, p, pinv, r1);
+        else first_loads<E, LOGTC, 0, E / 2, true>(vv[0], limbs, col0, S, toff, nl, p, pinv, r1);
+#pragma unroll
+        for (int i = E / 2; i < E; ++i) vv[0][i] = 0;
+      } else {
+        if (inside) first_loads<E, LOGTC, 0, E, false>(vv[0], limbs, col0, S, toff, nl, p, pinv, r1);
+        else first_loads<E, LOGTC, 0, E, true>(vv[0], limbs, col0, S, toff, nl, p, pinv, r1);
       }
       if constexpr (EDGE && LC::T0 == 0) {
         // one chunk: registers in, registers out

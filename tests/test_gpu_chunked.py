@@ -69,13 +69,23 @@ def test_chunked_all_ones():
     assert ok == 1 and (r == all_ones_product(an, bn)).all()
 
 
-def test_chunked_short_operand_of_one_limb():
+@pytest.mark.parametrize("tier", ["school_long", "chunked"])
+def test_chunked_short_operand_of_one_limb(tier):
+    """2^23 x 1 limbs: by default the short x long schoolbook takes it; with
+    that tier off, the chunked NTT (the same product either way)."""
     an = 1 << 23
-    assert sba.chunking(an, 1)[2] >= 2
-    st = [5]
-    x = xorshift_fill(an, st)
-    y = np.array([0xFFFFFFFFFFFFFFFF], np.uint64)
-    ok, r = sba.nt_mul(x, y)
+    old = sba.get_school_long_ratio()
+    if tier == "chunked":
+        sba.set_school_long_ratio(0)
+    try:
+        assert (sba.school_route(an, 1) == 2) == (tier == "school_long")
+        assert (sba.chunking(an, 1)[2] >= 2) == (tier == "chunked")
+        st = [5]
+        x = xorshift_fill(an, st)
+        y = np.array([0xFFFFFFFFFFFFFFFF], np.uint64)
+        ok, r = sba.nt_mul(x, y)
+    finally:
+        sba.set_school_long_ratio(old)
     want = np.zeros(an + 1, np.uint64)
     orc.lib().orc_mul_school(want, x, an, y, 1)
     assert ok == 1 and (r == want).all()
