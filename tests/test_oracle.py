@@ -165,3 +165,39 @@ def test_planted_mistakes_are_caught():
         assert sha(bad) != sha(good)
         assert any(orc.fingerprint(bad, q) != orc.fingerprint(a, q) * orc.fingerprint(b, q) % q
                    for q in qs)
+
+
+def test_oracle_generator_matches_the_scalar_recurrence():
+    """orc_fill_xorshift (the reference's xorshift64 13/7/17,
+    bench/nt_sweep.c:15-16) against the recurrence written out in Python,
+    and against the vectorised jump-matrix fill of the workloads module at
+    a length that takes its fast path; the state carries across calls."""
+    import ctypes
+    from simd_bigint_amd.workloads import xorshift_fill
+    M = (1 << 64) - 1
+
+    def xs(x):
+        x ^= (x << 13) & M
+        x ^= x >> 7
+        x ^= (x << 17) & M
+        return x
+
+    st = ctypes.c_uint64(42)
+    a = np.zeros(1000, np.uint64)
+    orc.lib().orc_fill_xorshift(a, len(a), ctypes.byref(st))
+    x, want = 42, []
+    for _ in range(1000):
+        x = xs(x)
+        want.append(x)
+    assert [int(v) for v in a] == want and st.value == want[-1]
+    b = np.zeros(5, np.uint64)
+    orc.lib().orc_fill_xorshift(b, 5, ctypes.byref(st))
+    for v in b:
+        x = xs(x)
+        assert int(v) == x
+    n = (1 << 17) + 12345                            # the jump-matrix path
+    st2 = ctypes.c_uint64(7)
+    c = np.zeros(n, np.uint64)
+    orc.lib().orc_fill_xorshift(c, n, ctypes.byref(st2))
+    pst = [7]
+    assert (xorshift_fill(n, pst) == c).all() and pst[0] == st2.value
