@@ -4,8 +4,8 @@
  * carry the reference's names, argument meaning, length conventions and
  * return codes. Operands are little-endian u64 limb arrays passed as
  * pointer + length. All arithmetic on the GPU is exact integer work (a
- * five-prime 31-bit NTT, Garner CRT and a carry-lookahead scan); results
- * are bit-identical to the reference's.
+ * five-prime NTT over primes below 2^31, Garner CRT and a carry-lookahead
+ * scan); results are bit-identical to the reference's.
  *
  * Host-pointer entries are synchronous: they copy the operands to the
  * device, multiply there, copy the product back and return. Device-pointer
@@ -23,9 +23,10 @@
  * the reference's arena does (include/scratch.h:66-70). Device entries
  * return the codes below.
  *
- * Device memory. The first GPU call on a device builds its root tables
- * once: 2 x 5 primes x 2^24 x 4 B = 640 MiB, held until the process ends
- * (one table serves every transform length up to 2^25). Entries without a
+ * Device memory. The first GPU call on a device builds the root tables of
+ * its two prime sets once: 2 x 5 primes x (2^24 + 2^22) x 4 B = 800 MiB,
+ * held until the process ends (one table per prime serves every transform
+ * length of its set). Entries without a
  * caller workspace add a grow-only workspace of about 40 B per transform
  * point of the largest product seen (2^23 points: 336 MiB).
  *
@@ -172,6 +173,13 @@ uint32_t sba_get_school_long_ratio(void);
  * on MI355X; batches always from 13) */
 void sba_set_r3_min_lg(int L);
 int sba_get_r3_min_lg(void);
+/* the prime set of power-of-two transforms up to 2^23: 1 (default) three
+ * primes below 2^30, whose butterflies take 8 instructions instead of 10,
+ * and two below 2^31; 0 the five primes below 2^31 that every other
+ * transform uses. Results are the same either way; batches keep the choice
+ * in force when they were created. */
+void sba_set_prime_set_h(int on);
+int sba_get_prime_set_h(void);
 /* 1 when a single product of (an, bn) runs truncated (a power-of-two length
  * 2^*L whose merged last pass transforms only the first *rows tiles of
  * 2^*last residues, rows = ceil((an+bn-1) / 2^last), and whose inverse

@@ -509,8 +509,30 @@ __global__ __launch_bounds__(NT) void k_twiddles(TwArgs ta, uint32_t *__restrict
 // ===========================================================================
 // host side of the thin layer
 // ===========================================================================
-static const uint32_t kPrimes[SBA_K] = {1107296257u, 1711276033u, 1811939329u, 2013265921u,
-                                        2113929217u};
+// Prime sets, each five primes in ascending order.
+//   SET_W (wide): k 2^25 + 1 < 2^31 with 3 | p - 1 -- every transform length
+//     up to 2^25, radix-3 lengths, CRT capacity 2^153.4.
+//   SET_H: three primes below 2^30 (Harvey's 8-instruction butterflies) and
+//     two below 2^31, all with 2^23 | p - 1 (none of the first two with
+//     3 | p - 1) -- power-of-two lengths up to 2^23, CRT capacity 2^151.3,
+//     which holds the largest coefficient such a product has,
+//     min(an, bn) (2^64 - 1)^2 < 2^22 * 2^128 = 2^150.
+// The merged last pass (k_fused2) runs Harvey's forms for the three small
+// primes: its butterfly work is (3 * 8 + 2 * 10) / (5 * 10) = 0.88 of the
+// set-W pass. The other kernels run the [0, 2p) forms for every prime.
+#define SET_W 0
+#define SET_H 1
+#define NSETS 2
+struct PrimeSetDef {
+  uint32_t p[SBA_K];
+  uint32_t nh;                         // leading primes below 2^30
+  int max_lg;                          // largest power-of-two transform
+};
+static const PrimeSetDef kSets[NSETS] = {
+    {{1107296257u, 1711276033u, 1811939329u, 2013265921u, 2113929217u}, 0, SBA_MAX_LG},
+    {{880803841u, 897581057u, 998244353u, 2113929217u, 2130706433u}, 3, 23},
+};
+static const uint32_t *const kPrimes = kSets[SET_W].p;   // the band's (CRT cap) set
 
 static uint64_t h_powm(uint64_t a, uint64_t e, uint64_t p) {
   uint64_t r = 1;
@@ -539,12 +561,17 @@ static uint64_t h_primroot(uint64_t p) {
   }
 }
 
-struct DevCtx {
-  int init = 0;
+// one prime set's constants and root tables on a device
+struct SetCtx {
   int tw_lg = 0;                       // tables hold 2^(tw_lg-1) node twiddles
   uint32_t *wn[SBA_K] = {}, *wni[SBA_K] = {};
   PrimeArgs pa;
   GarnerArgs ga_base;
+};
+
+struct DevCtx {
+  int init = 0;
+  SetCtx set[NSETS];
   // grow-only workspace
   void *ws = nullptr;
   size_t ws_bytes = 0;
@@ -569,10 +596,10 @@ static std::mutex g_mu;
     }                                                                                 \
   } while (0)
 
-static int ctx_init(DevCtx &c) {
-  if (c.init) return 0;
+static void set_init(SetCtx &c, const PrimeSetDef &d) {
+  const uint32_t *P = d.p;
   for (int q = 0; q < SBA_K; ++q) {
-    const uint64_t p = kPrimes[q];
+    const uint64_t p = P[q];
     uint32_t inv = 1;
     for (int i = 0; i < 5; ++i) inv *= 2u - (uint32_t)p * inv;   // p^-1 mod 2^32
     c.pa.p[q] = (uint32_t)p;
@@ -581,33 +608,44 @@ static int ctx_init(DevCtx &c) {
     c.ga_base.p[q] = (uint32_t)p;
     c.ga_base.pinv[q] = 0u - inv;
   }
+  c.pa.nh = d.nh;
   for (int q = 0; q < SBA_K; ++q) {
-    // length-3 DFT constants: w = g^((p-1)/3) a cube root of unity
-    const uint64_t p = kPrimes[q];
-    const uint64_t w = h_powm(h_primroot(p), (p - 1) / 3, p);
+    // length-3 DFT constants: w = g^((p-1)/3) a cube root of unity (only
+    // where 3 | p - 1: a set without them never runs a radix-3 class)
+    const uint64_t p = P[q];
     const uint64_t inv2 = (p + 1) / 2;
-    const uint64_t h2 = (w + p - w * w % p) % p * inv2 % p;
     c.pa.inv2[q] = h_mont(inv2, p);
-    c.pa.h1[q] = h_mont(p - inv2, p);
-    c.pa.h2[q] = h_mont(h2, p);
-    c.pa.h2i[q] = h_mont((p - h2) % p, p);
+    if ((p - 1) % 3 == 0) {
+      const uint64_t w = h_powm(h_primroot(p), (p - 1) / 3, p);
+      const uint64_t h2 = (w + p - w * w % p) % p * inv2 % p;
+      c.pa.h1[q] = h_mont(p - inv2, p);
+      c.pa.h2[q] = h_mont(h2, p);
+      c.pa.h2i[q] = h_mont((p - h2) % p, p);
+    } else {
+      c.pa.h1[q] = c.pa.h2[q] = c.pa.h2i[q] = 0;
+    }
   }
   for (int q = 1; q < SBA_K; ++q) {
-    const uint64_t pq = kPrimes[q];
+    const uint64_t pq = P[q];
     uint64_t pp = 1;
     for (int k = 0; k < q; ++k) {
       c.ga_base.cm[q][k] = h_mont(pp, pq);
-      pp = pp * (kPrimes[k] % pq) % pq;
+      pp = pp * (P[k] % pq) % pq;
     }
     c.ga_base.im[q] = h_mont(h_invm(pp, pq), pq);
   }
+}
+
+static int ctx_init(DevCtx &c) {
+  if (c.init) return 0;
+  for (int s = 0; s < NSETS; ++s) set_init(c.set[s], kSets[s]);
   c.init = 1;
   return 0;
 }
 
-// the root-table kernel's arguments for prime q
-static TwArgs tw_args(const DevCtx &c, int q) {
-  const uint64_t p = kPrimes[q];
+// the root-table kernel's arguments for prime q of a set
+static TwArgs tw_args(const SetCtx &c, const PrimeSetDef &d, int q) {
+  const uint64_t p = d.p[q];
   const uint64_t g = h_primroot(p);
   TwArgs ta;
   ta.p = (uint32_t)p;
@@ -616,32 +654,49 @@ static TwArgs tw_args(const DevCtx &c, int q) {
   for (int t = 0; t < SBA_MAX_LG; ++t) {
     // bit t of a node index carries zeta_{t+2}, a root of order 2^(t+2)
     const int ord = t + 2;
-    uint64_t z = ord <= SBA_MAX_LG ? h_powm(g, (p - 1) >> ord, p) : 1;
+    uint64_t z = ord <= d.max_lg ? h_powm(g, (p - 1) >> ord, p) : 1;
     ta.z[t] = h_mont(z, p);
     ta.zi[t] = h_mont(h_invm(z, p), p);
   }
   return ta;
 }
 
+// both sets' root tables, built once for each set's largest transform
+// (SET_W: 2^24 node twiddles per table, 640 MiB; SET_H: 2^22, 160 MiB)
 static int ctx_twiddles(DevCtx &c, int L, hipStream_t st) {
-  if (L <= c.tw_lg) return 0;
-  const int Lt = SBA_MAX_LG;            // build once for the largest transform
-  const uint32_t count = 1u << (Lt - 1);
-  for (int q = 0; q < SBA_K; ++q) {
-    if (c.wn[q]) { HIPCHK(hipFree(c.wn[q])); HIPCHK(hipFree(c.wni[q])); }
-    c.wn[q] = c.wni[q] = nullptr;
-    HIPCHK(hipMalloc(&c.wn[q], (size_t)count * 4));
-    HIPCHK(hipMalloc(&c.wni[q], (size_t)count * 4));
-    const TwArgs ta = tw_args(c, q);
-    hipLaunchKernelGGL(k_twiddles, dim3((count + NT - 1) / NT), dim3(NT), 0, st, ta, c.wn[q],
-                       c.wni[q], count);
-    HIPCHK(hipGetLastError());
-    c.pa.wn[q] = c.wn[q];
-    c.pa.wni[q] = c.wni[q];
+  (void)L;
+  for (int s = 0; s < NSETS; ++s) {
+    SetCtx &sc = c.set[s];
+    const int Lt = kSets[s].max_lg;
+    if (sc.tw_lg >= Lt) continue;
+    const uint32_t count = 1u << (Lt - 1);
+    for (int q = 0; q < SBA_K; ++q) {
+      if (sc.wn[q]) { HIPCHK(hipFree(sc.wn[q])); HIPCHK(hipFree(sc.wni[q])); }
+      sc.wn[q] = sc.wni[q] = nullptr;
+      HIPCHK(hipMalloc(&sc.wn[q], (size_t)count * 4));
+      HIPCHK(hipMalloc(&sc.wni[q], (size_t)count * 4));
+      const TwArgs ta = tw_args(sc, kSets[s], q);
+      hipLaunchKernelGGL(k_twiddles, dim3((count + NT - 1) / NT), dim3(NT), 0, st, ta,
+                         sc.wn[q], sc.wni[q], count);
+      HIPCHK(hipGetLastError());
+      sc.pa.wn[q] = sc.wn[q];
+      sc.pa.wni[q] = sc.wni[q];
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    sc.tw_lg = Lt;
   }
-  HIPCHK(hipStreamSynchronize(st));
-  c.tw_lg = Lt;
   return 0;
+}
+
+// the prime set of a transform: SET_H for power-of-two lengths it holds
+// (unless switched off), SET_W otherwise
+static std::atomic<int> g_use_set_h{1};
+extern "C" void sba_set_prime_set_h(int on) { g_use_set_h.store(on ? 1 : 0); }
+extern "C" int sba_get_prime_set_h(void) { return g_use_set_h.load(); }
+static int set_for(int L, int r3) {
+  if (!r3 && L <= kSets[SET_H].max_lg && g_use_set_h.load(std::memory_order_relaxed))
+    return SET_H;
+  return SET_W;
 }
 
 // workspace layout for transform length N: aspec, bspec (K*N u32 each),
@@ -924,17 +979,17 @@ extern "C" int sba_gpu_probe_butterflies(double *per_s) {
   return rc;
 }
 
-static int enqueue_garner(DevCtx &c, uint32_t Nt, uint32_t stride, const JobRef &jr,
+static int enqueue_garner(DevCtx &c, int set, uint32_t Nt, uint32_t stride, const JobRef &jr,
                           uint32_t njobs, uint32_t nbmax, double zsum, hipStream_t st);
 static const AccArgs kNoAcc = {nullptr, 0, 0, 0, 0};
 
 // the context's prime constants with the inverse's final scale for
 // transform length Nt: Nt^-1 * R^4 mod p undoes Nt, the pointwise R^-1, the
 // R^-1 each operand's limb residues carry and this multiply's own R^-1
-static PrimeArgs prime_args(const DevCtx &c, uint64_t Nt) {
-  PrimeArgs pa = c.pa;
+static PrimeArgs prime_args(const DevCtx &c, int set, uint64_t Nt) {
+  PrimeArgs pa = c.set[set].pa;
   for (int q = 0; q < SBA_K; ++q) {
-    const uint64_t p = kPrimes[q];
+    const uint64_t p = kSets[set].p[q];
     pa.scale[q] = (uint32_t)((unsigned __int128)h_invm(Nt % p, p) * h_powm(2, 128, p) % p);
   }
   return pa;
@@ -991,13 +1046,13 @@ extern "C" void sba_get_trunc(int *max_fill_pct, int *r3_min_pct, int *min_lg) {
 // nullptr: the single job jr.one) under plan pl. nbmax = the largest
 // carry-block count; zmin = the smallest product (coefficients) of the
 // class; zsum / asum / bsum feed the profiler's byte counts.
-static int enqueue_class(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef &jr,
+static int enqueue_class(DevCtx &c, int set, int L, int sqr, const Plan &pl, const JobRef &jr,
                          uint32_t njobs, uint32_t nbmax, uint32_t zmin, double asum, double bsum,
                          double zsum, hipStream_t st) {
-  if (!plan_ok(L, pl)) return SBA_BAND;
+  if (!plan_ok(L, pl) || L > kSets[set].max_lg) return SBA_BAND;
   const Launchers k = launchers(L, sqr, pl);
   const uint32_t N = 1u << L;
-  const PrimeArgs pa = prime_args(c, N);
+  const PrimeArgs pa = prime_args(c, set, N);
   const double KN4 = 4.0 * SBA_K * (double)N * njobs;
   const uint32_t gy1 = pl.split ? SBA_K : 1;
   if (pl.last == L) {
@@ -1047,16 +1102,16 @@ static int enqueue_class(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef
                          0, kNoAcc);
     }
   }
-  return enqueue_garner(c, N, N, jr, njobs, nbmax, zsum, st);
+  return enqueue_garner(c, set, N, N, jr, njobs, nbmax, zsum, st);
 }
 
 // Garner + carry scan for njobs jobs whose product residues sit in bspec
 // with prime stride `stride`, left scaled by the transform length Nt by the
 // unnormalised inverse
-static int enqueue_garner(DevCtx &c, uint32_t Nt, uint32_t stride, const JobRef &jr,
+static int enqueue_garner(DevCtx &c, int set, uint32_t Nt, uint32_t stride, const JobRef &jr,
                           uint32_t njobs, uint32_t nbmax, double zsum, hipStream_t st) {
   (void)Nt;
-  GarnerArgs ga = c.ga_base;
+  GarnerArgs ga = c.set[set].ga_base;
   {
     PROF("garner", 4.0 * SBA_K * zsum + 8.0 * zsum);
     KLAUNCH(k_garner_limbs, dim3(nbmax, 1, njobs), dim3(NT), 0, st, ga, jr, stride);
@@ -1165,7 +1220,8 @@ static int enqueue_chunked(DevCtx &c, const ChunkChoice &cp, uint64_t *rp, const
   const Plan pl = plan_for(Lc, 0, 1);
   if (!plan_ok(Lc, pl) || pl.last == Lc) return SBA_BAND;
   const Launchers k = launchers(Lc, 0, pl);
-  const PrimeArgs pa = prime_args(c, Nc);
+  const int set = set_for(Lc, 0);
+  const PrimeArgs pa = prime_args(c, set, Nc);
   const uint64_t Z = an + bn;
   const uint32_t Zs = (uint32_t)((Z + 255) & ~(uint64_t)255);
   const int ell1 = Lc - pl.last;
@@ -1226,7 +1282,7 @@ static int enqueue_chunked(DevCtx &c, const ChunkChoice &cp, uint64_t *rp, const
   g.Z = (uint32_t)Z;
   g.nb = carry_blocks(Z);
   g.bmaps = acc + (size_t)SBA_K * Zs;
-  return enqueue_garner(c, Nc, Zs, jg, 1, g.nb, (double)Z, st);
+  return enqueue_garner(c, set, Nc, Zs, jg, 1, g.nb, (double)Z, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1303,7 +1359,7 @@ static int enqueue_r3(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef &j
   const first_fn f1 = sba_r3_first(ell1);
   const inv3_fn i1 = sba_r3_inv(ell1);
   const uint32_t M = 1u << L;
-  const PrimeArgs pa = prime_args(c, 3 * (uint64_t)M);
+  const PrimeArgs pa = prime_args(c, SET_W, 3 * (uint64_t)M);
   const double KN4 = 4.0 * SBA_K * 3.0 * M * njobs;
   const uint32_t ntile1 = M >> 11, ntile2 = M >> pl.last;
   const uint32_t gy1 = pl.split ? SBA_K : 1;
@@ -1326,7 +1382,7 @@ static int enqueue_r3(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef &j
     PROF("inv_first", 2 * KN4);
     KLAUNCH(i1, dim3(ntile1, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L);
   }
-  return enqueue_garner(c, 3 * M, 3 * M, jr, njobs, nbmax, zsum, st);
+  return enqueue_garner(c, SET_W, 3 * M, 3 * M, jr, njobs, nbmax, zsum, st);
 }
 
 // one transform of length <= 2^25 (batches use only this)
@@ -1529,8 +1585,8 @@ static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64
   if (L3)
     return enqueue_r3(*c, L3, sqr, r3_plan(L3, sqr, 1), jr, 1, j.nb, (double)an, (double)bn,
                       (double)j.Z, st);
-  return enqueue_class(*c, L, sqr, plan_for(L, sqr, 1), jr, 1, j.nb, j.zlen, (double)an,
-                       (double)bn, (double)j.Z, st);
+  return enqueue_class(*c, set_for(L, 0), L, sqr, plan_for(L, sqr, 1), jr, 1, j.nb, j.zlen,
+                       (double)an, (double)bn, (double)j.Z, st);
 }
 
 extern "C" int sba_gpu_mul_dev(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64_t *bp,
@@ -1591,7 +1647,7 @@ static int na_setup(DevCtx &c, hipStream_t st) {
   if (!c.na_tables) {
     uint32_t *tw = (uint32_t *)c.na_scratch;
     for (int q = 0; q < SBA_K; ++q) {
-      const TwArgs ta = tw_args(c, q);
+      const TwArgs ta = tw_args(c.set[SET_W], kSets[SET_W], q);
       KLAUNCH(k_twiddles, dim3((uint32_t)((NA_TW_COUNT + NT - 1) / NT)), dim3(NT), 0,
                          st, ta, tw + (2 * q) * NA_TW_COUNT, tw + (2 * q + 1) * NA_TW_COUNT,
                          (uint32_t)NA_TW_COUNT);
@@ -1646,13 +1702,14 @@ extern "C" int sba_gpu_mul_dev_noalloc(uint64_t *rp, const uint64_t *ap, int64_t
   if (rc) return rc;
   if ((rc = na_setup(*c, st)) != 0) return rc;
   // the context with the static tables in place of the allocated ones
+  // (the wide prime set only: the static tables are that set's)
   DevCtx nc;
-  nc.pa = c->pa;
-  nc.ga_base = c->ga_base;
+  nc.set[SET_W].pa = c->set[SET_W].pa;
+  nc.set[SET_W].ga_base = c->set[SET_W].ga_base;
   const uint32_t *tw = (const uint32_t *)c->na_scratch;
   for (int q = 0; q < SBA_K; ++q) {
-    nc.pa.wn[q] = tw + (2 * q) * NA_TW_COUNT;
-    nc.pa.wni[q] = tw + (2 * q + 1) * NA_TW_COUNT;
+    nc.set[SET_W].pa.wn[q] = tw + (2 * q) * NA_TW_COUNT;
+    nc.set[SET_W].pa.wni[q] = tw + (2 * q + 1) * NA_TW_COUNT;
   }
   // always one transform (power of two or 3 * 2^L), never chunked
   const int L3 = r3_lg(zlen, true);
@@ -1674,8 +1731,8 @@ extern "C" int sba_gpu_mul_dev_noalloc(uint64_t *rp, const uint64_t *ap, int64_t
   if (L3)
     return enqueue_r3(nc, L3, sqr, r3_plan(L3, sqr, 1), jr, 1, j.nb, (double)an, (double)bn,
                       (double)j.Z, st);
-  return enqueue_class(nc, L, sqr, plan_for(L, sqr, 1), jr, 1, j.nb, j.zlen, (double)an,
-                       (double)bn, (double)j.Z, st);
+  return enqueue_class(nc, SET_W, L, sqr, plan_for(L, sqr, 1), jr, 1, j.nb, j.zlen,
+                       (double)an, (double)bn, (double)j.Z, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1875,8 +1932,9 @@ static int enqueue_classes(DevCtx &c, Job *djobs, const ClassDesc *cls, int ncls
     const int rc =
         cls[k].r3 ? enqueue_r3(c, cls[k].L, cls[k].sqr, cls[k].plan, jr, cls[k].n, cls[k].nbmax,
                                cls[k].as, cls[k].bs, cls[k].zs, st)
-                  : enqueue_class(c, cls[k].L, cls[k].sqr, cls[k].plan, jr, cls[k].n,
-                                  cls[k].nbmax, cls[k].zmin, cls[k].as, cls[k].bs, cls[k].zs, st);
+                  : enqueue_class(c, set_for(cls[k].L, 0), cls[k].L, cls[k].sqr, cls[k].plan, jr,
+                                  cls[k].n, cls[k].nbmax, cls[k].zmin, cls[k].as, cls[k].bs,
+                                  cls[k].zs, st);
     if (rc != SBA_OK) return rc;
   }
   return SBA_OK;

@@ -19,6 +19,11 @@ struct PrimeArgs {
   // (Nt the transform length; see the host's prime_args)
   uint32_t scale[SBA_K];
   uint32_t inv2[SBA_K];                          // 1/2 (Montgomery, canonical)
+  // the set's leading primes q < nh are below 2^30: the merged last pass
+  // gives them Harvey's butterflies (forward values lazy in [0, 4p)). Every
+  // other kernel runs the [0, 2p) forms for all primes (valid for any p <
+  // 2^31), so spectra cross kernel boundaries in [0, 2p) either way.
+  uint32_t nh;
   const uint32_t *wn[SBA_K];
   const uint32_t *wni[SBA_K];
 };
@@ -53,6 +58,9 @@ struct GarnerArgs {
 // first pass's limb loads, fixed by measurement
 #ifndef SBA_FUSED2_WAVES
 #define SBA_FUSED2_WAVES 3
+#endif
+#ifndef SBA_FUSED2_REGS
+#define SBA_FUSED2_REGS 1    // merged last pass: both ends and the pointwise product in registers
 #endif
 #ifndef SBA_COL_REGS
 #define SBA_COL_REGS 1       // column passes: level-0 chunk straight from / to registers
@@ -112,6 +120,10 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
   if (n & 7) return b;
   return (b & 7) * (n >> 3) + (b >> 3);
 }
+
+// a compile-time flag passed to generic lambdas (the per-prime bodies of
+// the pass kernels, instantiated once per butterfly form)
+template <bool V> struct BoolC { static constexpr bool value = V; };
 
 template <int X> struct ILog2 { static constexpr int v = 1 + ILog2<X / 2>::v; };
 template <> struct ILog2<1> { static constexpr int v = 0; };
@@ -176,8 +188,9 @@ template <int ELL, int LOGTC, int T0, int B> struct ChunkRegs {
 // GUARD (inverse only, column passes: H = 0): a butterfly runs only when its
 // block of the column transform lies inside rows [0, gM) -- the complete
 // blocks of a truncated inverse; the others leave their values as they are.
+// HV: the prime is below 2^30 (Harvey's butterflies, gpu_arith.h).
 template <int ELL, int LOGTC, int T0, int B, bool INV, int ZL = 0, int IO = CHUNK_LDS,
-          int TE = TW_EARLY, bool GUARD = false>
+          int TE = TW_EARLY, bool GUARD = false, bool HV = false>
 __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__restrict__ tw,
                                               uint32_t H, uint32_t p, uint32_t pinv,
                                               typename ChunkRegs<ELL, LOGTC, T0, B>::type &vv,
@@ -256,7 +269,8 @@ __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__r
 #pragma unroll
           for (int r = 0; r < half; ++r) {
             const int m = (m2 << (B - u)) | r;
-            bfly_fwd(v[m], v[m + half], w, p, pinv);
+            if (HV) bfly_fwd_h(v[m], v[m + half], w, p, pinv);
+            else bfly_fwd(v[m], v[m + half], w, p, pinv);
           }
         }
       }
@@ -272,7 +286,10 @@ __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__r
 #pragma unroll
           for (int r = 0; r < half; ++r) {
             const int m = (m2 << (B - u)) | r;
-            if (run) bfly_inv(v[m], v[m + half], w, p, pinv);
+            if (run) {
+              if (HV) bfly_inv_h(v[m], v[m + half], w, p, pinv);
+              else bfly_inv(v[m], v[m + half], w, p, pinv);
+            }
           }
         }
       }
@@ -286,11 +303,13 @@ __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__r
   }
 }
 
-template <int ELL, int LOGTC, int T0, int B, bool INV, int ZL = 0, int TE = TW_EARLY>
+template <int ELL, int LOGTC, int T0, int B, bool INV, int ZL = 0, int TE = TW_EARLY,
+          bool HV = false>
 __device__ __forceinline__ void tile_chunk(uint32_t *lds, const uint32_t *__restrict__ tw,
                                            uint32_t H, uint32_t p, uint32_t pinv) {
   typename ChunkRegs<ELL, LOGTC, T0, B>::type unused;
-  tile_chunk_io<ELL, LOGTC, T0, B, INV, ZL, CHUNK_LDS, TE>(lds, tw, H, p, pinv, unused);
+  tile_chunk_io<ELL, LOGTC, T0, B, INV, ZL, CHUNK_LDS, TE, false, HV>(lds, tw, H, p, pinv,
+                                                                     unused);
 }
 
 // chunk sizes: the largest B <= BM whose group addressing is linear,
@@ -311,35 +330,35 @@ template <int ELL, int LOGTC, int BMAX> struct ChunkPlan {
   }
 };
 
-template <int ELL, int LOGTC, int BMAX, int T0, int ZL = 0, int TE = TW_EARLY>
+template <int ELL, int LOGTC, int BMAX, int T0, int ZL = 0, int TE = TW_EARLY, bool HV = false>
 __device__ __forceinline__ void tile_fwd(uint32_t *lds, const uint32_t *tw, uint32_t H,
                                          uint32_t p, uint32_t pinv) {
   if constexpr (T0 < ELL) {
     constexpr int B = ChunkPlan<ELL, LOGTC, BMAX>::pick(T0);
-    tile_chunk<ELL, LOGTC, T0, B, false, ZL, TE>(lds, tw, H, p, pinv);
-    tile_fwd<ELL, LOGTC, BMAX, T0 + B, ZL, TE>(lds, tw, H, p, pinv);
+    tile_chunk<ELL, LOGTC, T0, B, false, ZL, TE, HV>(lds, tw, H, p, pinv);
+    tile_fwd<ELL, LOGTC, BMAX, T0 + B, ZL, TE, HV>(lds, tw, H, p, pinv);
   }
 }
 
 // inverse chunks of the levels from T0 below TEND (highest first), LDS to LDS
-template <int ELL, int LOGTC, int BMAX, int T0, int TEND>
+template <int ELL, int LOGTC, int BMAX, int T0, int TEND, bool HV = false, int TE = TW_EARLY>
 __device__ __forceinline__ void tile_inv_upto(uint32_t *lds, const uint32_t *tw, uint32_t H,
                                               uint32_t p, uint32_t pinv) {
   if constexpr (T0 < TEND) {
     constexpr int B = ChunkPlan<ELL, LOGTC, BMAX>::pick(T0);
-    tile_inv_upto<ELL, LOGTC, BMAX, T0 + B, TEND>(lds, tw, H, p, pinv);
-    tile_chunk<ELL, LOGTC, T0, B, true>(lds, tw, H, p, pinv);
+    tile_inv_upto<ELL, LOGTC, BMAX, T0 + B, TEND, HV, TE>(lds, tw, H, p, pinv);
+    tile_chunk<ELL, LOGTC, T0, B, true, 0, TE, HV>(lds, tw, H, p, pinv);
   }
 }
 
 // forward chunks from T0 up to (not including) level TEND, LDS to LDS
-template <int ELL, int LOGTC, int BMAX, int T0, int TEND>
+template <int ELL, int LOGTC, int BMAX, int T0, int TEND, bool HV = false, int TE = TW_EARLY>
 __device__ __forceinline__ void tile_fwd_range(uint32_t *lds, const uint32_t *tw, uint32_t H,
                                                uint32_t p, uint32_t pinv) {
   if constexpr (T0 < TEND) {
     constexpr int B = ChunkPlan<ELL, LOGTC, BMAX>::pick(T0);
-    tile_chunk<ELL, LOGTC, T0, B, false>(lds, tw, H, p, pinv);
-    tile_fwd_range<ELL, LOGTC, BMAX, T0 + B, TEND>(lds, tw, H, p, pinv);
+    tile_chunk<ELL, LOGTC, T0, B, false, 0, TE, HV>(lds, tw, H, p, pinv);
+    tile_fwd_range<ELL, LOGTC, BMAX, T0 + B, TEND, HV, TE>(lds, tw, H, p, pinv);
   }
 }
 
@@ -383,11 +402,11 @@ template <int ELL, int LOGTC, int T0, int B> struct ChunkIO {
 // the forward levels of a first pass, with its first level a copy when the
 // operand fills at most half of the transform (SBA_ZERO_LEVEL; one uniform
 // branch per block, each side straight-line code)
-template <int ELL, int LOGTC, int BMAX>
+template <int ELL, int LOGTC, int BMAX, bool HV = false>
 __device__ __forceinline__ void tile_fwd_first(uint32_t *lds, const uint32_t *tw, uint32_t p,
                                                uint32_t pinv, bool half_zero) {
-  if (SBA_ZERO_LEVEL && half_zero) tile_fwd<ELL, LOGTC, BMAX, 0, 1>(lds, tw, 0, p, pinv);
-  else tile_fwd<ELL, LOGTC, BMAX, 0, 0>(lds, tw, 0, p, pinv);
+  if (SBA_ZERO_LEVEL && half_zero) tile_fwd<ELL, LOGTC, BMAX, 0, 1, TW_EARLY, HV>(lds, tw, 0, p, pinv);
+  else tile_fwd<ELL, LOGTC, BMAX, 0, 0, TW_EARLY, HV>(lds, tw, 0, p, pinv);
 }
 
 
@@ -405,13 +424,13 @@ __device__ __forceinline__ void tile_inv_guard(uint32_t *lds, const uint32_t *tw
   }
 }
 
-template <int ELL, int LOGTC, int BMAX, int T0, int TE = TW_EARLY>
+template <int ELL, int LOGTC, int BMAX, int T0, int TE = TW_EARLY, bool HV = false>
 __device__ __forceinline__ void tile_inv(uint32_t *lds, const uint32_t *tw, uint32_t H,
                                          uint32_t p, uint32_t pinv) {
   if constexpr (T0 < ELL) {
     constexpr int B = ChunkPlan<ELL, LOGTC, BMAX>::pick(T0);
-    tile_inv<ELL, LOGTC, BMAX, T0 + B, TE>(lds, tw, H, p, pinv);
-    tile_chunk<ELL, LOGTC, T0, B, true, 0, TE>(lds, tw, H, p, pinv);
+    tile_inv<ELL, LOGTC, BMAX, T0 + B, TE, HV>(lds, tw, H, p, pinv);
+    tile_chunk<ELL, LOGTC, T0, B, true, 0, TE, HV>(lds, tw, H, p, pinv);
   }
 }
 
@@ -510,6 +529,12 @@ __device__ __forceinline__ void first_loads(uint32_t (&v)[E], gc_u64 *limbs, uin
 // First forward pass (global levels [0, ELL)): limbs -> residues of all K
 // primes -> ELL levels -> spec[q][.]. Tiles = column tiles (H = 0).
 // which = 0: operand a into aspec; 1: operand b into bspec; 2: both (op_sel).
+#ifndef SBA_FIRST_HZ_SKIP
+#define SBA_FIRST_HZ_SKIP 1
+#endif
+#ifndef SBA_FIRST_NOCHECK
+#define SBA_FIRST_NOCHECK 0   // measured: unchecked loads with the zero-half skip were slower
+#endif
 template <int ELL, int LOGTC, int BMAX>
 __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa, JobRef jr, int which,
                                                              int L) {
@@ -543,8 +568,9 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
   // columns keep 32-byte row segments in that layout too)
   using LC = LastChunk<ELL, LOGTC, BMAX>;
   constexpr bool EDGE = DIRECT && SBA_COL_EDGE && ChunkIO<ELL, LOGTC, LC::T0, LC::B>::OK;
-#pragma unroll 1
-  for (uint32_t q = q0; q < q1; ++q) {
+  // one prime's residues and levels; HV: the prime takes Harvey's butterflies
+  auto prime = [&](uint32_t q, auto hv) {
+    constexpr bool HV = decltype(hv)::value;
     const uint32_t p = pa.p[q], pinv = pa.pinv[q], r1 = pa.r1[q];
     __syncthreads();
     // the tile's limbs are re-read per prime (cache-resident after the first)
@@ -555,12 +581,6 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
       // the level-0 copy overwrites them, so they are neither loaded nor
       // reduced; rows wholly inside the operand load without a bounds check
       typename ChunkRegs<ELL, LOGTC, 0, B0>::type vv;
-#ifndef SBA_FIRST_HZ_SKIP
-#define SBA_FIRST_HZ_SKIP 1
-#endif
-#ifndef SBA_FIRST_NOCHECK
-#define SBA_FIRST_NOCHECK 0   // measured: unchecked loads with the zero-half skip were slower
-#endif
       const bool hz = SBA_FIRST_HZ_SKIP && SBA_ZERO_LEVEL && half_zero;
       const uint32_t rows = hz ? (1u << (ELL - 1)) : (1u << ELL);
       const bool inside = SBA_FIRST_NOCHECK && col0 + (rows - 1) * S + (TC - 1) < nl;   // uniform
@@ -576,25 +596,29 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
       if constexpr (EDGE && LC::T0 == 0) {
         // one chunk: registers in, registers out
         if (SBA_ZERO_LEVEL && half_zero)
-          tile_chunk_io<ELL, LOGTC, 0, B0, false, 1, CHUNK_REGS>(lds, pa.wn[q], 0, p, pinv, vv);
+          tile_chunk_io<ELL, LOGTC, 0, B0, false, 1, CHUNK_REGS, TW_EARLY, false, HV>(
+              lds, pa.wn[q], 0, p, pinv, vv);
         else
-          tile_chunk_io<ELL, LOGTC, 0, B0, false, 0, CHUNK_REGS>(lds, pa.wn[q], 0, p, pinv, vv);
+          tile_chunk_io<ELL, LOGTC, 0, B0, false, 0, CHUNK_REGS, TW_EARLY, false, HV>(
+              lds, pa.wn[q], 0, p, pinv, vv);
         using IO = ChunkIO<ELL, LOGTC, 0, B0>;
         g_u32 *out = spec + q * N + col0;
         const uint32_t t = IO::thr(S);
 #pragma unroll
         for (int m = 0; m < (1 << B0); ++m) (out + IO::uni(0, m, S))[t] = vv[0][m];
-        continue;
+        return;
       }
       if (SBA_ZERO_LEVEL && half_zero)
-        tile_chunk_io<ELL, LOGTC, 0, B0, false, 1, CHUNK_FROM_REGS>(lds, pa.wn[q], 0, p, pinv, vv);
+        tile_chunk_io<ELL, LOGTC, 0, B0, false, 1, CHUNK_FROM_REGS, TW_EARLY, false, HV>(
+            lds, pa.wn[q], 0, p, pinv, vv);
       else
-        tile_chunk_io<ELL, LOGTC, 0, B0, false, 0, CHUNK_FROM_REGS>(lds, pa.wn[q], 0, p, pinv, vv);
+        tile_chunk_io<ELL, LOGTC, 0, B0, false, 0, CHUNK_FROM_REGS, TW_EARLY, false, HV>(
+            lds, pa.wn[q], 0, p, pinv, vv);
       if constexpr (EDGE) {
-        tile_fwd_range<ELL, LOGTC, BMAX, B0, LC::T0>(lds, pa.wn[q], 0, p, pinv);
+        tile_fwd_range<ELL, LOGTC, BMAX, B0, LC::T0, HV>(lds, pa.wn[q], 0, p, pinv);
         typename ChunkRegs<ELL, LOGTC, LC::T0, LC::B>::type vl;
-        tile_chunk_io<ELL, LOGTC, LC::T0, LC::B, false, 0, CHUNK_TO_REGS>(lds, pa.wn[q], 0, p, pinv,
-                                                                       vl);
+        tile_chunk_io<ELL, LOGTC, LC::T0, LC::B, false, 0, CHUNK_TO_REGS, TW_EARLY, false, HV>(
+            lds, pa.wn[q], 0, p, pinv, vl);
         using IO = ChunkIO<ELL, LOGTC, LC::T0, LC::B>;
         g_u32 *out = spec + q * N + col0;
         const uint32_t t = IO::thr(S);
@@ -603,9 +627,9 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
         for (int gi = 0; gi < GL; ++gi)
 #pragma unroll
           for (int m = 0; m < (1 << LC::B); ++m) (out + IO::uni(gi, m, S))[t] = vl[gi][m];
-        continue;
+        return;
       }
-      tile_fwd<ELL, LOGTC, BMAX, B0>(lds, pa.wn[q], 0, p, pinv);
+      tile_fwd<ELL, LOGTC, BMAX, B0, 0, TW_EARLY, HV>(lds, pa.wn[q], 0, p, pinv);
     } else {
 #pragma unroll SBA_FIRST_LOAD_UNROLL
       for (int i = 0; i < E; ++i) {
@@ -613,7 +637,7 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
         gc_u64 *lp = limbs + uoff;
         lt[i * Pad<LOGTC>::words(NT)] = limb_residue(uoff + toff < nl ? lp[toff] : 0, p, pinv, r1);
       }
-      tile_fwd_first<ELL, LOGTC, BMAX>(lds, pa.wn[q], p, pinv, half_zero);
+      tile_fwd_first<ELL, LOGTC, BMAX, HV>(lds, pa.wn[q], p, pinv, half_zero);
     }
     __syncthreads();
     g_u32 *out = spec + q * N;
@@ -622,6 +646,10 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
       const uint32_t uoff = col0 + (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
       (out + uoff)[toff] = lt[i * Pad<LOGTC>::words(NT)];
     }
+  };
+#pragma unroll 1
+  for (uint32_t q = q0; q < q1; ++q) {
+    prime(q, BoolC<false>{});
   }
 }
 
@@ -657,79 +685,84 @@ __global__ __launch_bounds__(NT, INV ? SBA_INV_WAVES : SBA_WAVES) void k_pass(Pr
   using LC = LastChunk<ELL, LOGTC, BMAX>;
   constexpr bool EDGE =
       DIRECT && INV && SBA_COL_EDGE && ChunkIO<ELL, LOGTC, LC::T0, LC::B>::OK;
-  typename ChunkRegs<ELL, LOGTC, 0, B0>::type vv;
-  if constexpr (EDGE && LC::T0 == 0) {
-    // one chunk: registers in, registers out
-    using IO = ChunkIO<ELL, LOGTC, 0, B0>;
-    const uint32_t t = IO::thr(S);
+  // the [0, 2p) butterflies for every prime (HV = false)
+  auto body = [&](auto hv) {
+    constexpr bool HV = decltype(hv)::value;
+    typename ChunkRegs<ELL, LOGTC, 0, B0>::type vv;
+    if constexpr (EDGE && LC::T0 == 0) {
+      // one chunk: registers in, registers out
+      using IO = ChunkIO<ELL, LOGTC, 0, B0>;
+      const uint32_t t = IO::thr(S);
 #pragma unroll
-    for (int m = 0; m < (1 << B0); ++m) vv[0][m] = (buf + IO::uni(0, m, S))[t];
-    tile_chunk_io<ELL, LOGTC, 0, B0, true, 0, CHUNK_REGS>(lds, pa.wni[q], H, p, pinv, vv);
-  } else if constexpr (EDGE) {
-    using IO = ChunkIO<ELL, LOGTC, LC::T0, LC::B>;
-    typename ChunkRegs<ELL, LOGTC, LC::T0, LC::B>::type vf;
-    constexpr int GL = ChunkRegs<ELL, LOGTC, LC::T0, LC::B>::GPT;
-    const uint32_t t = IO::thr(S);
+      for (int m = 0; m < (1 << B0); ++m) vv[0][m] = (buf + IO::uni(0, m, S))[t];
+      tile_chunk_io<ELL, LOGTC, 0, B0, true, 0, CHUNK_REGS, TW_EARLY, false, HV>(lds, pa.wni[q], H, p, pinv, vv);
+    } else if constexpr (EDGE) {
+      using IO = ChunkIO<ELL, LOGTC, LC::T0, LC::B>;
+      typename ChunkRegs<ELL, LOGTC, LC::T0, LC::B>::type vf;
+      constexpr int GL = ChunkRegs<ELL, LOGTC, LC::T0, LC::B>::GPT;
+      const uint32_t t = IO::thr(S);
 #pragma unroll
-    for (int gi = 0; gi < GL; ++gi)
+      for (int gi = 0; gi < GL; ++gi)
 #pragma unroll
-      for (int m = 0; m < (1 << LC::B); ++m) vf[gi][m] = (buf + IO::uni(gi, m, S))[t];
-    tile_chunk_io<ELL, LOGTC, LC::T0, LC::B, true, 0, CHUNK_FROM_REGS>(lds, pa.wni[q], H, p, pinv,
-                                                                     vf);
-    tile_inv_upto<ELL, LOGTC, BMAX, B0, LC::T0>(lds, pa.wni[q], H, p, pinv);
-    tile_chunk_io<ELL, LOGTC, 0, B0, true, 0, CHUNK_TO_REGS>(lds, pa.wni[q], H, p, pinv, vv);
-  } else if (DIRECT && !INV) {
+        for (int m = 0; m < (1 << LC::B); ++m) vf[gi][m] = (buf + IO::uni(gi, m, S))[t];
+      tile_chunk_io<ELL, LOGTC, LC::T0, LC::B, true, 0, CHUNK_FROM_REGS, TW_EARLY, false, HV>(lds, pa.wni[q], H, p, pinv,
+                                                                       vf);
+      tile_inv_upto<ELL, LOGTC, BMAX, B0, LC::T0, HV>(lds, pa.wni[q], H, p, pinv);
+      tile_chunk_io<ELL, LOGTC, 0, B0, true, 0, CHUNK_TO_REGS, TW_EARLY, false, HV>(lds, pa.wni[q], H, p, pinv, vv);
+    } else if (DIRECT && !INV) {
 #pragma unroll SBA_PASS_LOAD_UNROLL
-    for (int i = 0; i < E; ++i) {
-      const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
-      vv[0][i] = (buf + uoff)[toff];
-    }
-    tile_chunk_io<ELL, LOGTC, 0, B0, false, 0, CHUNK_FROM_REGS>(lds, pa.wn[q], H, p, pinv, vv);
-    tile_fwd<ELL, LOGTC, BMAX, B0>(lds, pa.wn[q], H, p, pinv);
-  } else {
-#pragma unroll SBA_PASS_LOAD_UNROLL
-    for (int i = 0; i < E; ++i) {
-      const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
-      lt[i * Pad<LOGTC>::words(NT)] = (buf + uoff)[toff];
-    }
-    if (!INV) {
-      tile_fwd<ELL, LOGTC, BMAX, 0>(lds, pa.wn[q], H, p, pinv);
-    } else if (DIRECT) {
-      tile_inv_upto<ELL, LOGTC, BMAX, B0, ELL>(lds, pa.wni[q], H, p, pinv);
-      tile_chunk_io<ELL, LOGTC, 0, B0, true, 0, CHUNK_TO_REGS>(lds, pa.wni[q], H, p, pinv, vv);
+      for (int i = 0; i < E; ++i) {
+        const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+        vv[0][i] = (buf + uoff)[toff];
+      }
+      tile_chunk_io<ELL, LOGTC, 0, B0, false, 0, CHUNK_FROM_REGS, TW_EARLY, false, HV>(lds, pa.wn[q], H, p, pinv, vv);
+      tile_fwd<ELL, LOGTC, BMAX, B0, 0, TW_EARLY, HV>(lds, pa.wn[q], H, p, pinv);
     } else {
-      tile_inv<ELL, LOGTC, BMAX, 0>(lds, pa.wni[q], H, p, pinv);
-    }
-  }
-  // the inverse's element i: in registers (DIRECT) or in LDS
-  const bool inreg = DIRECT && INV;
-  if (!inreg) __syncthreads();
-  // the inverse's last levels (s0 == 0) finish the transform: scale here
-  const bool fin = INV && s0 == 0;
-  const uint32_t sc = pa.scale[q];
-  if (INV && acc.dst) {
-    // one chunk of a chunked product: coefficient x of the chunk lands at
-    // acc.off + x of the accumulator; x < acc.ovl adds to the previous
-    // chunk's tail, x >= acc.lim (beyond the chunk's product) is skipped
-    g_u32 *d = (g_u32 *)acc.dst + q * acc.stride + acc.off + base;
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
-      const uint32_t x = base + uoff + toff;
-      if (x < acc.lim) {
-        uint32_t v = mont_mul(inreg ? vv[0][i] : lt[i * Pad<LOGTC>::words(NT)], sc, p, pinv);
-        if (x < acc.ovl) v = red1(v, p) + red1((d + uoff)[toff], p);
-        (d + uoff)[toff] = v;
+#pragma unroll SBA_PASS_LOAD_UNROLL
+      for (int i = 0; i < E; ++i) {
+        const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+        lt[i * Pad<LOGTC>::words(NT)] = (buf + uoff)[toff];
+      }
+      if (!INV) {
+        tile_fwd<ELL, LOGTC, BMAX, 0, 0, TW_EARLY, HV>(lds, pa.wn[q], H, p, pinv);
+      } else if (DIRECT) {
+        tile_inv_upto<ELL, LOGTC, BMAX, B0, ELL, HV>(lds, pa.wni[q], H, p, pinv);
+        tile_chunk_io<ELL, LOGTC, 0, B0, true, 0, CHUNK_TO_REGS, TW_EARLY, false, HV>(lds, pa.wni[q], H, p, pinv, vv);
+      } else {
+        tile_inv<ELL, LOGTC, BMAX, 0, TW_EARLY, HV>(lds, pa.wni[q], H, p, pinv);
       }
     }
-    return;
-  }
+    // the inverse's element i: in registers (DIRECT) or in LDS
+    const bool inreg = DIRECT && INV;
+    if (!inreg) __syncthreads();
+    // the inverse's last levels (s0 == 0) finish the transform: scale here
+    const bool fin = INV && s0 == 0;
+    const uint32_t sc = pa.scale[q];
+    if (INV && acc.dst) {
+      // one chunk of a chunked product: coefficient x of the chunk lands at
+      // acc.off + x of the accumulator; x < acc.ovl adds to the previous
+      // chunk's tail, x >= acc.lim (beyond the chunk's product) is skipped
+      g_u32 *d = (g_u32 *)acc.dst + q * acc.stride + acc.off + base;
 #pragma unroll
-  for (int i = 0; i < E; ++i) {
-    const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
-    const uint32_t v = inreg ? vv[0][i] : lt[i * Pad<LOGTC>::words(NT)];
-    (buf + uoff)[toff] = fin ? mont_mul(v, sc, p, pinv) : v;
-  }
+      for (int i = 0; i < E; ++i) {
+        const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+        const uint32_t x = base + uoff + toff;
+        if (x < acc.lim) {
+          uint32_t v = mont_mul(inreg ? vv[0][i] : lt[i * Pad<LOGTC>::words(NT)], sc, p, pinv);
+          if (x < acc.ovl) v = red1(v, p) + red1((d + uoff)[toff], p);
+          (d + uoff)[toff] = v;
+        }
+      }
+      return;
+    }
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const uint32_t uoff = (((uint32_t)i * NT) >> LOGTC) * S + (((uint32_t)i * NT) & (TC - 1));
+      const uint32_t v = inreg ? vv[0][i] : lt[i * Pad<LOGTC>::words(NT)];
+      (buf + uoff)[toff] = fin ? mont_mul(v, sc, p, pinv) : v;
+    }
+  };
+  body(BoolC<false>{});
 }
 
 // Last forward levels of b + pointwise product with a's spectrum (or b's own
@@ -769,34 +802,97 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
   gc_u64 *bl = (gc_u64 *)jb.b + base;
   uint32_t *lt = lds + lpadt<0>(threadIdx.x);
   const uint32_t tid = threadIdx.x;
+  // REGS (as in k_fused2): the first chunk starts from the loaded registers
+  // and the inverse's last chunk ends in the stored ones; a square also
+  // keeps its last forward chunk in registers through the pointwise product
+  // into the inverse's first chunk (a multiply reads a's spectrum in the
+  // natural layout, so its product goes through LDS)
+  constexpr int B0 = ChunkPlan<ELL, 0, BMAX>::pick(0);
+  using LC = LastChunk<ELL, 0, BMAX>;
+  constexpr bool REGS = SBA_FUSED2_REGS && B0 == ChunkPlan<ELL, 0, BMAX>::LE && LC::T0 >= B0;
+  // the [0, 2p) butterflies for every prime (HV = false)
+  auto body = [&](auto hv) {
+    constexpr bool HV = decltype(hv)::value;
+    if constexpr (REGS) {
+      typedef typename ChunkRegs<ELL, 0, 0, B0>::type R0;
+      R0 v0;
 #pragma unroll
-  for (int i = 0; i < E; ++i) {
-    uint32_t v;
-    if (LIMB_IN) {
-      v = limb_residue(base + i * NT + tid < nb ? (bl + i * NT)[tid] : 0, p, pinv, pa.r1[q]);
+      for (int i = 0; i < E; ++i)
+        v0[0][i] = LIMB_IN ? limb_residue(base + i * NT + tid < nb ? (bl + i * NT)[tid] : 0, p,
+                                          pinv, pa.r1[q])
+                           : (buf + i * NT)[tid];
+      // LIMB_IN: one tile (H = 0) holds the whole transform of the padded
+      // limbs; a half-empty operand's first level is a copy
+      if (LIMB_IN && SBA_ZERO_LEVEL && nb <= N / 2)
+        tile_chunk_io<ELL, 0, 0, B0, false, 1, CHUNK_FROM_REGS, TW_EARLY, false, HV>(
+            lds, pa.wn[q], H, p, pinv, v0);
+      else
+        tile_chunk_io<ELL, 0, 0, B0, false, 0, CHUNK_FROM_REGS, TW_EARLY, false, HV>(
+            lds, pa.wn[q], H, p, pinv, v0);
+      tile_fwd_range<ELL, 0, BMAX, B0, LC::T0, HV>(lds, pa.wn[q], H, p, pinv);
+      if constexpr (SQR) {
+        typename ChunkRegs<ELL, 0, LC::T0, LC::B>::type bv;
+        constexpr int GL = ChunkRegs<ELL, 0, LC::T0, LC::B>::GPT;
+        tile_chunk_io<ELL, 0, LC::T0, LC::B, false, 0, CHUNK_TO_REGS, TW_EARLY, false, HV>(
+            lds, pa.wn[q], H, p, pinv, bv);
+#pragma unroll
+        for (int g = 0; g < GL; ++g)
+#pragma unroll
+          for (int m = 0; m < (1 << LC::B); ++m)
+            bv[g][m] = mont_mul(bv[g][m], canon<HV>(bv[g][m], p), p, pinv);
+        __syncthreads();               // the last LDS reads before the inverse's writes
+        tile_chunk_io<ELL, 0, LC::T0, LC::B, true, 0, CHUNK_FROM_REGS, TW_EARLY, false, HV>(
+            lds, pa.wni[q], H, p, pinv, bv);
+        tile_inv_upto<ELL, 0, BMAX, B0, LC::T0, HV>(lds, pa.wni[q], H, p, pinv);
+      } else {
+        tile_chunk<ELL, 0, LC::T0, LC::B, false, 0, TW_EARLY, HV>(lds, pa.wn[q], H, p, pinv);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+          const uint32_t v = lt[i * Pad<0>::words(NT)];
+          lt[i * Pad<0>::words(NT)] = mont_mul(v, canon<HV>((as + i * NT)[tid], p), p, pinv);
+        }
+        tile_inv_upto<ELL, 0, BMAX, B0, ELL, HV>(lds, pa.wni[q], H, p, pinv);
+      }
+      R0 vo;
+      tile_chunk_io<ELL, 0, 0, B0, true, 0, CHUNK_TO_REGS, TW_EARLY, false, HV>(lds, pa.wni[q], H,
+                                                                               p, pinv, vo);
+      // LIMB_IN: the single tile is the whole transform, so the inverse ends here
+#pragma unroll
+      for (int i = 0; i < E; ++i)
+        (buf + i * NT)[tid] = LIMB_IN ? mont_mul(vo[0][i], pa.scale[q], p, pinv) : vo[0][i];
     } else {
-      v = (buf + i * NT)[tid];
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        uint32_t v;
+        if (LIMB_IN) {
+          v = limb_residue(base + i * NT + tid < nb ? (bl + i * NT)[tid] : 0, p, pinv, pa.r1[q]);
+        } else {
+          v = (buf + i * NT)[tid];
+        }
+        lt[i * Pad<0>::words(NT)] = v;
+      }
+      // LIMB_IN: one tile (H = 0) holds the whole transform of the padded limbs
+      if (LIMB_IN) tile_fwd_first<ELL, 0, BMAX, HV>(lds, pa.wn[q], p, pinv, nb <= N / 2);
+      else tile_fwd<ELL, 0, BMAX, 0, 0, TW_EARLY, HV>(lds, pa.wn[q], H, p, pinv);
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const uint32_t v = lt[i * Pad<0>::words(NT)];
+        const uint32_t o = SQR ? canon<HV>(v, p) : canon<HV>((as + i * NT)[tid], p);
+        lt[i * Pad<0>::words(NT)] = mont_mul(v, o, p, pinv);
+      }
+      tile_inv<ELL, 0, BMAX, 0, TW_EARLY, HV>(lds, pa.wni[q], H, p, pinv);
+      __syncthreads();
+      // LIMB_IN: the single tile is the whole transform, so the inverse ends here
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const uint32_t v = lt[i * Pad<0>::words(NT)];
+        (buf + i * NT)[tid] = LIMB_IN ? mont_mul(v, pa.scale[q], p, pinv) : v;
+      }
     }
-    lt[i * Pad<0>::words(NT)] = v;
-  }
-  // LIMB_IN: one tile (H = 0) holds the whole transform of the padded limbs
-  if (LIMB_IN) tile_fwd_first<ELL, 0, BMAX>(lds, pa.wn[q], p, pinv, nb <= N / 2);
-  else tile_fwd<ELL, 0, BMAX, 0>(lds, pa.wn[q], H, p, pinv);
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < E; ++i) {
-    const uint32_t v = lt[i * Pad<0>::words(NT)];
-    const uint32_t o = SQR ? red1(v, p) : red1((as + i * NT)[tid], p);
-    lt[i * Pad<0>::words(NT)] = mont_mul(v, o, p, pinv);
-  }
-  tile_inv<ELL, 0, BMAX, 0>(lds, pa.wni[q], H, p, pinv);
-  __syncthreads();
-  // LIMB_IN: the single tile is the whole transform, so the inverse ends here
-#pragma unroll
-  for (int i = 0; i < E; ++i) {
-    const uint32_t v = lt[i * Pad<0>::words(NT)];
-    (buf + i * NT)[tid] = LIMB_IN ? mont_mul(v, pa.scale[q], p, pinv) : v;
-  }
+  };
+  body(BoolC<false>{});
 }
 
 // Multiply, two-pass plans: the last forward levels of a AND of b on the same
@@ -825,25 +921,79 @@ __global__ __launch_bounds__(NT, BMAX == 5 ? SBA_FUSED2_WAVES : SBA_WAVES) void 
   gc_u32 *as = (gc_u32 *)jb.aspec + q * pstride + base;
   uint32_t *lt = lds + lpadt<0>(threadIdx.x);
   const uint32_t tid = threadIdx.x;
-  uint32_t av[E];
+  // REGS: the transforms' first chunk starts from the loaded registers (its
+  // groups are a thread's elements tid + i NT), the forward's last chunk
+  // leaves its groups in registers, where the pointwise product and the
+  // inverse's first chunk (the same groups) take them, and the inverse's last
+  // chunk ends in the registers that are stored: two LDS round trips and two
+  // barriers fewer per transform than staging every chunk through LDS
+  constexpr int B0 = ChunkPlan<ELL, 0, BMAX>::pick(0);
+  using LC = LastChunk<ELL, 0, BMAX>;
+  constexpr bool REGS =
+      SBA_FUSED2_REGS && B0 == ChunkPlan<ELL, 0, BMAX>::LE && LC::T0 >= B0;
+  // HV: the prime takes Harvey's butterflies
+  auto body = [&](auto hv) {
+    constexpr bool HV = decltype(hv)::value;
+    if constexpr (REGS) {
+      typedef typename ChunkRegs<ELL, 0, 0, B0>::type R0;
+      typedef typename ChunkRegs<ELL, 0, LC::T0, LC::B>::type RL;
+      constexpr int GL = ChunkRegs<ELL, 0, LC::T0, LC::B>::GPT;
+      // one operand's last forward levels: HBM -> registers -> ... -> registers
+      auto fwd = [&](gc_u32 *src, RL &out) {
+        R0 v0;
 #pragma unroll
-  for (int i = 0; i < E; ++i) lt[i * Pad<0>::words(NT)] = (as + i * NT)[tid];
-  tile_fwd<ELL, 0, BMAX, 0, 0, F2TE>(lds, pa.wn[q], H, p, pinv);
-  __syncthreads();
-  // each thread takes back only its own elements, then refills them with b
+        for (int i = 0; i < E; ++i) v0[0][i] = (src + i * NT)[tid];
+        tile_chunk_io<ELL, 0, 0, B0, false, 0, CHUNK_FROM_REGS, F2TE, false, HV>(
+            lds, pa.wn[q], H, p, pinv, v0);
+        tile_fwd_range<ELL, 0, BMAX, B0, LC::T0, HV, F2TE>(lds, pa.wn[q], H, p, pinv);
+        tile_chunk_io<ELL, 0, LC::T0, LC::B, false, 0, CHUNK_TO_REGS, F2TE, false, HV>(
+            lds, pa.wn[q], H, p, pinv, out);
+      };
+      RL av, bv;
+      fwd(as, av);
 #pragma unroll
-  for (int i = 0; i < E; ++i) av[i] = red1(lt[i * Pad<0>::words(NT)], p);
+      for (int g = 0; g < GL; ++g)
 #pragma unroll
-  for (int i = 0; i < E; ++i) lt[i * Pad<0>::words(NT)] = (buf + i * NT)[tid];
-  tile_fwd<ELL, 0, BMAX, 0, 0, F2TE>(lds, pa.wn[q], H, p, pinv);
-  __syncthreads();
+        for (int m = 0; m < (1 << LC::B); ++m) av[g][m] = canon<HV>(av[g][m], p);
+      __syncthreads();                 // a's last LDS reads before b's first writes
+      fwd(buf, bv);
 #pragma unroll
-  for (int i = 0; i < E; ++i)
-    lt[i * Pad<0>::words(NT)] = mont_mul(lt[i * Pad<0>::words(NT)], av[i], p, pinv);
-  tile_inv<ELL, 0, BMAX, 0, F2TE>(lds, pa.wni[q], H, p, pinv);
-  __syncthreads();
+      for (int g = 0; g < GL; ++g)
 #pragma unroll
-  for (int i = 0; i < E; ++i) (buf + i * NT)[tid] = lt[i * Pad<0>::words(NT)];
+        for (int m = 0; m < (1 << LC::B); ++m) bv[g][m] = mont_mul(bv[g][m], av[g][m], p, pinv);
+      __syncthreads();                 // b's last LDS reads before the inverse's writes
+      tile_chunk_io<ELL, 0, LC::T0, LC::B, true, 0, CHUNK_FROM_REGS, F2TE, false, HV>(
+          lds, pa.wni[q], H, p, pinv, bv);
+      tile_inv_upto<ELL, 0, BMAX, B0, LC::T0, HV, F2TE>(lds, pa.wni[q], H, p, pinv);
+      R0 vo;
+      tile_chunk_io<ELL, 0, 0, B0, true, 0, CHUNK_TO_REGS, F2TE, false, HV>(lds, pa.wni[q], H,
+                                                                           p, pinv, vo);
+#pragma unroll
+      for (int i = 0; i < E; ++i) (buf + i * NT)[tid] = vo[0][i];
+    } else {
+      uint32_t av[E];
+#pragma unroll
+      for (int i = 0; i < E; ++i) lt[i * Pad<0>::words(NT)] = (as + i * NT)[tid];
+      tile_fwd<ELL, 0, BMAX, 0, 0, F2TE, HV>(lds, pa.wn[q], H, p, pinv);
+      __syncthreads();
+      // each thread takes back only its own elements, then refills them with b
+#pragma unroll
+      for (int i = 0; i < E; ++i) av[i] = canon<HV>(lt[i * Pad<0>::words(NT)], p);
+#pragma unroll
+      for (int i = 0; i < E; ++i) lt[i * Pad<0>::words(NT)] = (buf + i * NT)[tid];
+      tile_fwd<ELL, 0, BMAX, 0, 0, F2TE, HV>(lds, pa.wn[q], H, p, pinv);
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < E; ++i)
+        lt[i * Pad<0>::words(NT)] = mont_mul(lt[i * Pad<0>::words(NT)], av[i], p, pinv);
+      tile_inv<ELL, 0, BMAX, 0, F2TE, HV>(lds, pa.wni[q], H, p, pinv);
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < E; ++i) (buf + i * NT)[tid] = lt[i * Pad<0>::words(NT)];
+    }
+  };
+  if ((uint32_t)q < pa.nh) body(BoolC<true>{});
+  else body(BoolC<false>{});
 }
 
 // ---------------------------------------------------------------------------

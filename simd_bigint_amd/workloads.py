@@ -122,8 +122,15 @@ def from_int(v: int, n: int) -> np.ndarray:
 
 
 def all_ones_product(an: int, bn: int) -> np.ndarray:
-    """(2^64a - 1)(2^64b - 1) = 2^64(a+b) - 2^64a - 2^64b + 1, as limbs."""
-    return from_int(((1 << (64 * an)) - 1) * ((1 << (64 * bn)) - 1), an + bn)
+    """(B^a - 1)(B^b - 1) = B^(a+b) - B^a - B^b + 1 with B = 2^64, as limbs
+    (a >= b, either order taken): limb 0 is 1, limbs 1 .. b-1 are 0, limbs
+    b .. a-1 are B - 1, limb a is B - 2 and limbs a+1 .. a+b-1 are B - 1."""
+    a, b = max(an, bn), min(an, bn)
+    r = np.full(a + b, np.uint64(2**64 - 1), dtype=np.uint64)
+    r[0] = 1
+    r[1:b] = 0
+    r[a] = np.uint64(2**64 - 2)
+    return r
 
 
 def batch_workload(n: int = 4096, seed: int = 4, lo: int = 64, hi: int = 65536,

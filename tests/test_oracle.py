@@ -97,6 +97,9 @@ def test_schoolbook_and_nt_vs_python_ints():
 
 
 def test_all_ones_closed_form():
+    # the limb pattern against Python integers, then the oracle against it
+    for an, bn in ((1, 1), (2, 1), (1, 3), (7, 7), (40, 13), (13, 40), (1000, 999)):
+        assert to_int(all_ones_product(an, bn)) == ((1 << (64 * an)) - 1) * ((1 << (64 * bn)) - 1)
     for an, bn in ((1, 1), (17, 5), (4096, 4096), (3000, 70000)):
         a = np.full(an, 2**64 - 1, np.uint64)
         b = np.full(bn, 2**64 - 1, np.uint64)
