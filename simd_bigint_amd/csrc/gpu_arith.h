@@ -76,9 +76,27 @@ __device__ __forceinline__ void bfly_inv_h(uint32_t &x, uint32_t &y, uint32_t wi
   y = mont_mul(d, wi, p, pinv);
 }
 
+// which butterfly forms a pass runs: the [0, 2p) forms (HV_W), Harvey's
+// (HV_H, only for p < 2^30), or Harvey's exactly where the prime allows them
+// (HV_AUTO: a uniform branch on the block's prime)
+#define HV_W 0
+#define HV_H 1
+#define HV_AUTO 2
+__device__ __forceinline__ bool harvey_prime(uint32_t p) { return p < (1u << 30); }
+
+// a stored forward value as the input of a pass running the forms HV: the
+// forward values of a prime below 2^30 may lie in [0, 4p) (a pass with
+// Harvey's forms wrote them); the [0, 2p) forms need them folded once
+template <int HV>
+__device__ __forceinline__ uint32_t fwd_in(uint32_t x, uint32_t p) {
+  return (HV == HV_W && harvey_prime(p)) ? red2(x, p + p) : x;
+}
+
 // a forward output as the canonical Montgomery operand of the pointwise
-// product: [0, 4p) (HV) or [0, 2p) -> [0, p)
-template <bool HV>
+// product: [0, 4p) (Harvey's forms) or [0, 2p) -> [0, p). Under HV_AUTO a
+// prime below 2^30 had Harvey's forms.
+template <int HV>
 __device__ __forceinline__ uint32_t canon(uint32_t x, uint32_t p) {
-  return HV ? red1(red2(x, p + p), p) : red1(x, p);
+  const bool h = HV == HV_H || (HV == HV_AUTO && harvey_prime(p));
+  return h ? red1(red2(x, p + p), p) : red1(x, p);
 }

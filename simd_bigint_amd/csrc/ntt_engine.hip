@@ -525,12 +525,11 @@ __global__ __launch_bounds__(NT) void k_twiddles(TwArgs ta, uint32_t *__restrict
 #define NSETS 2
 struct PrimeSetDef {
   uint32_t p[SBA_K];
-  uint32_t nh;                         // leading primes below 2^30
   int max_lg;                          // largest power-of-two transform
 };
 static const PrimeSetDef kSets[NSETS] = {
-    {{1107296257u, 1711276033u, 1811939329u, 2013265921u, 2113929217u}, 0, SBA_MAX_LG},
-    {{880803841u, 897581057u, 998244353u, 2113929217u, 2130706433u}, 3, 23},
+    {{1107296257u, 1711276033u, 1811939329u, 2013265921u, 2113929217u}, SBA_MAX_LG},
+    {{880803841u, 897581057u, 998244353u, 2113929217u, 2130706433u}, 23},
 };
 static const uint32_t *const kPrimes = kSets[SET_W].p;   // the band's (CRT cap) set
 
@@ -608,7 +607,6 @@ static void set_init(SetCtx &c, const PrimeSetDef &d) {
     c.ga_base.p[q] = (uint32_t)p;
     c.ga_base.pinv[q] = 0u - inv;
   }
-  c.pa.nh = d.nh;
   for (int q = 0; q < SBA_K; ++q) {
     // length-3 DFT constants: w = g^((p-1)/3) a cube root of unity (only
     // where 3 | p - 1: a set without them never runs a radix-3 class)

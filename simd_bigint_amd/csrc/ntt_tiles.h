@@ -19,11 +19,6 @@ struct PrimeArgs {
   // (Nt the transform length; see the host's prime_args)
   uint32_t scale[SBA_K];
   uint32_t inv2[SBA_K];                          // 1/2 (Montgomery, canonical)
-  // the set's leading primes q < nh are below 2^30: the merged last pass
-  // gives them Harvey's butterflies (forward values lazy in [0, 4p)). Every
-  // other kernel runs the [0, 2p) forms for all primes (valid for any p <
-  // 2^31), so spectra cross kernel boundaries in [0, 2p) either way.
-  uint32_t nh;
   const uint32_t *wn[SBA_K];
   const uint32_t *wni[SBA_K];
 };
@@ -58,6 +53,20 @@ struct GarnerArgs {
 // first pass's limb loads, fixed by measurement
 #ifndef SBA_FUSED2_WAVES
 #define SBA_FUSED2_WAVES 3
+#endif
+// butterfly forms of the passes other than the merged last pass: HV_AUTO
+// (Harvey's for primes below 2^30) or HV_W
+#ifndef SBA_COL_HV
+#define SBA_COL_HV HV_AUTO
+#endif
+// butterfly forms of k_fused (squares, chunks, single-tile products)
+#ifndef SBA_FUSED_HV
+#define SBA_FUSED_HV HV_W
+#endif
+// 1: the merged last pass instantiates its whole body once per form and
+// branches once; 0: one body, a branch per chunk (HV_AUTO)
+#ifndef SBA_FUSED2_SPLIT
+#define SBA_FUSED2_SPLIT 1
 #endif
 #ifndef SBA_FUSED2_REGS
 #define SBA_FUSED2_REGS 1    // merged last pass: both ends and the pointwise product in registers
@@ -121,8 +130,9 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
   return (b & 7) * (n >> 3) + (b >> 3);
 }
 
-// a compile-time flag passed to generic lambdas (the per-prime bodies of
-// the pass kernels, instantiated once per butterfly form)
+
+// a compile-time flag passed to generic lambdas (code instantiated once per
+// butterfly form)
 template <bool V> struct BoolC { static constexpr bool value = V; };
 
 template <int X> struct ILog2 { static constexpr int v = 1 + ILog2<X / 2>::v; };
@@ -188,9 +198,11 @@ template <int ELL, int LOGTC, int T0, int B> struct ChunkRegs {
 // GUARD (inverse only, column passes: H = 0): a butterfly runs only when its
 // block of the column transform lies inside rows [0, gM) -- the complete
 // blocks of a truncated inverse; the others leave their values as they are.
-// HV: the prime is below 2^30 (Harvey's butterflies, gpu_arith.h).
+// HV: the butterfly forms (gpu_arith.h): HV_W the [0, 2p) forms, HV_H
+// Harvey's (p < 2^30), HV_AUTO Harvey's exactly when the block's prime is
+// below 2^30 (one uniform branch around the chunk's arithmetic).
 template <int ELL, int LOGTC, int T0, int B, bool INV, int ZL = 0, int IO = CHUNK_LDS,
-          int TE = TW_EARLY, bool GUARD = false, bool HV = false>
+          int TE = TW_EARLY, bool GUARD = false, int HV = HV_W>
 __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__restrict__ tw,
                                               uint32_t H, uint32_t p, uint32_t pinv,
                                               typename ChunkRegs<ELL, LOGTC, T0, B>::type &vv,
@@ -253,46 +265,56 @@ __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__r
       v[m] = IN_REGS
                  ? vv[gi][m]
                  : (G::LINEAR ? l0[m * G::DP] : lds[lpadt<LOGTC>(i0 + ((uint32_t)m << G::LOGD))]);
-    if (!INV) {
+    auto flies = [&](auto hc) {
+      constexpr bool HVC = decltype(hc)::value;
+      if (!INV) {
 #pragma unroll
-      for (int u = 0; u < B; ++u) {
-        const int half = 1 << (B - u - 1);
-        if (T0 + u < ZL) {
+        for (int u = 0; u < B; ++u) {
+          const int half = 1 << (B - u - 1);
+          if (T0 + u < ZL) {
 #pragma unroll
-          for (int m = 0; m < (1 << B); ++m)
-            if ((m >> (B - u - 1)) & 1) v[m] = v[m - half];
-          continue;
-        }
+            for (int m = 0; m < (1 << B); ++m)
+              if ((m >> (B - u - 1)) & 1) v[m] = v[m - half];
+            continue;
+          }
 #pragma unroll
-        for (int m2 = 0; m2 < (1 << u); ++m2) {
-          const uint32_t w = EARLY ? tv[gi][(1 << u) - 1 + m2] : tw[(hj << u) | (uint32_t)m2];
+          for (int m2 = 0; m2 < (1 << u); ++m2) {
+            const uint32_t w = EARLY ? tv[gi][(1 << u) - 1 + m2] : tw[(hj << u) | (uint32_t)m2];
 #pragma unroll
-          for (int r = 0; r < half; ++r) {
-            const int m = (m2 << (B - u)) | r;
-            if (HV) bfly_fwd_h(v[m], v[m + half], w, p, pinv);
-            else bfly_fwd(v[m], v[m + half], w, p, pinv);
+            for (int r = 0; r < half; ++r) {
+              const int m = (m2 << (B - u)) | r;
+              if (HVC) bfly_fwd_h(v[m], v[m + half], w, p, pinv);
+              else bfly_fwd(v[m], v[m + half], w, p, pinv);
+            }
           }
         }
-      }
-    } else {
+      } else {
 #pragma unroll
-      for (int u = B - 1; u >= 0; --u) {
-        const int half = 1 << (B - u - 1);
+        for (int u = B - 1; u >= 0; --u) {
+          const int half = 1 << (B - u - 1);
 #pragma unroll
-        for (int m2 = 0; m2 < (1 << u); ++m2) {
-          const uint32_t w = EARLY ? tv[gi][(1 << u) - 1 + m2] : tw[(hj << u) | (uint32_t)m2];
-          // the block of this sub-node: rows [b, b + 1) << (ELL - T0 - u)
-          const bool run = !GUARD || (((((hb << u) | (uint32_t)m2) + 1u) << (ELL - T0 - u)) <= gM);
+          for (int m2 = 0; m2 < (1 << u); ++m2) {
+            const uint32_t w = EARLY ? tv[gi][(1 << u) - 1 + m2] : tw[(hj << u) | (uint32_t)m2];
+            // the block of this sub-node: rows [b, b + 1) << (ELL - T0 - u)
+            const bool run =
+                !GUARD || (((((hb << u) | (uint32_t)m2) + 1u) << (ELL - T0 - u)) <= gM);
 #pragma unroll
-          for (int r = 0; r < half; ++r) {
-            const int m = (m2 << (B - u)) | r;
-            if (run) {
-              if (HV) bfly_inv_h(v[m], v[m + half], w, p, pinv);
-              else bfly_inv(v[m], v[m + half], w, p, pinv);
+            for (int r = 0; r < half; ++r) {
+              const int m = (m2 << (B - u)) | r;
+              if (run) {
+                if (HVC) bfly_inv_h(v[m], v[m + half], w, p, pinv);
+                else bfly_inv(v[m], v[m + half], w, p, pinv);
+              }
             }
           }
         }
       }
+    };
+    if constexpr (HV == HV_AUTO) {
+      if (harvey_prime(p)) flies(BoolC<true>{});
+      else flies(BoolC<false>{});
+    } else {
+      flies(BoolC<HV == HV_H>{});
     }
 #pragma unroll
     for (int m = 0; m < (1 << B); ++m) {
@@ -304,7 +326,7 @@ __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__r
 }
 
 template <int ELL, int LOGTC, int T0, int B, bool INV, int ZL = 0, int TE = TW_EARLY,
-          bool HV = false>
+          int HV = HV_W>
 __device__ __forceinline__ void tile_chunk(uint32_t *lds, const uint32_t *__restrict__ tw,
                                            uint32_t H, uint32_t p, uint32_t pinv) {
   typename ChunkRegs<ELL, LOGTC, T0, B>::type unused;
@@ -330,7 +352,7 @@ template <int ELL, int LOGTC, int BMAX> struct ChunkPlan {
   }
 };
 
-template <int ELL, int LOGTC, int BMAX, int T0, int ZL = 0, int TE = TW_EARLY, bool HV = false>
+template <int ELL, int LOGTC, int BMAX, int T0, int ZL = 0, int TE = TW_EARLY, int HV = HV_W>
 __device__ __forceinline__ void tile_fwd(uint32_t *lds, const uint32_t *tw, uint32_t H,
                                          uint32_t p, uint32_t pinv) {
   if constexpr (T0 < ELL) {
@@ -341,7 +363,7 @@ __device__ __forceinline__ void tile_fwd(uint32_t *lds, const uint32_t *tw, uint
 }
 
 // inverse chunks of the levels from T0 below TEND (highest first), LDS to LDS
-template <int ELL, int LOGTC, int BMAX, int T0, int TEND, bool HV = false, int TE = TW_EARLY>
+template <int ELL, int LOGTC, int BMAX, int T0, int TEND, int HV = HV_W, int TE = TW_EARLY>
 __device__ __forceinline__ void tile_inv_upto(uint32_t *lds, const uint32_t *tw, uint32_t H,
                                               uint32_t p, uint32_t pinv) {
   if constexpr (T0 < TEND) {
@@ -352,7 +374,7 @@ __device__ __forceinline__ void tile_inv_upto(uint32_t *lds, const uint32_t *tw,
 }
 
 // forward chunks from T0 up to (not including) level TEND, LDS to LDS
-template <int ELL, int LOGTC, int BMAX, int T0, int TEND, bool HV = false, int TE = TW_EARLY>
+template <int ELL, int LOGTC, int BMAX, int T0, int TEND, int HV = HV_W, int TE = TW_EARLY>
 __device__ __forceinline__ void tile_fwd_range(uint32_t *lds, const uint32_t *tw, uint32_t H,
                                                uint32_t p, uint32_t pinv) {
   if constexpr (T0 < TEND) {
@@ -402,7 +424,7 @@ template <int ELL, int LOGTC, int T0, int B> struct ChunkIO {
 // the forward levels of a first pass, with its first level a copy when the
 // operand fills at most half of the transform (SBA_ZERO_LEVEL; one uniform
 // branch per block, each side straight-line code)
-template <int ELL, int LOGTC, int BMAX, bool HV = false>
+template <int ELL, int LOGTC, int BMAX, int HV = HV_W>
 __device__ __forceinline__ void tile_fwd_first(uint32_t *lds, const uint32_t *tw, uint32_t p,
                                                uint32_t pinv, bool half_zero) {
   if (SBA_ZERO_LEVEL && half_zero) tile_fwd<ELL, LOGTC, BMAX, 0, 1, TW_EARLY, HV>(lds, tw, 0, p, pinv);
@@ -424,7 +446,7 @@ __device__ __forceinline__ void tile_inv_guard(uint32_t *lds, const uint32_t *tw
   }
 }
 
-template <int ELL, int LOGTC, int BMAX, int T0, int TE = TW_EARLY, bool HV = false>
+template <int ELL, int LOGTC, int BMAX, int T0, int TE = TW_EARLY, int HV = HV_W>
 __device__ __forceinline__ void tile_inv(uint32_t *lds, const uint32_t *tw, uint32_t H,
                                          uint32_t p, uint32_t pinv) {
   if constexpr (T0 < ELL) {
@@ -568,9 +590,9 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
   // columns keep 32-byte row segments in that layout too)
   using LC = LastChunk<ELL, LOGTC, BMAX>;
   constexpr bool EDGE = DIRECT && SBA_COL_EDGE && ChunkIO<ELL, LOGTC, LC::T0, LC::B>::OK;
-  // one prime's residues and levels; HV: the prime takes Harvey's butterflies
-  auto prime = [&](uint32_t q, auto hv) {
-    constexpr bool HV = decltype(hv)::value;
+  // one prime's residues and levels (butterfly forms chosen by the prime)
+  auto prime = [&](uint32_t q) {
+    constexpr int HV = SBA_COL_HV;
     const uint32_t p = pa.p[q], pinv = pa.pinv[q], r1 = pa.r1[q];
     __syncthreads();
     // the tile's limbs are re-read per prime (cache-resident after the first)
@@ -649,7 +671,7 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
   };
 #pragma unroll 1
   for (uint32_t q = q0; q < q1; ++q) {
-    prime(q, BoolC<false>{});
+    prime(q);
   }
 }
 
@@ -685,9 +707,9 @@ __global__ __launch_bounds__(NT, INV ? SBA_INV_WAVES : SBA_WAVES) void k_pass(Pr
   using LC = LastChunk<ELL, LOGTC, BMAX>;
   constexpr bool EDGE =
       DIRECT && INV && SBA_COL_EDGE && ChunkIO<ELL, LOGTC, LC::T0, LC::B>::OK;
-  // the [0, 2p) butterflies for every prime (HV = false)
-  auto body = [&](auto hv) {
-    constexpr bool HV = decltype(hv)::value;
+  // butterfly forms chosen by the prime
+  auto body = [&]() {
+    constexpr int HV = SBA_COL_HV;
     typename ChunkRegs<ELL, LOGTC, 0, B0>::type vv;
     if constexpr (EDGE && LC::T0 == 0) {
       // one chunk: registers in, registers out
@@ -762,7 +784,7 @@ __global__ __launch_bounds__(NT, INV ? SBA_INV_WAVES : SBA_WAVES) void k_pass(Pr
       (buf + uoff)[toff] = fin ? mont_mul(v, sc, p, pinv) : v;
     }
   };
-  body(BoolC<false>{});
+  body();
 }
 
 // Last forward levels of b + pointwise product with a's spectrum (or b's own
@@ -810,9 +832,12 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
   constexpr int B0 = ChunkPlan<ELL, 0, BMAX>::pick(0);
   using LC = LastChunk<ELL, 0, BMAX>;
   constexpr bool REGS = SBA_FUSED2_REGS && B0 == ChunkPlan<ELL, 0, BMAX>::LE && LC::T0 >= B0;
-  // the [0, 2p) butterflies for every prime (HV = false)
-  auto body = [&](auto hv) {
-    constexpr bool HV = decltype(hv)::value;
+  // butterfly forms: SBA_FUSED_HV (the [0, 2p) forms by default: choosing
+  // by the prime needs more registers than 3 waves per SIMD leave). Stored
+  // forward values (b's, and a's read for the product) may come from a pass
+  // with Harvey's forms: fwd_in and canon<HV_AUTO> take either range.
+  auto body = [&]() {
+    constexpr int HV = SBA_FUSED_HV;
     if constexpr (REGS) {
       typedef typename ChunkRegs<ELL, 0, 0, B0>::type R0;
       R0 v0;
@@ -820,7 +845,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
       for (int i = 0; i < E; ++i)
         v0[0][i] = LIMB_IN ? limb_residue(base + i * NT + tid < nb ? (bl + i * NT)[tid] : 0, p,
                                           pinv, pa.r1[q])
-                           : (buf + i * NT)[tid];
+                           : fwd_in<HV>((buf + i * NT)[tid], p);
       // LIMB_IN: one tile (H = 0) holds the whole transform of the padded
       // limbs; a half-empty operand's first level is a copy
       if (LIMB_IN && SBA_ZERO_LEVEL && nb <= N / 2)
@@ -850,7 +875,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
 #pragma unroll
         for (int i = 0; i < E; ++i) {
           const uint32_t v = lt[i * Pad<0>::words(NT)];
-          lt[i * Pad<0>::words(NT)] = mont_mul(v, canon<HV>((as + i * NT)[tid], p), p, pinv);
+          lt[i * Pad<0>::words(NT)] = mont_mul(v, canon<HV_AUTO>((as + i * NT)[tid], p), p, pinv);
         }
         tile_inv_upto<ELL, 0, BMAX, B0, ELL, HV>(lds, pa.wni[q], H, p, pinv);
       }
@@ -868,7 +893,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
         if (LIMB_IN) {
           v = limb_residue(base + i * NT + tid < nb ? (bl + i * NT)[tid] : 0, p, pinv, pa.r1[q]);
         } else {
-          v = (buf + i * NT)[tid];
+          v = fwd_in<HV>((buf + i * NT)[tid], p);
         }
         lt[i * Pad<0>::words(NT)] = v;
       }
@@ -879,7 +904,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         const uint32_t v = lt[i * Pad<0>::words(NT)];
-        const uint32_t o = SQR ? canon<HV>(v, p) : canon<HV>((as + i * NT)[tid], p);
+        const uint32_t o = SQR ? canon<HV>(v, p) : canon<HV_AUTO>((as + i * NT)[tid], p);
         lt[i * Pad<0>::words(NT)] = mont_mul(v, o, p, pinv);
       }
       tile_inv<ELL, 0, BMAX, 0, TW_EARLY, HV>(lds, pa.wni[q], H, p, pinv);
@@ -892,7 +917,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
       }
     }
   };
-  body(BoolC<false>{});
+  body();
 }
 
 // Multiply, two-pass plans: the last forward levels of a AND of b on the same
@@ -931,9 +956,10 @@ __global__ __launch_bounds__(NT, BMAX == 5 ? SBA_FUSED2_WAVES : SBA_WAVES) void 
   using LC = LastChunk<ELL, 0, BMAX>;
   constexpr bool REGS =
       SBA_FUSED2_REGS && B0 == ChunkPlan<ELL, 0, BMAX>::LE && LC::T0 >= B0;
-  // HV: the prime takes Harvey's butterflies
+  // HV: the butterfly forms (the whole body instantiated per form, or one
+  // body choosing by the prime chunk by chunk)
   auto body = [&](auto hv) {
-    constexpr bool HV = decltype(hv)::value;
+    constexpr int HV = SBA_FUSED2_SPLIT ? (decltype(hv)::value ? HV_H : HV_W) : HV_AUTO;
     if constexpr (REGS) {
       typedef typename ChunkRegs<ELL, 0, 0, B0>::type R0;
       typedef typename ChunkRegs<ELL, 0, LC::T0, LC::B>::type RL;
@@ -992,7 +1018,7 @@ __global__ __launch_bounds__(NT, BMAX == 5 ? SBA_FUSED2_WAVES : SBA_WAVES) void 
       for (int i = 0; i < E; ++i) (buf + i * NT)[tid] = lt[i * Pad<0>::words(NT)];
     }
   };
-  if ((uint32_t)q < pa.nh) body(BoolC<true>{});
+  if (SBA_FUSED2_SPLIT && harvey_prime(p)) body(BoolC<true>{});
   else body(BoolC<false>{});
 }
 
