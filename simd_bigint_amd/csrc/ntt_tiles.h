@@ -131,6 +131,31 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
 }
 
 
+// 1: uniform twiddles come through scalar loads
+#ifndef SBA_TW_SCALAR
+#define SBA_TW_SCALAR 1
+#endif
+
+// A workgroup barrier that orders LDS only: every barrier of the tile
+// kernels separates LDS writes from other threads' LDS reads, while each
+// thread's global loads and stores concern only itself. __syncthreads()
+// would also wait for all outstanding global accesses (its fence covers
+// every address space), which drains loads issued ahead of a barrier to
+// overlap it (next-tile prefetch, early twiddles); the LDS-scoped fences
+// wait for LDS accesses (lgkmcnt) alone.
+#ifndef SBA_LDS_BARRIER
+#define SBA_LDS_BARRIER 1
+#endif
+__device__ __forceinline__ void lds_barrier() {
+  if (SBA_LDS_BARRIER) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+  } else {
+    __syncthreads();
+  }
+}
+
 // a compile-time flag passed to generic lambdas (code instantiated once per
 // butterfly form)
 template <bool V> struct BoolC { static constexpr bool value = V; };
@@ -194,6 +219,42 @@ template <int ELL, int LOGTC, int T0, int B> struct ChunkRegs {
   static constexpr int GPT = (1 << (ELL + LOGTC)) / ((1 << B) * NT);
   typedef uint32_t type[GPT][1 << B];
 };
+// a chunk's twiddles: group gi, level u, sub-node m2 at [gi][(1 << u) - 1 + m2]
+template <int ELL, int LOGTC, int T0, int B> struct ChunkTw {
+  typedef uint32_t type[ChunkRegs<ELL, LOGTC, T0, B>::GPT][(1 << B) - 1];
+};
+
+// the twiddles of chunk (T0, B) of tile row H: a chunk whose groups all lie
+// in one node (NT threads x GPT groups under 2^(LOGTC + LB)) shares them
+// across the block, and a uniform index lets them come through the scalar
+// cache. ZL: forward levels below ZL are copies and take none.
+template <int ELL, int LOGTC, int T0, int B, bool INV, int ZL = 0, bool OFF32 = false>
+__device__ __forceinline__ void chunk_twiddles(const uint32_t *__restrict__ tw, uint32_t H,
+                                               typename ChunkTw<ELL, LOGTC, T0, B>::type &tv) {
+  constexpr int GPT = ChunkRegs<ELL, LOGTC, T0, B>::GPT;
+  constexpr int LB = ELL - T0 - B;
+  constexpr bool UNI = NT * GPT <= (1 << (LOGTC + LB));
+  // uniform twiddles through the constant address space: scalar loads
+  // (the tables are read-only while any kernel runs), which also keeps them
+  // out of the in-order count of the thread's vector memory accesses
+  typedef __attribute__((address_space(4))) const uint32_t cc_u32;
+  cc_u32 *twc = (cc_u32 *)tw;
+#pragma unroll
+  for (int gi = 0; gi < GPT; ++gi) {
+    const uint32_t hj =
+        UNI ? (H << T0) : (H << T0) | ((threadIdx.x + gi * NT) >> (LOGTC + LB));
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      if (!INV && T0 + u < ZL) continue;
+#pragma unroll
+      for (int m2 = 0; m2 < (1 << u); ++m2) {
+        const uint32_t i = (hj << u) | (uint32_t)m2;
+        tv[gi][(1 << u) - 1 + m2] =
+            SBA_TW_SCALAR && UNI ? twc[__builtin_amdgcn_readfirstlane(i)] : (OFF32 ? tw_at(tw, i) : tw[i]);
+      }
+    }
+  }
+}
 
 // GUARD (inverse only, column passes: H = 0): a butterfly runs only when its
 // block of the column transform lies inside rows [0, gM) -- the complete
@@ -222,32 +283,14 @@ __device__ __forceinline__ void tile_chunk_io(uint32_t *lds, const uint32_t *__r
   // TW_EARLY32 addresses them by 32-bit offsets (fewer live address
   // registers where a kernel holds other values across its chunks).
   constexpr bool EARLY = SBA_TW_EARLY && TE != TW_LATE;
-  constexpr int TWN = EARLY ? (1 << B) - 1 : 1;
-  uint32_t tv[GPT][TWN];
+  typename ChunkTw<ELL, LOGTC, T0, B>::type tv;
   if constexpr (EARLY) {
     // not hoisted above the caller's preceding work (the pointwise product
     // before the first inverse chunk), where they would stretch live ranges
     __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int gi = 0; gi < GPT; ++gi) {
-      // a chunk whose groups all lie in one node (NT threads x GPT groups
-      // under 2^(LOGTC + LB)) shares its twiddles across the block: a
-      // uniform index lets them come through the scalar cache
-      const uint32_t hj = (NT * GPT <= (1 << (LOGTC + LB)))
-                              ? (H << T0)
-                              : (H << T0) | ((threadIdx.x + gi * NT) >> (LOGTC + LB));
-#pragma unroll
-      for (int u = 0; u < B; ++u) {
-        if (!INV && T0 + u < ZL) continue;
-#pragma unroll
-        for (int m2 = 0; m2 < (1 << u); ++m2) {
-          const uint32_t i = (hj << u) | (uint32_t)m2;
-          tv[gi][(1 << u) - 1 + m2] = TE == TW_EARLY32 ? tw_at(tw, i) : tw[i];
-        }
-      }
-    }
+    chunk_twiddles<ELL, LOGTC, T0, B, INV, ZL, TE == TW_EARLY32>(tw, H, tv);
   }
-  if (!IN_REGS) __syncthreads();
+  if (!IN_REGS) lds_barrier();
 #pragma unroll
   for (int gi = 0; gi < GPT; ++gi) {
     const uint32_t g = threadIdx.x + gi * NT;
@@ -475,6 +518,16 @@ typedef __attribute__((address_space(1))) const uint32_t gc_u32;
 typedef __attribute__((address_space(1))) uint64_t g_u64;
 typedef __attribute__((address_space(1))) const uint64_t gc_u64;
 typedef __attribute__((address_space(1))) uint8_t g_u8;
+typedef __attribute__((address_space(1))) const uint8_t gc_u8;
+// element i (i < 2^30) of a uniform base through a 32-bit byte offset: the
+// access takes the base in SGPRs and one VGPR offset instead of a 64-bit
+// address per lane
+__device__ __forceinline__ uint32_t ld_u32(const g_u32 *base, uint32_t i) {
+  return *(const g_u32 *)((gc_u8 *)base + (i << 2));
+}
+__device__ __forceinline__ void st_u32(g_u32 *base, uint32_t i, uint32_t v) {
+  *(g_u32 *)((g_u8 *)base + (i << 2)) = v;
+}
 
 // accumulator target of a chunked product's last inverse pass (dst ==
 // nullptr: ordinary in-place store)
@@ -594,7 +647,7 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
   auto prime = [&](uint32_t q) {
     constexpr int HV = SBA_COL_HV;
     const uint32_t p = pa.p[q], pinv = pa.pinv[q], r1 = pa.r1[q];
-    __syncthreads();
+    lds_barrier();
     // the tile's limbs are re-read per prime (cache-resident after the first)
     if constexpr (DIRECT) {
       // the level-0 chunk starts from the residues in registers. Element i
@@ -661,7 +714,7 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
       }
       tile_fwd_first<ELL, LOGTC, BMAX, HV>(lds, pa.wn[q], p, pinv, half_zero);
     }
-    __syncthreads();
+    lds_barrier();
     g_u32 *out = spec + q * N;
 #pragma unroll
     for (int i = 0; i < E; ++i) {
@@ -756,7 +809,7 @@ __global__ __launch_bounds__(NT, INV ? SBA_INV_WAVES : SBA_WAVES) void k_pass(Pr
     }
     // the inverse's element i: in registers (DIRECT) or in LDS
     const bool inreg = DIRECT && INV;
-    if (!inreg) __syncthreads();
+    if (!inreg) lds_barrier();
     // the inverse's last levels (s0 == 0) finish the transform: scale here
     const bool fin = INV && s0 == 0;
     const uint32_t sc = pa.scale[q];
@@ -865,13 +918,13 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
 #pragma unroll
           for (int m = 0; m < (1 << LC::B); ++m)
             bv[g][m] = mont_mul(bv[g][m], canon<HV>(bv[g][m], p), p, pinv);
-        __syncthreads();               // the last LDS reads before the inverse's writes
+        lds_barrier();               // the last LDS reads before the inverse's writes
         tile_chunk_io<ELL, 0, LC::T0, LC::B, true, 0, CHUNK_FROM_REGS, TW_EARLY, false, HV>(
             lds, pa.wni[q], H, p, pinv, bv);
         tile_inv_upto<ELL, 0, BMAX, B0, LC::T0, HV>(lds, pa.wni[q], H, p, pinv);
       } else {
         tile_chunk<ELL, 0, LC::T0, LC::B, false, 0, TW_EARLY, HV>(lds, pa.wn[q], H, p, pinv);
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int i = 0; i < E; ++i) {
           const uint32_t v = lt[i * Pad<0>::words(NT)];
@@ -900,7 +953,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
       // LIMB_IN: one tile (H = 0) holds the whole transform of the padded limbs
       if (LIMB_IN) tile_fwd_first<ELL, 0, BMAX, HV>(lds, pa.wn[q], p, pinv, nb <= N / 2);
       else tile_fwd<ELL, 0, BMAX, 0, 0, TW_EARLY, HV>(lds, pa.wn[q], H, p, pinv);
-      __syncthreads();
+      lds_barrier();
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         const uint32_t v = lt[i * Pad<0>::words(NT)];
@@ -908,7 +961,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
         lt[i * Pad<0>::words(NT)] = mont_mul(v, o, p, pinv);
       }
       tile_inv<ELL, 0, BMAX, 0, TW_EARLY, HV>(lds, pa.wni[q], H, p, pinv);
-      __syncthreads();
+      lds_barrier();
       // LIMB_IN: the single tile is the whole transform, so the inverse ends here
 #pragma unroll
       for (int i = 0; i < E; ++i) {
@@ -981,13 +1034,13 @@ __global__ __launch_bounds__(NT, BMAX == 5 ? SBA_FUSED2_WAVES : SBA_WAVES) void 
       for (int g = 0; g < GL; ++g)
 #pragma unroll
         for (int m = 0; m < (1 << LC::B); ++m) av[g][m] = canon<HV>(av[g][m], p);
-      __syncthreads();                 // a's last LDS reads before b's first writes
+      lds_barrier();                 // a's last LDS reads before b's first writes
       fwd(buf, bv);
 #pragma unroll
       for (int g = 0; g < GL; ++g)
 #pragma unroll
         for (int m = 0; m < (1 << LC::B); ++m) bv[g][m] = mont_mul(bv[g][m], av[g][m], p, pinv);
-      __syncthreads();                 // b's last LDS reads before the inverse's writes
+      lds_barrier();                 // b's last LDS reads before the inverse's writes
       tile_chunk_io<ELL, 0, LC::T0, LC::B, true, 0, CHUNK_FROM_REGS, F2TE, false, HV>(
           lds, pa.wni[q], H, p, pinv, bv);
       tile_inv_upto<ELL, 0, BMAX, B0, LC::T0, HV, F2TE>(lds, pa.wni[q], H, p, pinv);
@@ -1001,19 +1054,19 @@ __global__ __launch_bounds__(NT, BMAX == 5 ? SBA_FUSED2_WAVES : SBA_WAVES) void 
 #pragma unroll
       for (int i = 0; i < E; ++i) lt[i * Pad<0>::words(NT)] = (as + i * NT)[tid];
       tile_fwd<ELL, 0, BMAX, 0, 0, F2TE, HV>(lds, pa.wn[q], H, p, pinv);
-      __syncthreads();
+      lds_barrier();
       // each thread takes back only its own elements, then refills them with b
 #pragma unroll
       for (int i = 0; i < E; ++i) av[i] = canon<HV>(lt[i * Pad<0>::words(NT)], p);
 #pragma unroll
       for (int i = 0; i < E; ++i) lt[i * Pad<0>::words(NT)] = (buf + i * NT)[tid];
       tile_fwd<ELL, 0, BMAX, 0, 0, F2TE, HV>(lds, pa.wn[q], H, p, pinv);
-      __syncthreads();
+      lds_barrier();
 #pragma unroll
       for (int i = 0; i < E; ++i)
         lt[i * Pad<0>::words(NT)] = mont_mul(lt[i * Pad<0>::words(NT)], av[i], p, pinv);
       tile_inv<ELL, 0, BMAX, 0, F2TE, HV>(lds, pa.wni[q], H, p, pinv);
-      __syncthreads();
+      lds_barrier();
 #pragma unroll
       for (int i = 0; i < E; ++i) (buf + i * NT)[tid] = lt[i * Pad<0>::words(NT)];
     }
@@ -1094,7 +1147,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd3(PrimeArgs pa, JobR
 #pragma unroll 1
   for (uint32_t q = q0; q < q1; ++q) {
     const uint32_t p = pa.p[q], pinv = pa.pinv[q], r1 = pa.r1[q];
-    __syncthreads();
+    lds_barrier();
     Regs0 v0, v1, v2;
 #pragma unroll
     for (int i = 0; i < E; ++i) {
@@ -1149,7 +1202,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd3(PrimeArgs pa, JobR
 #pragma unroll 1
       for (int k = 0; k < 3; ++k) tile_fwd<ELL, LOGTC, 5, 0>(lds + k * W, pa.wn[q], 0, p, pinv);
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       g_u32 *out = spec + q * 3 * M + (uint32_t)k * M;
@@ -1222,7 +1275,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_inv_first3(PrimeArgs pa, JobR
       }
 #pragma unroll 1
     for (int k = 0; k < 3; ++k) tile_inv<ELL, LOGTC, 5, 0>(lds + k * W, pa.wni[q], 0, p, pinv);
-    __syncthreads();
+    lds_barrier();
   }
 #pragma unroll
   for (int i = 0; i < E; ++i) {
@@ -1303,11 +1356,11 @@ __global__ __launch_bounds__(NT) void k_inv_trunc(PrimeArgs pa, JobRef jr, int L
     const uint32_t k = e >> LOGTC, c = e & (TC - 1);
     at(k, c) = k < M ? (buf + k * S)[c] : 0u;
   }
-  __syncthreads();
+  lds_barrier();
   // phase 1: the complete blocks, every inverse level (register chunks as
   // in the full inverse column pass, each butterfly guarded)
   tile_inv_guard<ELL, LOGTC, 5, 0>(lds, wni, p, pinv, M);
-  __syncthreads();
+  lds_barrier();
   if (M < (uint32_t)R) {
     // phase 2: down the spine (dec bit d: the node at depth d went right)
     uint32_t dec = 0;
@@ -1342,7 +1395,7 @@ __global__ __launch_bounds__(NT) void k_inv_trunc(PrimeArgs pa, JobRef jr, int L
       }
       m = h;
       nd = d + 1;
-      __syncthreads();
+      lds_barrier();
     }
     // phase 3: back up the spine
 #pragma unroll 1
@@ -1374,7 +1427,7 @@ __global__ __launch_bounds__(NT) void k_inv_trunc(PrimeArgs pa, JobRef jr, int L
           at(r + h, c) = y;
         }
       }
-      __syncthreads();
+      lds_barrier();
     }
   }
   // the column pass finishes the inverse: scale and store the rows k < M
@@ -1393,6 +1446,7 @@ __global__ __launch_bounds__(NT) void k_inv_trunc(PrimeArgs pa, JobRef jr, int L
 typedef void (*first_fn)(PrimeArgs, JobRef, int, int);
 typedef void (*pass_fn)(PrimeArgs, JobRef, int, int, int, AccArgs);
 typedef void (*fused_fn)(PrimeArgs, JobRef, int, int);
+typedef void (*inv3_fn)(PrimeArgs, JobRef, int);
 struct Plan { int last, bm, split; };
 
 #define SBA_ELL_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
@@ -1479,7 +1533,6 @@ template <int BM> static Launchers launchers_bm(int L, int sqr, int last) {
 }
 // radix-3 first / last column passes: 2^11 residues of each row per tile,
 // ELL = L - last first-pass levels (defined in the radix-32 object)
-typedef void (*inv3_fn)(PrimeArgs, JobRef, int);
 first_fn sba_r3_first(int ell);
 inv3_fn sba_r3_inv(int ell);
 // the truncated inverse column pass of ELL = L - last levels (column tiles of
