@@ -5,15 +5,6 @@
 
 #ifndef SBA_TU_BM
 #error "SBA_TU_BM must be defined"
-#define SBA_TRUNC_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
-inv3_fn sba_trunc_inv(int ell) {
-  switch (ell) {
-#define SBA_C(E) case E: return k_inv_trunc<E, SBA_FIRST_TILE_LG - E>;
-    SBA_TRUNC_CASES(SBA_C)
-#undef SBA_C
-  }
-  return nullptr;
-}
 #endif
 #define SBA_CAT2(a, b) a##b
 #define SBA_CAT(a, b) SBA_CAT2(a, b)
