@@ -59,7 +59,12 @@ struct GarnerArgs {
 #ifndef SBA_COL_HV
 #define SBA_COL_HV HV_AUTO
 #endif
-// butterfly forms of k_fused (squares, chunks, single-tile products)
+// butterfly forms of k_fused (squares, chunks, single-tile products):
+// SBA_FUSED_SPLIT 1 instantiates the body once per form and the block takes
+// Harvey's for a prime below 2^30 (one uniform branch); 0: SBA_FUSED_HV
+#ifndef SBA_FUSED_SPLIT
+#define SBA_FUSED_SPLIT 1
+#endif
 #ifndef SBA_FUSED_HV
 #define SBA_FUSED_HV HV_W
 #endif
@@ -885,12 +890,13 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
   constexpr int B0 = ChunkPlan<ELL, 0, BMAX>::pick(0);
   using LC = LastChunk<ELL, 0, BMAX>;
   constexpr bool REGS = SBA_FUSED2_REGS && B0 == ChunkPlan<ELL, 0, BMAX>::LE && LC::T0 >= B0;
-  // butterfly forms: SBA_FUSED_HV (the [0, 2p) forms by default: choosing
-  // by the prime needs more registers than 3 waves per SIMD leave). Stored
-  // forward values (b's, and a's read for the product) may come from a pass
-  // with Harvey's forms: fwd_in and canon<HV_AUTO> take either range.
-  auto body = [&]() {
-    constexpr int HV = SBA_FUSED_HV;
+  // butterfly forms: with SBA_FUSED_SPLIT the body is instantiated for
+  // Harvey's forms and for the [0, 2p) forms, and the block runs the one its
+  // prime allows. Stored forward values (b's, and a's read for the product)
+  // may come from a pass with Harvey's forms: fwd_in and canon<HV_AUTO>
+  // take either range.
+  auto body = [&](auto hv) {
+    constexpr int HV = SBA_FUSED_SPLIT ? (decltype(hv)::value ? HV_H : HV_W) : SBA_FUSED_HV;
     if constexpr (REGS) {
       typedef typename ChunkRegs<ELL, 0, 0, B0>::type R0;
       R0 v0;
@@ -970,7 +976,8 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_fused(PrimeArgs pa, JobRef jr
       }
     }
   };
-  body();
+  if (SBA_FUSED_SPLIT && harvey_prime(p)) body(BoolC<true>{});
+  else body(BoolC<false>{});
 }
 
 // Multiply, two-pass plans: the last forward levels of a AND of b on the same
