@@ -24,9 +24,9 @@
  * return the codes below.
  *
  * Device memory. The first GPU call on a device builds the root tables of
- * its two prime sets once: 2 x 5 primes x (2^24 + 2^22) x 4 B = 800 MiB,
- * held until the process ends (one table per prime serves every transform
- * length of its set). Entries without a
+ * its three prime sets once: 2 x 5 primes x (2^24 + 2^22 + 2^21) x 4 B =
+ * 880 MiB, held until the process ends (one table per prime serves every
+ * transform length of its set). Entries without a
  * caller workspace add a grow-only workspace of about 40 B per transform
  * point of the largest product seen (2^23 points: 336 MiB).
  *
@@ -173,11 +173,12 @@ uint32_t sba_get_school_long_ratio(void);
  * on MI355X; batches always from 13) */
 void sba_set_r3_min_lg(int L);
 int sba_get_r3_min_lg(void);
-/* the prime set of power-of-two transforms up to 2^23: 1 (default) three
- * primes below 2^30, whose butterflies take 8 instructions instead of 10,
- * and two below 2^31; 0 the five primes below 2^31 that every other
- * transform uses. Results are the same either way; batches keep the choice
- * in force when they were created. */
+/* the prime sets of power-of-two transforms up to 2^23 and of radix-3
+ * transforms up to 3 x 2^22: 1 (default) three primes below 2^30, whose
+ * butterflies take 8 instructions instead of 10, and two below 2^31 (one
+ * set for each kind of length); 0 the five primes below 2^31 that every
+ * other transform uses. Results are the same either way; batches keep the
+ * choice in force when they were created. */
 void sba_set_prime_set_h(int on);
 int sba_get_prime_set_h(void);
 /* 1 when a single product of (an, bn) runs truncated (a power-of-two length

@@ -517,19 +517,27 @@ __global__ __launch_bounds__(NT) void k_twiddles(TwArgs ta, uint32_t *__restrict
 //     3 | p - 1) -- power-of-two lengths up to 2^23, CRT capacity 2^151.3,
 //     which holds the largest coefficient such a product has,
 //     min(an, bn) (2^64 - 1)^2 < 2^22 * 2^128 = 2^150.
-// The merged last pass (k_fused2) runs Harvey's forms for the three small
-// primes: its butterfly work is (3 * 8 + 2 * 10) / (5 * 10) = 0.88 of the
-// set-W pass. The other kernels run the [0, 2p) forms for every prime.
+//   SET_R: the same shape for the radix-3 lengths 3 * 2^L, L <= 22: three
+//     primes below 2^30 and two below 2^31, all with 3 * 2^22 | p - 1, CRT
+//     capacity 2^151.2 against min(an, bn) (2^64 - 1)^2 < 3 * 2^21 * 2^128
+//     = 2^150.6. (No set of three such primes below 2^30 also has
+//     2^23 | p - 1 and enough capacity, hence a set of its own.)
+// The merged last pass (k_fused2, k_fused) runs Harvey's forms for the
+// three small primes: its butterfly work is (3 * 8 + 2 * 10) / (5 * 10) =
+// 0.88 of the set-W pass; the power-of-two column passes choose the form
+// chunk by chunk, the radix-3 ones run the [0, 2p) forms.
 #define SET_W 0
 #define SET_H 1
-#define NSETS 2
+#define SET_R 2
+#define NSETS 3
 struct PrimeSetDef {
   uint32_t p[SBA_K];
-  int max_lg;                          // largest power-of-two transform
+  int max_lg;                          // largest power-of-two transform (row)
 };
 static const PrimeSetDef kSets[NSETS] = {
     {{1107296257u, 1711276033u, 1811939329u, 2013265921u, 2113929217u}, SBA_MAX_LG},
     {{880803841u, 897581057u, 998244353u, 2113929217u, 2130706433u}, 23},
+    {{880803841u, 918552577u, 943718401u, 2088763393u, 2113929217u}, 22},
 };
 static const uint32_t *const kPrimes = kSets[SET_W].p;   // the band's (CRT cap) set
 
@@ -659,8 +667,9 @@ static TwArgs tw_args(const SetCtx &c, const PrimeSetDef &d, int q) {
   return ta;
 }
 
-// both sets' root tables, built once for each set's largest transform
-// (SET_W: 2^24 node twiddles per table, 640 MiB; SET_H: 2^22, 160 MiB)
+// every set's root tables, built once for each set's largest transform
+// (SET_W: 2^24 node twiddles per table, 640 MiB; SET_H: 2^22, 160 MiB;
+// SET_R: 2^21, 80 MiB)
 static int ctx_twiddles(DevCtx &c, int L, hipStream_t st) {
   (void)L;
   for (int s = 0; s < NSETS; ++s) {
@@ -686,14 +695,16 @@ static int ctx_twiddles(DevCtx &c, int L, hipStream_t st) {
   return 0;
 }
 
-// the prime set of a transform: SET_H for power-of-two lengths it holds
-// (unless switched off), SET_W otherwise
+// the prime set of a transform (L: the row length's log for radix 3): the
+// sets with primes below 2^30 where they hold the length (unless switched
+// off), SET_W otherwise
 static std::atomic<int> g_use_set_h{1};
 extern "C" void sba_set_prime_set_h(int on) { g_use_set_h.store(on ? 1 : 0); }
 extern "C" int sba_get_prime_set_h(void) { return g_use_set_h.load(); }
 static int set_for(int L, int r3) {
-  if (!r3 && L <= kSets[SET_H].max_lg && g_use_set_h.load(std::memory_order_relaxed))
-    return SET_H;
+  if (!g_use_set_h.load(std::memory_order_relaxed)) return SET_W;
+  if (!r3 && L <= kSets[SET_H].max_lg) return SET_H;
+  if (r3 && L <= kSets[SET_R].max_lg) return SET_R;
   return SET_W;
 }
 
@@ -1348,7 +1359,8 @@ static bool plan_ok3(int L, const Plan &p) {
 }
 static Plan r3_plan(int L, int sqr, uint32_t njobs) { return plan_for3(L, sqr, njobs); }
 
-static int enqueue_r3(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef &jr,
+// set: the prime set (set_for(L, 1), or SET_W where only its tables exist)
+static int enqueue_r3(DevCtx &c, int set, int L, int sqr, const Plan &pl, const JobRef &jr,
                       uint32_t njobs, uint32_t nbmax, double asum, double bsum, double zsum,
                       hipStream_t st) {
   const int ell1 = L - pl.last;
@@ -1357,7 +1369,8 @@ static int enqueue_r3(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef &j
   const first_fn f1 = sba_r3_first(ell1);
   const inv3_fn i1 = sba_r3_inv(ell1);
   const uint32_t M = 1u << L;
-  const PrimeArgs pa = prime_args(c, SET_W, 3 * (uint64_t)M);
+  if (L > kSets[set].max_lg || !c.set[set].pa.wn[0]) return SBA_BAND;
+  const PrimeArgs pa = prime_args(c, set, 3 * (uint64_t)M);
   const double KN4 = 4.0 * SBA_K * 3.0 * M * njobs;
   const uint32_t ntile1 = M >> 11, ntile2 = M >> pl.last;
   const uint32_t gy1 = pl.split ? SBA_K : 1;
@@ -1380,7 +1393,7 @@ static int enqueue_r3(DevCtx &c, int L, int sqr, const Plan &pl, const JobRef &j
     PROF("inv_first", 2 * KN4);
     KLAUNCH(i1, dim3(ntile1, SBA_K, njobs), dim3(NT), 0, st, pa, jr, L);
   }
-  return enqueue_garner(c, SET_W, 3 * M, 3 * M, jr, njobs, nbmax, zsum, st);
+  return enqueue_garner(c, set, 3 * M, 3 * M, jr, njobs, nbmax, zsum, st);
 }
 
 // one transform of length <= 2^25 (batches use only this)
@@ -1581,8 +1594,8 @@ static int mul_common(uint64_t *rp, const uint64_t *ap, int64_t an, const uint64
   j.nb = carry_blocks(j.Z);
   job_layout(j, ws, (uint32_t)Ntot);
   if (L3)
-    return enqueue_r3(*c, L3, sqr, r3_plan(L3, sqr, 1), jr, 1, j.nb, (double)an, (double)bn,
-                      (double)j.Z, st);
+    return enqueue_r3(*c, set_for(L3, 1), L3, sqr, r3_plan(L3, sqr, 1), jr, 1, j.nb,
+                      (double)an, (double)bn, (double)j.Z, st);
   return enqueue_class(*c, set_for(L, 0), L, sqr, plan_for(L, sqr, 1), jr, 1, j.nb, j.zlen,
                        (double)an, (double)bn, (double)j.Z, st);
 }
@@ -1727,8 +1740,8 @@ extern "C" int sba_gpu_mul_dev_noalloc(uint64_t *rp, const uint64_t *ap, int64_t
   j.nb = carry_blocks(j.Z);
   job_layout(j, c->na_scratch + NA_TW_BYTES, (uint32_t)Ntot);
   if (L3)
-    return enqueue_r3(nc, L3, sqr, r3_plan(L3, sqr, 1), jr, 1, j.nb, (double)an, (double)bn,
-                      (double)j.Z, st);
+    return enqueue_r3(nc, SET_W, L3, sqr, r3_plan(L3, sqr, 1), jr, 1, j.nb, (double)an,
+                      (double)bn, (double)j.Z, st);
   return enqueue_class(nc, SET_W, L, sqr, plan_for(L, sqr, 1), jr, 1, j.nb, j.zlen,
                        (double)an, (double)bn, (double)j.Z, st);
 }
@@ -1928,7 +1941,8 @@ static int enqueue_classes(DevCtx &c, Job *djobs, const ClassDesc *cls, int ncls
       continue;
     }
     const int rc =
-        cls[k].r3 ? enqueue_r3(c, cls[k].L, cls[k].sqr, cls[k].plan, jr, cls[k].n, cls[k].nbmax,
+        cls[k].r3 ? enqueue_r3(c, set_for(cls[k].L, 1), cls[k].L, cls[k].sqr, cls[k].plan, jr,
+                               cls[k].n, cls[k].nbmax,
                                cls[k].as, cls[k].bs, cls[k].zs, st)
                   : enqueue_class(c, set_for(cls[k].L, 0), cls[k].L, cls[k].sqr, cls[k].plan, jr,
                                   cls[k].n, cls[k].nbmax, cls[k].zmin, cls[k].as, cls[k].bs,
