@@ -524,8 +524,8 @@ __global__ __launch_bounds__(NT) void k_twiddles(TwArgs ta, uint32_t *__restrict
 //     2^23 | p - 1 and enough capacity, hence a set of its own.)
 // The merged last pass (k_fused2, k_fused) runs Harvey's forms for the
 // three small primes: its butterfly work is (3 * 8 + 2 * 10) / (5 * 10) =
-// 0.88 of the set-W pass; the power-of-two column passes choose the form
-// chunk by chunk, the radix-3 ones run the [0, 2p) forms.
+// 0.88 of the set-W pass; the column passes (power-of-two and radix-3)
+// choose the form chunk by chunk.
 #define SET_W 0
 #define SET_H 1
 #define SET_R 2

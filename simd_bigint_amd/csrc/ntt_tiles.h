@@ -1116,11 +1116,23 @@ __device__ __forceinline__ uint32_t r3_pick(uint32_t r, uint32_t sm, uint32_t mu
   return t == 0 ? x0 : (t == 1 ? x1 : x2);
 }
 
+// butterfly forms of the radix-3 column passes (as SBA_COL_HV: chosen by
+// the prime chunk by chunk; the column DFT-3 takes and gives [0, 2p))
+#ifndef SBA_R3_COL_HV
+#define SBA_R3_COL_HV SBA_COL_HV
+#endif
+constexpr int R3HV = SBA_R3_COL_HV;
+// the radix-3 first pass at 4 waves/SIMD: at most 128 VGPRs, no spills
+// (with the forms chosen by the prime it needs up to 134 unbounded)
+#ifndef SBA_R3_FIRST_WAVES
+#define SBA_R3_FIRST_WAVES 4
+#endif
+
 // First forward pass of a radix-3 class: limbs -> residues -> column DFT-3
 // -> the first ELL levels of all three rows. Tiles: 2^ELL rows x 2^LOGTC
 // columns of each of the three length-M rows (three LDS tiles).
 template <int ELL, int LOGTC>
-__global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd3(PrimeArgs pa, JobRef jr, int which,
+__global__ __launch_bounds__(NT, SBA_R3_FIRST_WAVES) void k_first_fwd3(PrimeArgs pa, JobRef jr, int which,
                                                               int L) {
   constexpr int TC = 1 << LOGTC;
   constexpr int NE = 1 << (ELL + LOGTC);
@@ -1183,12 +1195,13 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd3(PrimeArgs pa, JobR
     if constexpr (DIRECT) {
       auto row = [&](int k, Regs0 &vk) {
         uint32_t *lk = lds + k * W;
-        tile_chunk_io<ELL, LOGTC, 0, B0, false, 0, CHUNK_FROM_REGS>(lk, pa.wn[q], 0, p, pinv, vk);
+        tile_chunk_io<ELL, LOGTC, 0, B0, false, 0, CHUNK_FROM_REGS, TW_EARLY, false, R3HV>(lk, pa.wn[q], 0, p, pinv, vk);
         if constexpr (EDGE) {
           using IO = ChunkIO<ELL, LOGTC, LC::T0, LC::B>;
-          tile_fwd_range<ELL, LOGTC, 5, B0, LC::T0>(lk, pa.wn[q], 0, p, pinv);
+          tile_fwd_range<ELL, LOGTC, 5, B0, LC::T0, R3HV>(lk, pa.wn[q], 0, p, pinv);
           typename ChunkRegs<ELL, LOGTC, LC::T0, LC::B>::type vl;
-          tile_chunk_io<ELL, LOGTC, LC::T0, LC::B, false, 0, CHUNK_TO_REGS>(lk, pa.wn[q], 0, p,
+          tile_chunk_io<ELL, LOGTC, LC::T0, LC::B, false, 0, CHUNK_TO_REGS, TW_EARLY, false, R3HV>(
+              lk, pa.wn[q], 0, p,
                                                                          pinv, vl);
           g_u32 *out = spec + q * 3 * M + (uint32_t)k * M + col0;
           const uint32_t t = IO::thr(S);
@@ -1198,7 +1211,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd3(PrimeArgs pa, JobR
 #pragma unroll
             for (int m = 0; m < (1 << LC::B); ++m) (out + IO::uni(gi, m, S))[t] = vl[gi][m];
         } else {
-          tile_fwd<ELL, LOGTC, 5, B0>(lk, pa.wn[q], 0, p, pinv);
+          tile_fwd<ELL, LOGTC, 5, B0, 0, TW_EARLY, R3HV>(lk, pa.wn[q], 0, p, pinv);
         }
       };
       row(0, v0);
@@ -1207,7 +1220,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_first_fwd3(PrimeArgs pa, JobR
       if constexpr (EDGE) continue;
     } else {
 #pragma unroll 1
-      for (int k = 0; k < 3; ++k) tile_fwd<ELL, LOGTC, 5, 0>(lds + k * W, pa.wn[q], 0, p, pinv);
+      for (int k = 0; k < 3; ++k) tile_fwd<ELL, LOGTC, 5, 0, 0, TW_EARLY, R3HV>(lds + k * W, pa.wn[q], 0, p, pinv);
     }
     lds_barrier();
 #pragma unroll
@@ -1264,10 +1277,12 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_inv_first3(PrimeArgs pa, JobR
       for (int gi = 0; gi < GL; ++gi)
 #pragma unroll
         for (int m = 0; m < (1 << LC::B); ++m) vf[gi][m] = (in + IO::uni(gi, m, S))[t];
-      tile_chunk_io<ELL, LOGTC, LC::T0, LC::B, true, 0, CHUNK_FROM_REGS>(lk, pa.wni[q], 0, p, pinv,
+      tile_chunk_io<ELL, LOGTC, LC::T0, LC::B, true, 0, CHUNK_FROM_REGS, TW_EARLY, false, R3HV>(
+          lk, pa.wni[q], 0, p, pinv,
                                                                        vf);
-      tile_inv_upto<ELL, LOGTC, 5, B0, LC::T0>(lk, pa.wni[q], 0, p, pinv);
-      tile_chunk_io<ELL, LOGTC, 0, B0, true, 0, CHUNK_TO_REGS>(lk, pa.wni[q], 0, p, pinv, vk);
+      tile_inv_upto<ELL, LOGTC, 5, B0, LC::T0, R3HV>(lk, pa.wni[q], 0, p, pinv);
+      tile_chunk_io<ELL, LOGTC, 0, B0, true, 0, CHUNK_TO_REGS, TW_EARLY, false, R3HV>(
+          lk, pa.wni[q], 0, p, pinv, vk);
     };
     row(0, v0);
     row(1, v1);
@@ -1281,7 +1296,7 @@ __global__ __launch_bounds__(NT, SBA_WAVES) void k_inv_first3(PrimeArgs pa, JobR
         lt[k * W + i * Pad<LOGTC>::words(NT)] = (buf + (uint32_t)k * M + uoff)[toff];
       }
 #pragma unroll 1
-    for (int k = 0; k < 3; ++k) tile_inv<ELL, LOGTC, 5, 0>(lds + k * W, pa.wni[q], 0, p, pinv);
+    for (int k = 0; k < 3; ++k) tile_inv<ELL, LOGTC, 5, 0, TW_EARLY, R3HV>(lds + k * W, pa.wni[q], 0, p, pinv);
     lds_barrier();
   }
 #pragma unroll
