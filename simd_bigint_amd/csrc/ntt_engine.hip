@@ -115,6 +115,12 @@ __device__ __forceinline__ void garner_coef(const GarnerArgs &ga, gc_u32 *spec,
 #define CARRY_LPB (CARRY_LPT * NT)      // limbs per (carry) block
 #define SCAN_NT 1024                    // k_carry_blocks threads
 #define CARRY_WGS 256                   // k_carry_blocks workgroups per product (at most)
+// k_carry_blocks blocks per workgroup (at least): a workgroup of SCAN_NT
+// threads per block of 1024 limbs would make a batch of many small products
+// launch 16 waves for every few limbs that need a carry
+#ifndef CARRY_MIN_SPAN
+#define CARRY_MIN_SPAN 16
+#endif
 #define C2_ID 2u                        // 2-state map c -> c: bit c = carry out for carry in c
 
 __device__ __forceinline__ uint32_t c2_apply(uint32_t f, uint32_t c) { return (f >> c) & 1u; }
@@ -1006,6 +1012,11 @@ static PrimeArgs prime_args(const DevCtx &c, int set, uint64_t Nt) {
 
 // carry blocks of a product of Z limbs
 static uint32_t carry_blocks(uint64_t Z) { return (uint32_t)((Z + CARRY_LPB - 1) / CARRY_LPB); }
+// k_carry_blocks workgroups for products of at most nb blocks
+static uint32_t carry_grid(uint32_t nb) {
+  const uint32_t g = (nb + CARRY_MIN_SPAN - 1) / CARRY_MIN_SPAN;
+  return g < 1 ? 1 : (g > CARRY_WGS ? CARRY_WGS : g);
+}
 
 // Truncated products (two-pass plans): the merged last pass transforms only
 // the tiles H < ceil(zlen / 2^last) of a job and the truncated inverse column
@@ -1127,7 +1138,7 @@ static int enqueue_garner(DevCtx &c, int set, uint32_t Nt, uint32_t stride, cons
   }
   {
     PROF("carry_blocks", 8.0 * zsum / CARRY_LPB);
-    const uint32_t g = nbmax < CARRY_WGS ? nbmax : CARRY_WGS;
+    const uint32_t g = carry_grid(nbmax);
     KLAUNCH(k_carry_blocks, dim3(g, 1, njobs), dim3(SCAN_NT), 0, st, jr);
   }
   HIPCHK(hipGetLastError());
@@ -1176,7 +1187,7 @@ extern "C" int sba_gpu_carry_blocks(uint64_t *rp, const uint32_t *bmaps, uint64_
   j.Z = (uint32_t)Z;
   j.zlen = (uint32_t)(Z - 1);
   j.nb = carry_blocks(Z);
-  const uint32_t g = j.nb < CARRY_WGS ? j.nb : CARRY_WGS;
+  const uint32_t g = carry_grid(j.nb);
   KLAUNCH(k_carry_blocks, dim3(g, 1, 1), dim3(SCAN_NT), 0, (hipStream_t)stream, jr);
   HIPCHK(hipGetLastError());
   return SBA_OK;
@@ -1497,7 +1508,7 @@ static int enqueue_school_long_one(uint64_t *rp, const uint64_t *ap, uint64_t an
   }
   {
     PROF("carry_blocks", 8.0 * j.Z / CARRY_LPB);
-    const uint32_t g = j.nb < CARRY_WGS ? j.nb : CARRY_WGS;
+    const uint32_t g = carry_grid(j.nb);
     KLAUNCH(k_carry_blocks, dim3(g, 1, 1), dim3(SCAN_NT), 0, st, jr);
   }
   HIPCHK(hipGetLastError());
@@ -1934,7 +1945,7 @@ static int enqueue_classes(DevCtx &c, Job *djobs, const ClassDesc *cls, int ncls
       }
       {
         PROF("carry_blocks", 8.0 * cls[k].zs / CARRY_LPB);
-        const uint32_t g = cls[k].nbmax < CARRY_WGS ? cls[k].nbmax : CARRY_WGS;
+        const uint32_t g = carry_grid(cls[k].nbmax);
         KLAUNCH(k_carry_blocks, dim3(g, 1, cls[k].n), dim3(SCAN_NT), 0, st, jr);
       }
       HIPCHK(hipGetLastError());
