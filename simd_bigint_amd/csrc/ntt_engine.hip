@@ -100,7 +100,7 @@ __device__ __forceinline__ void garner_coef(const GarnerArgs &ga, gc_u32 *spec,
 // into the limbs are (A + B + c_in) ^ A ^ B. Carries leave a run or a block
 // only on a generate followed by all-ones limbs, so for most data every
 // block's carry-in is 0.
-//   k_garner_limbs  Garner + limb words; carries inside each 2048-limb
+//   k_garner_limbs  Garner + limb words; carries inside each 1024-limb
 //                   block (block carry-in 0); the block's 2-state carry map
 //   k_carry_blocks  the block carry-ins (a scan of the block maps), added
 //                   where they are 1 (usually to one limb)
