@@ -44,9 +44,12 @@ __device__ __forceinline__ void garner_coef(const GarnerArgs &ga, gc_u32 *spec,
                                             uint32_t N, uint32_t i, uint64_t &c0, uint64_t &c1,
                                             uint64_t &c2) {
   uint32_t u[SBA_K], x[SBA_K];
+  // prime q's residues start at spec + q N, a uniform base kept in scalar
+  // registers (readfirstlane: the compiler does not fold it into a 64-bit
+  // per-lane address); coefficient i is a 32-bit byte offset (i < 2^30)
 #pragma unroll
   for (int q = 0; q < SBA_K; ++q)
-    u[q] = red1(spec[q * N + i], ga.p[q]);           // already scaled by the inverse
+    u[q] = red1(ld_u32(uni_ptr(spec + (size_t)q * N), i), ga.p[q]);   // already scaled by the inverse
   // digit q: x_q = (u_q - x_0 - sum_{0<k<q} x_k c_qk) / (p_0 ... p_{q-1})
   // mod p_q. The products x_k c_qk (each < 2^62, the c_qk in Montgomery
   // form) are summed two at a time in 64 bits and reduced once per pair:
