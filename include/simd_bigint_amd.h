@@ -24,9 +24,11 @@
  * return the codes below.
  *
  * Device memory. The first GPU call on a device builds the root tables of
- * its three prime sets once: 2 x 5 primes x (2^24 + 2^22 + 2^21) x 4 B =
- * 880 MiB, held until the process ends (one table per prime serves every
- * transform length of its set). Entries without a
+ * the two prime sets with primes below 2^30: 2 x 5 primes x (2^22 + 2^21)
+ * x 4 B = 240 MiB; the wide set's, 2 x 5 x 2^24 x 4 B = 640 MiB, are built
+ * by the first call that can use them (a transform above 2^23 points, or
+ * any call after sba_set_prime_set_h(0)). All are held until the process
+ * ends (one table per prime serves every transform length of its set). Entries without a
  * caller workspace add a grow-only workspace of about 40 B per transform
  * point of the largest product seen (2^23 points: 336 MiB).
  *

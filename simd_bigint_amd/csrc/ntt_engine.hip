@@ -676,38 +676,61 @@ static TwArgs tw_args(const SetCtx &c, const PrimeSetDef &d, int q) {
   return ta;
 }
 
-// every set's root tables, built once for each set's largest transform
-// (SET_W: 2^24 node twiddles per table, 640 MiB; SET_H: 2^22, 160 MiB;
-// SET_R: 2^21, 80 MiB)
-static int ctx_twiddles(DevCtx &c, int L, hipStream_t st) {
-  (void)L;
-  for (int s = 0; s < NSETS; ++s) {
-    SetCtx &sc = c.set[s];
-    const int Lt = kSets[s].max_lg;
-    if (sc.tw_lg >= Lt) continue;
-    const uint32_t count = 1u << (Lt - 1);
-    for (int q = 0; q < SBA_K; ++q) {
-      if (sc.wn[q]) { HIPCHK(hipFree(sc.wn[q])); HIPCHK(hipFree(sc.wni[q])); }
-      sc.wn[q] = sc.wni[q] = nullptr;
-      HIPCHK(hipMalloc(&sc.wn[q], (size_t)count * 4));
-      HIPCHK(hipMalloc(&sc.wni[q], (size_t)count * 4));
-      const TwArgs ta = tw_args(sc, kSets[s], q);
-      hipLaunchKernelGGL(k_twiddles, dim3((count + NT - 1) / NT), dim3(NT), 0, st, ta,
-                         sc.wn[q], sc.wni[q], count);
-      HIPCHK(hipGetLastError());
-      sc.pa.wn[q] = sc.wn[q];
-      sc.pa.wni[q] = sc.wni[q];
-    }
-    HIPCHK(hipStreamSynchronize(st));
-    sc.tw_lg = Lt;
+// the prime sets with primes below 2^30 (sets H and R) are used unless
+// switched off (sba_set_prime_set_h)
+static std::atomic<int> g_use_set_h{1};
+
+// one set's root tables, built once for its largest transform (SET_W: 2^24
+// node twiddles per table, 640 MiB; SET_H: 2^22, 160 MiB; SET_R: 2^21,
+// 80 MiB); tw_lg records the length they cover once every table is in place
+static int ctx_set_tables(DevCtx &c, int s, hipStream_t st) {
+  SetCtx &sc = c.set[s];
+  const int Lt = kSets[s].max_lg;
+  if (sc.tw_lg >= Lt) return 0;
+  const uint32_t count = 1u << (Lt - 1);
+  for (int q = 0; q < SBA_K; ++q) {
+    if (sc.wn[q]) { HIPCHK(hipFree(sc.wn[q])); HIPCHK(hipFree(sc.wni[q])); }
+    sc.wn[q] = sc.wni[q] = nullptr;
+    HIPCHK(hipMalloc(&sc.wn[q], (size_t)count * 4));
+    HIPCHK(hipMalloc(&sc.wni[q], (size_t)count * 4));
+    const TwArgs ta = tw_args(sc, kSets[s], q);
+    hipLaunchKernelGGL(k_twiddles, dim3((count + NT - 1) / NT), dim3(NT), 0, st, ta,
+                       sc.wn[q], sc.wni[q], count);
+    HIPCHK(hipGetLastError());
+    sc.pa.wn[q] = sc.wn[q];
+    sc.pa.wni[q] = sc.wni[q];
   }
+  HIPCHK(hipStreamSynchronize(st));
+  sc.tw_lg = Lt;
   return 0;
+}
+
+// the tables the transforms of a call (power-of-two lengths up to 2^L, or
+// radix-3 rows) may use: sets H and R on the first call (240 MiB), set W
+// only once a transform longer than set H's can run or the small-prime
+// sets are switched off
+static int ctx_twiddles(DevCtx &c, int L, hipStream_t st) {
+  int rc = ctx_set_tables(c, SET_H, st);
+  if (!rc) rc = ctx_set_tables(c, SET_R, st);
+  if (!rc && (L > kSets[SET_H].max_lg || !g_use_set_h.load(std::memory_order_relaxed)))
+    rc = ctx_set_tables(c, SET_W, st);
+  return rc;
+}
+
+// a launch on set `set` for rows of 2^L points needs its tables in place;
+// they are missing only if the set switch changed between a call's set-up
+// and its launches (another thread): refused, nothing launched. 0 when
+// ready (not SBA_OK, which is 1), else the error to return
+static int set_ready(const DevCtx &c, int set, int L) {
+  if (c.set[set].tw_lg >= L) return 0;
+  fprintf(stderr, "simd_bigint_amd: prime set %d has no root tables for 2^%d points (the set "
+          "switch changed during the call)\n", set, L);
+  return SBA_ERR_HIP;
 }
 
 // the prime set of a transform (L: the row length's log for radix 3): the
 // sets with primes below 2^30 where they hold the length (unless switched
 // off), SET_W otherwise
-static std::atomic<int> g_use_set_h{1};
 extern "C" void sba_set_prime_set_h(int on) { g_use_set_h.store(on ? 1 : 0); }
 extern "C" int sba_get_prime_set_h(void) { return g_use_set_h.load(); }
 static int set_for(int L, int r3) {
@@ -1073,6 +1096,7 @@ static int enqueue_class(DevCtx &c, int set, int L, int sqr, const Plan &pl, con
                          uint32_t njobs, uint32_t nbmax, uint32_t zmin, double asum, double bsum,
                          double zsum, hipStream_t st) {
   if (!plan_ok(L, pl) || L > kSets[set].max_lg) return SBA_BAND;
+  if (const int rs = set_ready(c, set, L)) return rs;
   const Launchers k = launchers(L, sqr, pl);
   const uint32_t N = 1u << L;
   const PrimeArgs pa = prime_args(c, set, N);
@@ -1244,6 +1268,7 @@ static int enqueue_chunked(DevCtx &c, const ChunkChoice &cp, uint64_t *rp, const
   if (!plan_ok(Lc, pl) || pl.last == Lc) return SBA_BAND;
   const Launchers k = launchers(Lc, 0, pl);
   const int set = set_for(Lc, 0);
+  if (const int rs = set_ready(c, set, Lc)) return rs;
   const PrimeArgs pa = prime_args(c, set, Nc);
   const uint64_t Z = an + bn;
   const uint32_t Zs = (uint32_t)((Z + 255) & ~(uint64_t)255);
@@ -1383,7 +1408,8 @@ static int enqueue_r3(DevCtx &c, int set, int L, int sqr, const Plan &pl, const 
   const first_fn f1 = sba_r3_first(ell1);
   const inv3_fn i1 = sba_r3_inv(ell1);
   const uint32_t M = 1u << L;
-  if (L > kSets[set].max_lg || !c.set[set].pa.wn[0]) return SBA_BAND;
+  if (L > kSets[set].max_lg) return SBA_BAND;
+  if (const int rs = set_ready(c, set, L)) return rs;
   const PrimeArgs pa = prime_args(c, set, 3 * (uint64_t)M);
   const double KN4 = 4.0 * SBA_K * 3.0 * M * njobs;
   const uint32_t ntile1 = M >> 11, ntile2 = M >> pl.last;
@@ -1736,6 +1762,7 @@ extern "C" int sba_gpu_mul_dev_noalloc(uint64_t *rp, const uint64_t *ap, int64_t
     nc.set[SET_W].pa.wn[q] = tw + (2 * q) * NA_TW_COUNT;
     nc.set[SET_W].pa.wni[q] = tw + (2 * q + 1) * NA_TW_COUNT;
   }
+  nc.set[SET_W].tw_lg = SBA_NOALLOC_MAX_LG;     // the static tables' reach
   // always one transform (power of two or 3 * 2^L), never chunked
   const int L3 = r3_lg(zlen, true);
   const int L = L3 ? L3 : lg_for(zlen);
