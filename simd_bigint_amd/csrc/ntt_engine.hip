@@ -830,20 +830,26 @@ static Plan g_plan[2][SBA_MAX_LG + 1][SBA_PLAN_BUCKETS];
 static Plan g_plan3[2][SBA_MAX_LG + 1][SBA_PLAN_BUCKETS];   // radix-3 classes (row length 2^L)
 static int g_plan_init = 0;
 static Plan r3_fix(int L, Plan p);
-static Plan shipped_plan(int sq, int L, int k) {
-  if (L >= 0 && L < SBA_PT_NL && k < SBA_PLAN_BUCKETS) {
-    const signed char *e = kPlanTable[sq][L][k];
-    const Plan p = {e[0], e[1], e[2]};
-    if (e[0] > 0 && plan_ok(L, p)) return p;
-  }
-  return heuristic_plan(L);
+static bool plan_ok3(int L, const Plan &p);
+// the table's entry of (kind, L, bucket), kind = square? | radix-3? << 1;
+// {0, 0, 0} (not measured) or an entry the kind does not accept: none
+static bool table_plan(int kind, int L, int k, Plan *out) {
+  if (L < 0 || L >= SBA_PT_NL || k >= SBA_PLAN_BUCKETS) return false;
+  const signed char *e = kPlanTable[kind][L][k];
+  const Plan p = {e[0], e[1], e[2]};
+  if (e[0] <= 0 || !((kind & 2) ? plan_ok3(L, p) : plan_ok(L, p))) return false;
+  *out = p;
+  return true;
 }
 static void plan_defaults(void) {
   for (int sq = 0; sq < 2; ++sq)
     for (int L = 0; L <= SBA_MAX_LG; ++L)
       for (int k = 0; k < SBA_PLAN_BUCKETS; ++k) {
-        g_plan[sq][L][k] = shipped_plan(sq, L, k);
-        g_plan3[sq][L][k] = r3_fix(L, g_plan[sq][L][k]);
+        Plan p;
+        g_plan[sq][L][k] = table_plan(sq, L, k, &p) ? p : heuristic_plan(L);
+        // radix-3 classes: the measured entry, else the power-of-two plan
+        // moved into the radix-3 limits
+        g_plan3[sq][L][k] = table_plan(2 | sq, L, k, &p) ? p : r3_fix(L, g_plan[sq][L][k]);
       }
   g_plan_init = 1;
 }

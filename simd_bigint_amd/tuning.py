@@ -151,13 +151,29 @@ def shape_operands(L: int, sqr: int, fill: float) -> Tuple[int, int]:
     return an, bn
 
 
-def make_class_jobs(L: int, sqr: int, njobs: int, fill: float, pool, dev):
-    """njobs device jobs of one size class, operands sliced from `pool`."""
+def shape_operands3(L: int, sqr: int, fill: float) -> Tuple[int, int]:
+    """Operand lengths (an, bn) of a job in the radix-3 class of row length
+    2^L, whose product has about fill * 3 * 2^L limbs: 2^(L+1) < an+bn-1 <=
+    3 * 2^L (balanced; bn = an for a square)."""
+    N3 = 3 << L
+    Z = min(max(int(round(fill * N3)), (2 << L) + 4), N3 + 1)
+    an = (Z + 1) // 2
+    bn = an if sqr else Z - an
+    while an + bn - 1 > N3:
+        an -= 1
+        if sqr:
+            bn = an
+    return an, bn
+
+
+def make_class_jobs(L: int, sqr: int, njobs: int, fill: float, pool, dev, r3: bool = False):
+    """njobs device jobs of one size class (radix-3 with r3), operands
+    sliced from `pool`."""
     import torch
-    an, bn = shape_operands(L, sqr, fill)
+    an, bn = shape_operands3(L, sqr, fill) if r3 else shape_operands(L, sqr, fill)
     P = pool.numel()
     res = torch.empty(njobs * (an + bn), dtype=torch.int64, device=dev)
-    rng = np.random.default_rng(L * 1000 + njobs * 7 + sqr)
+    rng = np.random.default_rng(L * 1000 + njobs * 7 + sqr + (2 if r3 else 0))
     jobs = []
     for i in range(njobs):
         oa = int(rng.integers(0, P - an))
@@ -182,8 +198,23 @@ def default_shapes(max_elems: int = 1 << 24) -> List[Tuple[int, int, int, float]
     return out
 
 
-def collect_timings(shapes, reps: int = 3, log=None) -> List[dict]:
-    """Time every candidate plan of every shape on the GPU."""
+def default_shapes3(max_elems: int = 1 << 24) -> List[Tuple[int, int, int, float]]:
+    """(L, sqr, njobs, fill) grid of the radix-3 classes (row length 2^L,
+    transform 3 * 2^L; fill relative to 3 * 2^L, above 2/3)."""
+    out = []
+    for L in range(13, 23):
+        for sqr in (0, 1):
+            for nj in (1, 3, 8, 24, 64, 192, 512, 1024):
+                if nj * (3 << L) > max_elems:
+                    continue
+                for fill in (0.8, 0.95):
+                    out.append((L, sqr, nj, fill))
+    return out
+
+
+def collect_timings(shapes, reps: int = 3, log=None, r3: bool = False) -> List[dict]:
+    """Time every candidate plan of every shape on the GPU (radix-3 classes
+    with r3: rows of row length 2^L, fill relative to 3 * 2^L)."""
     import torch
     dev = torch.device("cuda:0")
     g = torch.Generator(device="cpu").manual_seed(11)
@@ -192,12 +223,12 @@ def collect_timings(shapes, reps: int = 3, log=None) -> List[dict]:
     rows = []
     t0 = time.time()
     for k, (L, sqr, nj, fill) in enumerate(shapes):
-        jobs = make_class_jobs(L, sqr, nj, fill, pool, dev)
+        jobs = make_class_jobs(L, sqr, nj, fill, pool, dev, r3=r3)
         arr = _jobs_arr(jobs)
         ws, _ = _ws_for(arr, nj, dev)
-        an, bn = shape_operands(L, sqr, fill)
-        zfill = (an + bn) / float(1 << L)
-        for plan in plan_candidates(L):
+        an, bn = shape_operands3(L, sqr, fill) if r3 else shape_operands(L, sqr, fill)
+        zfill = (an + bn) / float((3 if r3 else 1) << L)
+        for plan in plan_candidates(L, 2 if r3 else 0):
             ms = class_time(jobs, plan, reps=reps, ws=ws)
             rows.append({"L": L, "sqr": sqr, "njobs": nj, "fill": round(zfill, 6),
                          "plan": list(plan), "ms": round(ms, 6)})
@@ -207,15 +238,24 @@ def collect_timings(shapes, reps: int = 3, log=None) -> List[dict]:
     return rows
 
 
-def save_timings(rows, path: str = TIMINGS_PATH, meta: Optional[dict] = None) -> None:
+def save_timings(rows, path: str = TIMINGS_PATH, meta: Optional[dict] = None,
+                 rows3: Optional[List[dict]] = None) -> None:
     os.makedirs(os.path.dirname(path), exist_ok=True)
     with open(path, "w") as f:
-        json.dump({"meta": meta or {}, "rows": rows}, f, separators=(",", ":"))
+        json.dump({"meta": meta or {}, "rows": rows, "rows3": rows3 or []}, f,
+                  separators=(",", ":"))
 
 
 def load_timings(path: str = TIMINGS_PATH) -> List[dict]:
+    """The power-of-two classes' rows (the cost model's dataset)."""
     with open(path) as f:
         return json.load(f)["rows"]
+
+
+def load_timings3(path: str = TIMINGS_PATH) -> List[dict]:
+    """The radix-3 classes' rows (plan table only)."""
+    with open(path) as f:
+        return json.load(f).get("rows3", [])
 
 
 def dataset(rows) -> Tuple[np.ndarray, np.ndarray]:
@@ -397,52 +437,56 @@ def model_chooser(predict_log_ms):
 PLAN_HEADER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "plan_table.h")
 
 
-def shipped_table(rows) -> Dict[Tuple[int, int, int], Plan]:
-    """(sqr, L, bucket) -> the plan with the lowest mean slowdown (time over
-    the shape's best) over the measured shapes of that key; keys without
-    measurements take the nearest measured bucket of the same (sqr, L)."""
+def shipped_table(rows, rows3=None) -> Dict[Tuple[int, int, int], Plan]:
+    """(kind, L, bucket) -> the plan with the lowest mean slowdown (time over
+    the shape's best) over the measured shapes of that key, kind = square? |
+    radix-3? << 1 (rows3: the radix-3 classes); keys without measurements
+    take the nearest measured bucket of the same (kind, L)."""
+    tagged = [(r["sqr"], r) for r in rows] + [(2 | r["sqr"], r) for r in (rows3 or [])]
     best = {}
-    for r in rows:
-        k = shape_key(r)
+    for kind, r in tagged:
+        k = (kind,) + shape_key(r)
         best[k] = min(best.get(k, float("inf")), r["ms"])
     agg: Dict[Tuple[int, int, int], Dict[Plan, List[float]]] = {}
-    for r in rows:
-        key = (r["sqr"], r["L"], plan_bucket(r["njobs"]))
+    for kind, r in tagged:
+        key = (kind, r["L"], plan_bucket(r["njobs"]))
         agg.setdefault(key, {}).setdefault(tuple(r["plan"]), []).append(
-            r["ms"] / best[shape_key(r)])
+            r["ms"] / best[(kind,) + shape_key(r)])
     table = {}
     for key, plans in agg.items():
         table[key] = min(plans, key=lambda p: (float(np.mean(plans[p])), p))
     nb = int(lib().sba_plan_bucket(1 << 30)) + 1
-    for sqr in (0, 1):
+    for kind in range(4):
         for L in range(10, 26):
-            have = [b for b in range(nb) if (sqr, L, b) in table]
+            have = [b for b in range(nb) if (kind, L, b) in table]
             if not have:
                 continue
             for b in range(nb):
-                if (sqr, L, b) not in table:
+                if (kind, L, b) not in table:
                     near = min(have, key=lambda h: (abs(h - b), h))
-                    table[(sqr, L, b)] = table[(sqr, L, near)]
+                    table[(kind, L, b)] = table[(kind, L, near)]
     return table
 
 
 def emit_plan_header(table, path: str = PLAN_HEADER) -> None:
     nb = int(lib().sba_plan_bucket(1 << 30)) + 1
-    out = ["// plan_table.h -- the shipped plan table: per (square?, L, job-count",
+    out = ["// plan_table.h -- the shipped plan table: per (kind, L, job-count",
            "// bucket) the pass plan {last, bm, split} with the lowest mean slowdown",
            "// over the shapes of that key in the exhaustive MI355X (gfx950) sweep",
-           "// tuning_data/mi355x_plan_timings.json; keys the sweep did not reach take",
+           "// tuning_data/mi355x_plan_timings.json; kind = square? | radix-3? << 1",
+           "// (radix-3: L is the row length's log); keys the sweep did not reach take",
            "// the nearest measured bucket. {0, 0, 0}: not measured (the hand-written",
            "// rule applies). Generated by: python -m simd_bigint_amd.tuning table",
            "#pragma once",
            "#define SBA_PT_NL 26",
-           "static const signed char kPlanTable[2][SBA_PT_NL][SBA_PLAN_BUCKETS][3] = {"]
-    for sqr in (0, 1):
-        out.append("  {  // %s" % ("squares" if sqr else "multiplies"))
+           "static const signed char kPlanTable[4][SBA_PT_NL][SBA_PLAN_BUCKETS][3] = {"]
+    names = ("multiplies", "squares", "radix-3 multiplies", "radix-3 squares")
+    for kind in range(4):
+        out.append("  {  // %s" % names[kind])
         for L in range(26):
             ents = []
             for b in range(nb):
-                p = table.get((sqr, L, b), (0, 0, 0))
+                p = table.get((kind, L, b), (0, 0, 0))
                 ents.append("{%d, %d, %d}" % p)
             out.append("    {%s},  // L = %d" % (", ".join(ents), L))
         out.append("  },")
@@ -481,9 +525,13 @@ def main(argv=None) -> int:
         shapes = default_shapes()
         log("collect: %d shapes" % len(shapes))
         rows = collect_timings(shapes, reps=a.reps, log=log)
+        shapes3 = default_shapes3()
+        log("collect: %d radix-3 shapes" % len(shapes3))
+        rows3 = collect_timings(shapes3, reps=a.reps, log=log, r3=True)
         save_timings(rows, a.timings, {"device": torch.cuda.get_device_name(0),
-                                       "reps": a.reps, "shapes": len(shapes)})
-        log("collect: %d rows -> %s" % (len(rows), a.timings))
+                                       "reps": a.reps, "shapes": len(shapes),
+                                       "shapes3": len(shapes3)}, rows3=rows3)
+        log("collect: %d + %d radix-3 rows -> %s" % (len(rows), len(rows3), a.timings))
     elif a.cmd == "train":
         rows = load_timings(a.timings)
         tr, te = split_shapes(rows)
@@ -495,7 +543,7 @@ def main(argv=None) -> int:
         cm.save(a.model, {"train_rows": len(tr), "steps": a.steps, "lr": a.lr,
                           "split": "split_shapes(held_frac=0.3, seed=3)"})
     elif a.cmd == "table":
-        emit_plan_header(shipped_table(load_timings(a.timings)))
+        emit_plan_header(shipped_table(load_timings(a.timings), load_timings3(a.timings)))
         log("table: wrote %s" % PLAN_HEADER)
     else:
         rows = load_timings(a.timings)

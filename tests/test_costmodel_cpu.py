@@ -162,9 +162,12 @@ def test_committed_model_beats_heuristic_on_held_out_shapes():
 
 def test_shipped_plan_table_matches_dataset():
     """csrc/plan_table.h is what the generator makes of the committed
-    dataset, and the library serves it after a reset."""
+    dataset (power-of-two and radix-3 classes), and the library serves it
+    after a reset."""
     rows, _ = _committed()
-    table = tuning.shipped_table(rows)
+    rows3 = tuning.load_timings3()
+    table = tuning.shipped_table(rows, rows3)
+    assert any(kind & 2 for kind, _, _ in table) == bool(rows3)
     tuning.plan_reset()
-    for (sqr, L, b), plan in table.items():
-        assert tuning.plan_get(L, sqr, b) == plan
+    for (kind, L, b), plan in table.items():
+        assert tuning.plan_get(L, kind, b) == plan, (kind, L, b)
