@@ -13,6 +13,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <unistd.h>
 
 typedef unsigned __int128 u128;
 
@@ -117,38 +118,93 @@ static int grow_pinned(uint64_t **p, size_t *cap, size_t n) {
 #define COPY_THREADS 8
 #define COPY_MIN_PER_THREAD ((size_t)1 << 19)     /* bytes */
 struct copy_job { void *dst; const void *src; size_t n; };
-static void *copy_worker(void *arg) {
-  struct copy_job *j = (struct copy_job *)arg;
-  memcpy(j->dst, j->src, j->n);
+
+/* Copy threads: COPY_THREADS - 1 workers started by the first parallel copy
+ * and kept for the life of the process, so a staged copy costs a wake-up
+ * per piece instead of thread creation. One parallel copy runs at a time
+ * (g_pool_mu): it posts one slice per worker under a new generation number,
+ * copies the last slice itself and waits until every worker has finished
+ * that generation. A forked child (another pid) copies on one thread. */
+static pthread_mutex_t g_pool_mu = PTHREAD_MUTEX_INITIALIZER;  /* one copy at a time */
+static pthread_mutex_t g_pool_lk = PTHREAD_MUTEX_INITIALIZER;  /* guards the fields below */
+static pthread_cond_t g_pool_go = PTHREAD_COND_INITIALIZER;
+static pthread_cond_t g_pool_done = PTHREAD_COND_INITIALIZER;
+static struct copy_job g_pool_job[COPY_THREADS];
+static unsigned g_pool_gen;      /* generation of the posted slices */
+static int g_pool_pending;       /* workers still on the current generation */
+static int g_pool_n;             /* workers running */
+static unsigned g_pool_gen0[COPY_THREADS];  /* each worker's generation at its start */
+static atomic_int g_pool_pid;    /* the process that started them (0: none yet) */
+
+static void *pool_worker(void *arg) {
+  const int id = (int)(intptr_t)arg;
+  pthread_mutex_lock(&g_pool_lk);
+  unsigned seen = g_pool_gen0[id];
+  for (;;) {
+    while (g_pool_gen == seen) pthread_cond_wait(&g_pool_go, &g_pool_lk);
+    seen = g_pool_gen;
+    const struct copy_job j = g_pool_job[id];
+    pthread_mutex_unlock(&g_pool_lk);
+    if (j.n) memcpy(j.dst, j.src, j.n);
+    pthread_mutex_lock(&g_pool_lk);
+    if (--g_pool_pending == 0) pthread_cond_signal(&g_pool_done);
+  }
   return NULL;
 }
-/* memcpy split over up to COPY_THREADS threads */
+
+/* called with g_pool_mu held: the workers of this process, started once */
+static int pool_workers(void) {
+  const int me = (int)getpid();
+  if (atomic_load(&g_pool_pid) == me) return g_pool_n;
+  atomic_store(&g_pool_pid, me);
+  g_pool_n = 0;
+  pthread_attr_t at;
+  if (pthread_attr_init(&at) != 0) return 0;
+  (void)pthread_attr_setdetachstate(&at, PTHREAD_CREATE_DETACHED);
+  for (int t = 0; t < COPY_THREADS - 1; ++t) {
+    pthread_t th;
+    const int id = g_pool_n;
+    pthread_mutex_lock(&g_pool_lk);
+    g_pool_gen0[id] = g_pool_gen;
+    pthread_mutex_unlock(&g_pool_lk);
+    if (pthread_create(&th, &at, pool_worker, (void *)(intptr_t)id) != 0) break;
+    ++g_pool_n;
+  }
+  (void)pthread_attr_destroy(&at);
+  return g_pool_n;
+}
+
+/* memcpy split over the calling thread and up to COPY_THREADS - 1 workers */
 static void par_copy(void *dst, const void *src, size_t bytes) {
   size_t nt = bytes / COPY_MIN_PER_THREAD;
   if (nt > COPY_THREADS) nt = COPY_THREADS;
-  if (nt < 2) {
+  const int pid = atomic_load(&g_pool_pid);
+  if (nt < 2 || (pid && pid != (int)getpid())) {
     memcpy(dst, src, bytes);
     return;
   }
-  pthread_t th[COPY_THREADS];
-  struct copy_job jb[COPY_THREADS];
+  pthread_mutex_lock(&g_pool_mu);
+  const int nw = pool_workers();
+  if (nt > (size_t)nw + 1) nt = (size_t)nw + 1;
   const size_t per = (bytes / nt + 63) & ~(size_t)63;
-  size_t off = 0, started = 0;
-  for (size_t t = 0; t < nt && off < bytes; ++t) {
-    const size_t n = bytes - off < per ? bytes - off : per;
-    jb[t].dst = (char *)dst + off;
-    jb[t].src = (const char *)src + off;
-    jb[t].n = n;
+  size_t off = 0;
+  pthread_mutex_lock(&g_pool_lk);
+  for (int t = 0; t < nw; ++t) {      /* workers beyond nt - 1 get empty slices */
+    const size_t n = (size_t)t + 1 < nt && off < bytes ? (bytes - off < per ? bytes - off : per) : 0;
+    g_pool_job[t].dst = (char *)dst + off;
+    g_pool_job[t].src = (const char *)src + off;
+    g_pool_job[t].n = n;
     off += n;
-    if (t + 1 == nt || off >= bytes) {
-      copy_worker(&jb[t]);              /* the calling thread takes the last piece */
-    } else if (pthread_create(&th[started], NULL, copy_worker, &jb[t]) == 0) {
-      ++started;
-    } else {
-      copy_worker(&jb[t]);
-    }
   }
-  for (size_t t = 0; t < started; ++t) pthread_join(th[t], NULL);
+  g_pool_pending = nw;
+  ++g_pool_gen;
+  pthread_cond_broadcast(&g_pool_go);
+  pthread_mutex_unlock(&g_pool_lk);
+  if (off < bytes) memcpy((char *)dst + off, (const char *)src + off, bytes - off);
+  pthread_mutex_lock(&g_pool_lk);
+  while (g_pool_pending > 0) pthread_cond_wait(&g_pool_done, &g_pool_lk);
+  pthread_mutex_unlock(&g_pool_lk);
+  pthread_mutex_unlock(&g_pool_mu);
 }
 
 void sba_par_copy(void *dst, const void *src, size_t bytes) { par_copy(dst, src, bytes); }

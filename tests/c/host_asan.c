@@ -5,6 +5,7 @@
  * before any device is touched (simd_mpn_mul and the division finish on
  * the CPU tier; nt_* / p30_* / p50_* return 0). Exits 0 when every check
  * passed; ASan aborts on any invalid access or leak it sees. */
+#include <pthread.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -38,6 +39,32 @@ static void check(const char *what, size_t an, size_t bn, int ok) {
     printf("FAILED %s %zu x %zu\n", what, an, bn);
     g_fail = 1;
   }
+}
+
+/* the library's parallel host copy (exported for tests; the staged
+ * transfers of the host entries use it) */
+void sba_par_copy(void *dst, const void *src, size_t bytes);
+static void check_copy(size_t bytes) {
+  unsigned char *s = malloc(bytes + 1), *d = malloc(bytes + 1);
+  for (size_t i = 0; i < bytes; ++i) s[i] = (unsigned char)xr();
+  d[bytes] = 0xa5;
+  sba_par_copy(d, s, bytes);
+  check("par_copy", bytes, 0, !memcmp(d, s, bytes) && d[bytes] == 0xa5);
+  free(s);
+  free(d);
+}
+static void *copy_thread(void *arg) {
+  (void)arg;
+  for (int k = 0; k < 4; ++k) {
+    const size_t bytes = ((size_t)(2 + k) << 20) + 9;
+    unsigned char *s = malloc(bytes), *d = malloc(bytes);
+    for (size_t i = 0; i < bytes; ++i) s[i] = (unsigned char)(i * 131 + k);
+    sba_par_copy(d, s, bytes);
+    if (memcmp(d, s, bytes)) check("par_copy from threads", bytes, 0, 0);
+    free(s);
+    free(d);
+  }
+  return NULL;
 }
 
 int main(void) {
@@ -101,6 +128,15 @@ int main(void) {
     check("divrem_u64 fallback", an + bn, an, ok);
     free(N); free(q); free(rr);
     free(a); free(b); free(r); free(w);
+  }
+  /* the staged copies' thread pool: one caller, then three at once */
+  {
+    static const size_t sizes[] = {0, 1, 4096, (size_t)1 << 19, ((size_t)1 << 20) + 3,
+                                   ((size_t)8 << 20) + 5, ((size_t)24 << 20) + 7};
+    for (size_t k = 0; k < sizeof sizes / sizeof sizes[0]; ++k) check_copy(sizes[k]);
+    pthread_t th[3];
+    for (int t = 0; t < 3; ++t) pthread_create(&th[t], NULL, copy_thread, NULL);
+    for (int t = 0; t < 3; ++t) pthread_join(th[t], NULL);
   }
   printf(g_fail ? "host asan test: FAILED\n" : "host asan test: all checks passed\n");
   return g_fail;

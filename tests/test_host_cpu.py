@@ -130,3 +130,37 @@ def test_nt_band_matches_reference_trunk_counting():
     # the device band contains it
     assert L.sba_gpu_in_band(46137344, 1) == 1
     assert L.sba_gpu_in_band(22020096, 22020096) == 1
+
+
+def test_staging_copy_pool_in_a_forked_child():
+    """The host entries' staged transfers copy through a pool of threads
+    started on first use; a forked child (which has none of those threads)
+    copies on its own thread instead of waiting on them."""
+    import ctypes
+    L = sba.lib()
+    f = L.sba_par_copy
+    f.restype = None
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    n = (9 << 20) + 13
+    src = np.frombuffer(np.random.default_rng(3).bytes(n), np.uint8)
+    dst = np.zeros(n, np.uint8)
+    f(dst.ctypes.data, src.ctypes.data, n)           # starts the pool
+    assert (dst == src).all()
+    pid = os.fork()
+    if pid == 0:
+        d2 = np.zeros(n, np.uint8)
+        f(d2.ctypes.data, src.ctypes.data, n)
+        os._exit(0 if (d2 == src).all() else 1)
+    import signal
+    import time
+    t0 = time.time()
+    while True:
+        done, status = os.waitpid(pid, os.WNOHANG)
+        if done:
+            break
+        if time.time() - t0 > 60:
+            os.kill(pid, signal.SIGKILL)
+            os.waitpid(pid, 0)
+            pytest.fail("the forked child's copy did not finish within 60 s")
+        time.sleep(0.01)
+    assert os.WIFEXITED(status) and os.WEXITSTATUS(status) == 0
