@@ -51,12 +51,13 @@ def _expected(jobs):
     return out
 
 
-@pytest.mark.parametrize("L", [10, 11, 12, 13, 14, 15, 17, 20])
+@pytest.mark.parametrize("L", list(range(10, 23)))
 @pytest.mark.parametrize("sqr", [0, 1])
 def test_every_plan_bit_exact(L, sqr):
     torch, dev = _torch()
     g = torch.Generator(device="cpu").manual_seed(100 + L)
-    pool = torch.randint(-2**63, 2**63 - 1, (1 << 20,), dtype=torch.int64, generator=g).to(dev)
+    pool = torch.randint(-2**63, 2**63 - 1, (1 << max(20, L),), dtype=torch.int64,
+                         generator=g).to(dev)
     jobs = tuning.make_class_jobs(L, sqr, 3, 0.95, pool, dev)
     # one ragged job of the same class: a short second operand (or a square
     # filling the transform to its last coefficient)
@@ -81,7 +82,7 @@ def test_every_plan_bit_exact(L, sqr):
             assert np.array_equal(got, want[k]), "plan %r job %d differs" % (plan, k)
 
 
-@pytest.mark.parametrize("L", [13, 14, 16, 19])
+@pytest.mark.parametrize("L", [13, 14, 15, 16, 17, 18, 19, 20])
 @pytest.mark.parametrize("sqr", [0, 1])
 def test_every_radix3_plan_bit_exact(L, sqr):
     """Batch classes of length 3 * 2^L: every radix-3 plan, mixed fills,
