@@ -32,6 +32,12 @@
  * caller workspace add a grow-only workspace of about 40 B per transform
  * point of the largest product seen (2^23 points: 336 MiB).
  *
+ * Host resources. The host-memory entries stage their copies through
+ * grow-only pinned buffers (the operands and the product of the largest
+ * call seen), and the first staged copy of at least 1 MiB starts 7 copy
+ * threads that stay, idle, until the process ends. A forked child copies
+ * on its calling thread.
+ *
  * Citations are files of the reference repository (iamstupid/simd_bigint).
  */
 #ifndef SIMD_BIGINT_AMD_H
