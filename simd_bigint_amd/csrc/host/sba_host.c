@@ -211,7 +211,10 @@ void sba_par_copy(void *dst, const void *src, size_t bytes) { par_copy(dst, src,
 
 /* Staged copies in pieces: the host copy of piece k+1 (caller memory <->
  * pinned staging, several threads) overlaps the DMA of piece k. */
-#define STAGE_PIECE ((size_t)8 << 20)            /* bytes */
+#ifndef SBA_STAGE_PIECE_MB
+#define SBA_STAGE_PIECE_MB 8
+#endif
+#define STAGE_PIECE ((size_t)SBA_STAGE_PIECE_MB << 20)   /* bytes */
 #define STAGE_EVENTS 64
 static hipEvent_t g_stage_ev[STAGE_EVENTS];
 
