@@ -142,6 +142,30 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
   return (b & 7) * (n >> 3) + (b >> 3);
 }
 
+// blockIdx -> (tile, prime) of a first pass with one prime per block (grid
+// x = n tiles, y = per primes of the operand). With G > 0 each XCD takes
+// its tiles in groups of G neighbours and runs every prime of a group before
+// the next group: the primes' re-reads of the group's limbs then come from
+// that XCD's L2, and runs of G neighbouring tiles still merge their row
+// segments' writes there. G = 0: prime-major order (xcd_tile per prime).
+#ifndef SBA_FIRST_GROUP
+#define SBA_FIRST_GROUP 32
+#endif
+__device__ __forceinline__ void first_block(uint32_t x, uint32_t y, uint32_t n, uint32_t per,
+                                            uint32_t &tile, uint32_t &q) {
+  constexpr uint32_t G = SBA_FIRST_GROUP;
+  if (G == 0 || per < 2 || n % (8 * G) != 0) {
+    tile = xcd_tile(x, n);
+    q = y;
+    return;
+  }
+  const uint32_t l = x + n * y;             // dispatch order: x fastest
+  const uint32_t s = l >> 3;                // the block's slot on XCD l & 7
+  const uint32_t grp = s / (G * per), r = s - grp * (G * per);
+  q = r / G;
+  tile = (l & 7) * (n >> 3) + grp * G + (r - q * G);
+}
+
 
 // 1: uniform twiddles come through scalar loads
 #ifndef SBA_TW_SCALAR
@@ -638,8 +662,9 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
   const uint32_t N = 1u << L;             // N <= 2^25: 32-bit offsets throughout
   const uint32_t S = 1u << (L - ELL);
   const bool split = os.split;
-  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-  const uint32_t q0 = os.q0;
+  uint32_t tile, q0;
+  first_block(blockIdx.x, os.q0, gridDim.x, split ? (which == 2 ? gridDim.y >> 1 : gridDim.y) : 1u,
+              tile, q0);
   const uint32_t col0 = tile << LOGTC;
   const uint32_t nl = nlimbs < N ? nlimbs : N;
   uint32_t *lt = lds + lpadt<LOGTC>(threadIdx.x);
