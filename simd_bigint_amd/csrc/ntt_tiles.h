@@ -151,9 +151,14 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
 #ifndef SBA_FIRST_GROUP
 #define SBA_FIRST_GROUP 32
 #endif
+// the radix-3 first pass reads three rows per tile: a group of 32 tiles
+// holds 6 MiB of limbs, more than an XCD's L2, and measured no faster
+#ifndef SBA_R3_FIRST_GROUP
+#define SBA_R3_FIRST_GROUP 0
+#endif
+template <uint32_t G = SBA_FIRST_GROUP>
 __device__ __forceinline__ void first_block(uint32_t x, uint32_t y, uint32_t n, uint32_t per,
                                             uint32_t &tile, uint32_t &q) {
-  constexpr uint32_t G = SBA_FIRST_GROUP;
   if (G == 0 || per < 2 || n % (8 * G) != 0) {
     tile = xcd_tile(x, n);
     q = y;
@@ -1180,13 +1185,14 @@ __global__ __launch_bounds__(NT, SBA_R3_FIRST_WAVES) void k_first_fwd3(PrimeArgs
   const uint32_t M = 1u << L;
   const uint32_t S = 1u << (L - ELL);
   const uint32_t mu = (L & 1) ? 2u : 1u;
-  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const bool split = os.split;
+  uint32_t tile, q0;
+  first_block<SBA_R3_FIRST_GROUP>(blockIdx.x, os.q0, gridDim.x,
+                                  split ? (which == 2 ? gridDim.y >> 1 : gridDim.y) : 1u, tile, q0);
   const uint32_t col0 = tile << LOGTC;
   const uint32_t nl = nlimbs < 3 * M ? nlimbs : 3 * M;
   uint32_t *lt = lds + lpadt<LOGTC>(threadIdx.x);
   const uint32_t toff = (threadIdx.x >> LOGTC) * S + (threadIdx.x & (TC - 1));
-  const bool split = os.split;
-  const uint32_t q0 = os.q0;
   const uint32_t q1 = split ? q0 + 1 : SBA_K;
   // as in k_first_fwd: the rows' level-0 chunks start from the registers
   // and (EDGE) their last chunks go straight to HBM
