@@ -667,9 +667,10 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
   const uint32_t N = 1u << L;             // N <= 2^25: 32-bit offsets throughout
   const uint32_t S = 1u << (L - ELL);
   const bool split = os.split;
+  // grouped order only for the launch that covers both operands: with one
+  // operand per launch (squares) it measured slower (89 -> 99 us at 2^22)
   uint32_t tile, q0;
-  first_block(blockIdx.x, os.q0, gridDim.x, split ? (which == 2 ? gridDim.y >> 1 : gridDim.y) : 1u,
-              tile, q0);
+  first_block(blockIdx.x, os.q0, gridDim.x, split && which == 2 ? gridDim.y >> 1 : 1u, tile, q0);
   const uint32_t col0 = tile << LOGTC;
   const uint32_t nl = nlimbs < N ? nlimbs : N;
   uint32_t *lt = lds + lpadt<LOGTC>(threadIdx.x);
