@@ -328,3 +328,27 @@ def test_serving_results_same_alone_and_in_mix():
         a = to_int(q.a.cpu().numpy().view(np.uint64))
         b = a if q.b is None else to_int(q.b.cpu().numpy().view(np.uint64))
         assert to_int(q.r.cpu().numpy().view(np.uint64)) == a * b
+
+
+def test_uneven_products_at_the_large_lengths():
+    """Two-operand products at transform lengths 2^22, 2^23 and 2^24 (the
+    first pass covers both operands in one launch, its blocks taken in
+    groups of neighbouring tiles per XCD), operands of unequal and ragged
+    lengths: one longer and one shorter than half the transform. Against
+    the oracle at 2^22 and 2^23, by fingerprints mod two primes at 2^24."""
+    st = [77]
+    for an, bn, exact in (((1 << 21) + 777, (1 << 21) - 1000, True),
+                          ((5 << 20) + 3, (3 << 20) - 11, True),
+                          ((1 << 23) + 5, (1 << 22) + 3, False)):
+        assert sba.lib().sba_gpu_radix3(an, bn, 0) == 0
+        assert sba.chunking(an, bn)[2] == 0, (an, bn)      # one transform
+        a = xorshift_fill(an, st)
+        b = xorshift_fill(bn, st)
+        say("uneven", an, bn)
+        ok, r = sba.nt_mul(a, b)
+        assert ok == 1
+        if exact:
+            assert (r == orc.nt_mul(a, b)).all(), (an, bn)
+        else:
+            for q in ((1 << 61) - 1, (1 << 59) - 55):
+                assert orc.fingerprint(r, q) == orc.fingerprint(a, q) * orc.fingerprint(b, q) % q
