@@ -667,10 +667,15 @@ __global__ __launch_bounds__(NT, SBA_FIRST_WAVES) void k_first_fwd(PrimeArgs pa,
   const uint32_t N = 1u << L;             // N <= 2^25: 32-bit offsets throughout
   const uint32_t S = 1u << (L - ELL);
   const bool split = os.split;
-  // grouped order only for the launch that covers both operands: with one
-  // operand per launch (squares) it measured slower (89 -> 99 us at 2^22)
+  // grouped order where it measured faster: the launch that covers both
+  // operands, 8-column tiles, and a prime's sweep of at least 4 groups per
+  // XCD (128 tiles, 8 MiB of limbs: twice the L2; 2^23 points). Prime-major
+  // elsewhere: one operand per launch (squares at 2^22: 89 -> 99 us grouped),
+  // 2^22 points (a sweep's limbs already fit the L2: no gain), 4-column
+  // tiles at 2^24 (621 -> 644 us grouped).
+  const bool grouped = split && which == 2 && LOGTC == 3 && gridDim.x >= 32u * SBA_FIRST_GROUP;
   uint32_t tile, q0;
-  first_block(blockIdx.x, os.q0, gridDim.x, split && which == 2 ? gridDim.y >> 1 : 1u, tile, q0);
+  first_block(blockIdx.x, os.q0, gridDim.x, grouped ? gridDim.y >> 1 : 1u, tile, q0);
   const uint32_t col0 = tile << LOGTC;
   const uint32_t nl = nlimbs < N ? nlimbs : N;
   uint32_t *lt = lds + lpadt<LOGTC>(threadIdx.x);
